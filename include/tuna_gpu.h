@@ -1,0 +1,150 @@
+/*
+ * tuna_gpu.h — C ABI of the MI355X (gfx950) k-mer counting engine.
+ *
+ * The reference (vicLeva/tuna v1.4.0) has no FFI or C ABI: it is a
+ * header-only C++ library plus a CLI.  The entry points below replace its two
+ * in-memory phase seams and the public API built on them:
+ *
+ *   partition_kmers_mem<k,m>(cfg, bufs)            src/partition_hash.hpp:556-565
+ *   count_and_write_mem<k,m>(cfg, K, bufs, out, kff) src/count.hpp:875-970
+ *   count_and_callback_mem<k,m>(cfg, K, bufs, cb)  src/count.hpp:311-388
+ *   tuna::count / count_to / count_to_raw          include/tuna/tuna.hpp:91-195
+ *
+ * Conventions: every call returns 0 on success or a negative TG_E* status;
+ * tg_last_error() gives a thread-local message for the last failure.  All
+ * pointers are plain host or device pointers with explicit sizes; no C++ or
+ * torch types cross this boundary.  A context owns one HIP stream, the device
+ * buffers of its jobs (kept resident and reused across calls) and the result
+ * of its last job.  Calls on one context are serialised by the caller; distinct
+ * contexts may be used from distinct threads.
+ *
+ * K-mer keys use the layout of the reference's Kmer<k> (Kmer.hpp:44-48):
+ * ceil(k/32) little-endian 64-bit words, kache 2-bit codes (A0 C1 G2 T3), the
+ * last base in bits 1:0 of word 0, the first base most significant; keys are
+ * canonical (the smaller of the k-mer and its reverse complement).
+ */
+#ifndef TUNA_GPU_H
+#define TUNA_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK 0
+#define TG_EINVAL (-1)   /* bad argument (k, m, sizes, pointers) */
+#define TG_EDEVICE (-2)  /* HIP runtime failure */
+#define TG_ENOMEM (-3)   /* device memory admission failed */
+#define TG_ECAP (-4)     /* caller buffer too small */
+#define TG_EINPUT (-5)   /* input file / parse error */
+
+typedef struct tg_ctx tg_ctx;
+
+/* Counting parameters; names follow the reference Config (src/Config.hpp:10-26). */
+typedef struct {
+    uint32_t k;            /* k-mer length, 2..64 on this engine */
+    uint32_t m;            /* minimizer length, 1..min(32, k-1) */
+    uint32_t n_partitions; /* reference -n: rounded up to a power of 2; 0 -> 16 */
+    uint32_t ci;           /* report counts >= ci (reference -ci) */
+    uint64_t cx;           /* report counts <= cx (reference -cx) */
+} tg_params;
+
+/* Totals of one job; mirror the reference's structured stderr lines
+ * (src/pipeline.hpp:174-178) plus engine-side figures. */
+typedef struct {
+    uint64_t n_bases;      /* ACGT bases handed in */
+    uint64_t n_chunks;     /* ACGT runs */
+    uint64_t n_kmers;      /* k-mers inserted (reference total_inserted) */
+    uint64_t n_superkmers; /* reference "superkmers:" */
+    uint64_t n_unique;     /* distinct canonical k-mers before ci/cx */
+    uint64_t n_out;        /* reference "unique_kmers:" (after ci/cx) */
+    uint32_t n_parts;      /* reference "n_parts:" */
+    uint32_t n_subparts;   /* device sub-partitions (n_parts * s) */
+    uint32_t n_lds_groups; /* LDS-table work groups */
+    uint32_t n_hbm_parts;  /* partitions counted in HBM tables */
+    double ms_phase1;      /* device time: pack + superkmers + routing */
+    double ms_phase2;      /* device time: counting + drain */
+} tg_stats;
+
+/* ---- context -------------------------------------------------------- */
+int tg_ctx_create(int device, tg_ctx** out);
+void tg_ctx_destroy(tg_ctx* ctx);
+const char* tg_last_error(void);
+/* Per-kernel HIP-event timing on the context stream (0 = off). */
+int tg_set_profiling(tg_ctx* ctx, int on);
+/* Number of profiled kernels; fills names (NUL-separated, cap bytes), total ms
+ * and launch counts for up to n entries. */
+int tg_kernel_times(tg_ctx* ctx, char* names, uint64_t cap, double* ms,
+                    uint64_t* launches, int n);
+void tg_reset_kernel_times(tg_ctx* ctx);
+
+/* ---- device memory helpers (bench and tests keep inputs resident) ----- */
+int tg_dev_alloc(tg_ctx* ctx, uint64_t bytes, void** dptr);
+int tg_dev_free(tg_ctx* ctx, void* dptr);
+int tg_h2d(tg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int tg_d2h(tg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int tg_sync(tg_ctx* ctx);
+
+/* ---- whole job ------------------------------------------------------ */
+/* Count the canonical k-mers of ACGT chunks: bases[off[i]..off[i+1]) is chunk
+ * i (only [ACGTacgt] bytes), off has n_chunks+1 entries.  Host pointers; the
+ * data is copied to the device.  Replaces partition_kmers_mem +
+ * count_and_write_mem (src/pipeline.hpp:111-178). */
+int tg_count_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
+                  uint64_t n_bases, const uint64_t* off, uint64_t n_chunks,
+                  tg_stats* st);
+/* Same with bases/off already resident in device memory (d_bases, d_off). */
+int tg_count_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases,
+                    uint64_t n_bases, const uint64_t* d_off, uint64_t n_chunks,
+                    tg_stats* st);
+/* Result of the last job: n_out (key, count) pairs in unspecified order. */
+uint64_t tg_result_size(tg_ctx* ctx);
+/* keys: n*ceil(k/32) words; counts: n.  Copies min(n, cap) pairs. */
+int tg_result_copy(tg_ctx* ctx, uint64_t* keys, uint32_t* counts, uint64_t cap);
+
+/* ---- phase-level entry points (parity tests, tools) ------------------ */
+/* 2-bit pack of ASCII ACGT: word j holds bases 16j..16j+15, first base in
+ * bits 31:30, kache codes (src/partition_hash.hpp:67-70). */
+int tg_pack_host(tg_ctx* ctx, const uint8_t* bases, uint64_t n_bases, uint32_t* out_words);
+/* Phase 1 only: superkmers of every chunk (src/partition_hash.hpp:48-101).
+ * For superkmer i: start = global base index of its first base, len = bases,
+ * minpos = minimizer offset from start, pid = minimizer hash & (n_subparts-1).
+ * Order is unspecified.  *n_sk receives the count (even if > cap). */
+int tg_superkmers_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
+                       uint64_t n_bases, const uint64_t* off, uint64_t n_chunks,
+                       uint64_t* start, uint32_t* len, uint32_t* minpos,
+                       uint32_t* pid, uint64_t cap, uint64_t* n_sk,
+                       uint32_t* n_subparts);
+/* Phase 1 + routing: superkmers grouped by reference partition
+ * (hash & (n_partitions-1)); part_off has n_partitions+1 entries.  Each record
+ * is serialised in the reference byte format [len][min_pos][packed]
+ * (src/superkmer_io.hpp:154-172) into rec_bytes (rec_off per record,
+ * n_sk+1 entries).  Replaces partition_kmers_mem's per-partition buffers. */
+int tg_partitions_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
+                       uint64_t n_bases, const uint64_t* off, uint64_t n_chunks,
+                       uint64_t* part_off, uint8_t* rec_bytes, uint64_t bytes_cap,
+                       uint64_t* rec_off, uint64_t rec_cap, uint64_t* n_sk,
+                       uint64_t* n_bytes);
+
+/* ---- host input path (A1) --------------------------------------------- */
+/* Parse FASTA/FASTQ bytes (format by first byte, '@' = FASTQ) into ACGT
+ * chunks with the reference parser's rules (src/seq_source.hpp:33-39,169-185).
+ * out_bases needs len bytes; out_off n_chunks+1 entries.  Returns chunk count
+ * or a negative status (TG_ECAP when off_cap is short, TG_EINPUT on empty). */
+int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases,
+                        uint64_t* out_off, uint64_t off_cap);
+
+/* ---- synthetic inputs (seeded, deterministic) ------------------------- */
+/* Generate the FASTA/FASTQ text of a named workload into a library-owned
+ * buffer: "genome_fa" (uniform genome), "reads_fq" (short reads from a uniform
+ * genome with substitutions and N), "repeat_reads" (repetitive genome reads).
+ * Parameters are read from the key=value string spec.  Returns the size in
+ * bytes; tg_synth_copy copies it out. */
+int64_t tg_synth(const char* spec);
+int tg_synth_copy(uint8_t* dst, uint64_t cap);
+void tg_synth_free(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

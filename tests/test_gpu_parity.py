@@ -1,0 +1,186 @@
+"""GPU parity tests: the HIP path through the C ABI against the oracle and the
+reference's own golden vectors.  Bit-exact throughout (integer work)."""
+import json
+import os
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tuna_amd as T
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = T.Engine(0)
+    yield e
+    e.close()
+
+
+def rs(rng, n, alph="ACGT"):
+    return "".join(rng.choice(alph) for _ in range(n))
+
+
+def _ilog2(x):
+    return x.bit_length() - 1
+
+
+def _sorted_keys(keys, cnt):
+    if keys.shape[1] == 1:
+        order = np.lexsort((keys[:, 0],))
+    else:
+        order = np.lexsort((keys[:, 0], keys[:, 1]))
+    return keys[order], cnt[order]
+
+
+def check_counts(eng, bases, off, k, m, n=16, ci=1, cx=2**64 - 1, label=""):
+    say(f"count {label} k={k} m={m} n={n} ci={ci} N={len(bases)} chunks={len(off) - 1}")
+    gk, gc = eng.count(bases, off, k, m, n, ci, cx)
+    st = eng.last_stats
+    ok, oc, tot, nu = O.count(bases, off, k, ci, cx)
+    assert st.n_kmers == tot
+    assert st.n_unique == nu
+    assert st.n_out == len(oc) == len(gc)
+    gk, gc = _sorted_keys(gk, gc)
+    ok, oc = _sorted_keys(ok, oc)
+    assert np.array_equal(gk, ok)
+    assert np.array_equal(gc, oc)
+    return st
+
+
+def test_pack_matches_kache_codes(eng):
+    rng = random.Random(1)
+    s = rs(rng, 1000, "ACGTacgt").encode()
+    say("pack 1000")
+    w = eng.pack(s)
+    codes = [O.lib().orc_kache_code(c) for c in s]
+    exp = []
+    for i in range(0, len(codes), 16):
+        v = 0
+        for j in range(16):
+            v = (v << 2) | (codes[i + j] if i + j < len(codes) else 0)
+        exp.append(v)
+    assert list(w) == exp
+
+
+@pytest.mark.parametrize("name", sorted(GOLD["inputs"]))
+def test_partition_records_match_reference(eng, name):
+    text = GOLD["inputs"][name].encode()
+    bases, off = T.parse_buffer(text)
+    for key in sorted(GOLD["records"]):
+        nm, k, m, n = key.split("|")
+        if nm != name:
+            continue
+        k, m, n = int(k), int(m), int(n)
+        g = GOLD["records"][key]
+        say(f"records {name} k={k} m={m} n={n}")
+        parts = eng.partitions(bases, off, k, m, n)
+        for p in range(n):
+            blob = bytes.fromhex(g["parts"][p])
+            ref, q = [], 0
+            while q < len(blob):
+                L = blob[q]
+                ref.append(blob[q:q + 2 + (L + 3) // 4])
+                q += 2 + (L + 3) // 4
+            assert sorted(parts[p]) == sorted(ref), (key, p)
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (31, 23), (21, 15), (63, 21), (11, 5), (5, 3), (9, 8),
+                                 (64, 31), (32, 16), (17, 2), (3, 2), (45, 9), (33, 15), (7, 1)])
+def test_superkmers_match_oracle(eng, k, m):
+    rng = random.Random(k * 100 + m)
+    strings = [rs(rng, rng.randint(1, 400)) for _ in range(300)]
+    strings += [rs(rng, 9000), "A" * 3000 + rs(rng, 50), "CAG" * 1500, rs(rng, 5000, "AC")]
+    bases, off = O.chunks_of(strings)
+    n = 16
+    say(f"superkmers k={k} m={m}")
+    gs, gl, gm, gp, nsp = eng.superkmers(bases, off, k, m, n)
+    os_, ol, om, oh, km = O.superkmers(bases, off, k, m)
+    log2n, log2s = _ilog2(n), _ilog2(nsp) - _ilog2(n)
+    oh = oh.astype(np.uint64)
+    opid = ((oh & np.uint64(n - 1)) << np.uint64(log2s)) | ((oh >> np.uint64(log2n)) & np.uint64((1 << log2s) - 1))
+    a = sorted(zip(gs.tolist(), gl.tolist(), gm.tolist(), gp.tolist()))
+    b = sorted(zip(os_.tolist(), ol.tolist(), om.tolist(), opid.astype(np.uint32).tolist()))
+    assert len(a) == len(b)
+    assert a == b
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (21, 15), (63, 21), (2, 1), (5, 3), (15, 7), (32, 16),
+                                 (33, 15), (45, 9), (64, 31), (25, 11)])
+def test_counts_match_oracle_random_chunks(eng, k, m):
+    rng = random.Random(1000 + k)
+    strings = [rs(rng, rng.randint(1, 300)) for _ in range(500)] + [rs(rng, 20000)]
+    bases, off = O.chunks_of(strings)
+    check_counts(eng, bases, off, k, m, 16, label="random")
+
+
+def test_counts_low_complexity_and_caps(eng):
+    rng = random.Random(7)
+    strings = ["A" * 7000, "AC" * 3000, rs(rng, 3000) + "T" * 2600 + rs(rng, 3000),
+               "CAGT" * 2000, ("ACGGT" * 40 + rs(rng, 5)) * 30]
+    bases, off = O.chunks_of(strings)
+    for (k, m) in [(31, 21), (63, 21), (21, 15)]:
+        check_counts(eng, bases, off, k, m, 4, label="lowcomplexity")
+
+
+def test_counts_high_multiplicity_and_filters(eng):
+    rng = random.Random(8)
+    g = rs(rng, 5000)
+    reads = []
+    for _ in range(3000):
+        p = rng.randint(0, len(g) - 150)
+        reads.append(g[p:p + 150])
+    bases, off = O.chunks_of(reads)
+    check_counts(eng, bases, off, 31, 21, 16, label="coverage")
+    check_counts(eng, bases, off, 31, 21, 16, ci=2, label="ci2")
+    check_counts(eng, bases, off, 31, 21, 16, ci=50, cx=100, label="ci50cx100")
+    check_counts(eng, bases, off, 63, 21, 16, ci=3, label="k63ci3")
+
+
+def test_counts_edge_inputs(eng):
+    # all chunks shorter than k, single chunk of exactly k, empty input
+    for strings, k in [(["ACGT", "ACG", "A"], 5), (["ACGTACGTAC"], 10), ([], 31), (["ACGTACGTACGTA"], 13)]:
+        bases, off = O.chunks_of(strings) if strings else (np.zeros(0, np.uint8), np.zeros(1, np.uint64))
+        check_counts(eng, bases, off, k, min(3, k - 1), 16, label="edge")
+
+
+def test_counts_fastq_synthetic_matches_oracle(eng):
+    text = T.synth("kind=reads_fq genome=200000 seed=2 cov=20 rl=150 sub=0.005 nrate=0.0001")
+    bases, off = T.parse_buffer(text)
+    ob, oo = O.parse(text)
+    assert np.array_equal(bases, ob) and np.array_equal(off, oo)
+    check_counts(eng, bases, off, 31, 21, 256, ci=2, label="reads_fq ci2")
+    check_counts(eng, bases, off, 31, 21, 256, ci=1, label="reads_fq")
+
+
+def test_counts_genome_c1_shape(eng):
+    text = T.synth("kind=genome_fa len=4600000 seed=1")
+    bases, off = T.parse_buffer(text)
+    st = check_counts(eng, bases, off, 31, 21, 16, label="C1 genome")
+    _, _, _, _, km = O.superkmers(bases, off, 31, 21)
+    assert st.n_superkmers == len(O.superkmers(bases, off, 31, 21)[0])
+
+
+def test_repeat_genome_reads(eng):
+    text = T.synth("kind=repeat_fq genome=300000 seed=3 gc=0.45 cov=5 rl=150 sub=0.003")
+    bases, off = T.parse_buffer(text)
+    check_counts(eng, bases, off, 31, 23, 64, label="repeat reads")
+
+
+def test_repeated_calls_reuse_buffers(eng):
+    rng = random.Random(9)
+    for trial in range(3):
+        strings = [rs(rng, rng.randint(50, 2000)) for _ in range(50 - 20 * trial)]
+        bases, off = O.chunks_of(strings)
+        check_counts(eng, bases, off, 31, 21, 16, label=f"reuse{trial}")

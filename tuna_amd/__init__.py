@@ -1,0 +1,353 @@
+"""tuna_amd — MI355X (gfx950) k-mer counting engine, Python bindings.
+
+Thin ctypes layer over the C ABI in include/tuna_gpu.h (libtuna_gpu.so, built
+in-tree by ``make -C tuna_amd/csrc``).  There is no CPU fallback: if the
+library is missing, or no GPU is present, calls raise.
+
+The Python surface mirrors the reference library API (include/tuna/tuna.hpp
+of vicLeva/tuna: ``count``, ``count_to``, ``count_to_raw``, ``Options``)
+for the counting path.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtuna_gpu.so")
+_lib = None
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+# every symbol declared in include/tuna_gpu.h
+EXPORTS = [
+    "tg_ctx_create", "tg_ctx_destroy", "tg_last_error", "tg_set_profiling", "tg_kernel_times",
+    "tg_reset_kernel_times", "tg_dev_alloc", "tg_dev_free", "tg_h2d", "tg_d2h", "tg_sync",
+    "tg_count_host", "tg_count_device", "tg_result_size", "tg_result_copy", "tg_pack_host",
+    "tg_superkmers_host", "tg_partitions_host", "tg_parse_buffer", "tg_synth", "tg_synth_copy",
+    "tg_synth_free",
+]
+
+
+class TgParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("m", ctypes.c_uint32), ("n_partitions", ctypes.c_uint32),
+                ("ci", ctypes.c_uint32), ("cx", ctypes.c_uint64)]
+
+
+class TgStats(ctypes.Structure):
+    _fields_ = [("n_bases", ctypes.c_uint64), ("n_chunks", ctypes.c_uint64),
+                ("n_kmers", ctypes.c_uint64), ("n_superkmers", ctypes.c_uint64),
+                ("n_unique", ctypes.c_uint64), ("n_out", ctypes.c_uint64),
+                ("n_parts", ctypes.c_uint32), ("n_subparts", ctypes.c_uint32),
+                ("n_lds_groups", ctypes.c_uint32), ("n_hbm_parts", ctypes.c_uint32),
+                ("ms_phase1", ctypes.c_double), ("ms_phase2", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TunaGpuError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libtuna_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TunaGpuError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C tuna_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    L.tg_last_error.restype = ctypes.c_char_p
+    L.tg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.tg_ctx_destroy.argtypes = [_vp]
+    L.tg_ctx_destroy.restype = None
+    L.tg_set_profiling.argtypes = [_vp, ctypes.c_int]
+    L.tg_kernel_times.argtypes = [_vp, ctypes.c_char_p, ctypes.c_uint64,
+                                  ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int]
+    L.tg_reset_kernel_times.argtypes = [_vp]
+    L.tg_reset_kernel_times.restype = None
+    L.tg_dev_alloc.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]
+    L.tg_dev_free.argtypes = [_vp, _vp]
+    L.tg_h2d.argtypes = [_vp, _vp, _vp, ctypes.c_uint64]
+    L.tg_d2h.argtypes = [_vp, _vp, _vp, ctypes.c_uint64]
+    L.tg_sync.argtypes = [_vp]
+    L.tg_count_host.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
+                                ctypes.c_uint64, ctypes.POINTER(TgStats)]
+    L.tg_count_device.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
+                                  ctypes.c_uint64, ctypes.POINTER(TgStats)]
+    L.tg_result_size.argtypes = [_vp]
+    L.tg_result_size.restype = ctypes.c_uint64
+    L.tg_result_copy.argtypes = [_vp, _u64p, _u32p, ctypes.c_uint64]
+    L.tg_pack_host.argtypes = [_vp, _u8p, ctypes.c_uint64, _u32p]
+    L.tg_superkmers_host.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
+                                     ctypes.c_uint64, _u64p, _u32p, _u32p, _u32p, ctypes.c_uint64,
+                                     _u64p, _u32p]
+    L.tg_partitions_host.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
+                                     ctypes.c_uint64, _u64p, _u8p, ctypes.c_uint64, _u64p,
+                                     ctypes.c_uint64, _u64p, _u64p]
+    L.tg_parse_buffer.restype = ctypes.c_int64
+    L.tg_parse_buffer.argtypes = [_u8p, ctypes.c_uint64, _u8p, _u64p, ctypes.c_uint64]
+    L.tg_synth.restype = ctypes.c_int64
+    L.tg_synth.argtypes = [ctypes.c_char_p]
+    L.tg_synth_copy.argtypes = [_u8p, ctypes.c_uint64]
+    L.tg_synth_free.restype = None
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc != 0:
+        raise TunaGpuError(f"tuna_gpu error {rc}: {lib().tg_last_error().decode()}")
+
+
+def _u8(x):
+    if isinstance(x, (bytes, bytearray)):
+        return np.frombuffer(bytes(x), dtype=np.uint8)
+    if isinstance(x, str):
+        return np.frombuffer(x.encode(), dtype=np.uint8)
+    return np.ascontiguousarray(x, dtype=np.uint8)
+
+
+# ------------------------------------------------------------------ host ----
+def parse_buffer(data):
+    """FASTA/FASTQ bytes -> (bases uint8, chunk offsets uint64) on the host."""
+    d = _u8(data)
+    out = np.zeros(max(len(d), 1), dtype=np.uint8)
+    cap = len(d) // 2 + 2
+    off = np.zeros(cap, dtype=np.uint64)
+    n = lib().tg_parse_buffer(_p(d, _u8p), len(d), _p(out, _u8p), _p(off, _u64p), cap)
+    if n < 0:
+        raise TunaGpuError("Empty file" if n == -5 else f"parse failed ({n})")
+    off = off[: n + 1].copy()
+    return out[: int(off[-1])].copy(), off
+
+
+def parse_file(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    if not data:
+        raise TunaGpuError(f"Empty file: {path}")
+    return parse_buffer(data)
+
+
+def synth(spec):
+    """Seeded synthetic FASTA/FASTQ text (see tg_synth in include/tuna_gpu.h)."""
+    L = lib()
+    n = L.tg_synth(spec.encode())
+    if n < 0:
+        raise TunaGpuError(f"bad synth spec {spec!r}")
+    buf = np.empty(n, dtype=np.uint8)
+    _check(L.tg_synth_copy(_p(buf, _u8p), n))
+    L.tg_synth_free()
+    return buf
+
+
+# --------------------------------------------------------------- options ----
+@dataclass
+class Options:
+    """Mirror of tuna::Options (include/tuna/tuna.hpp:59-66)."""
+    threads: int = 1
+    min_count: int = 1
+    max_count: int = 2**32 - 1
+    partitions: int = 0
+    ram_gb: int = 0
+    work_dir: str = ""
+
+
+def _params(k, m, partitions=16, ci=1, cx=2**64 - 1):
+    return TgParams(k=k, m=m, n_partitions=partitions or 16, ci=ci, cx=min(cx, 2**64 - 1))
+
+
+# ---------------------------------------------------------------- engine ----
+class Engine:
+    """One device context: stream, resident buffers, last result."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = _vp()
+        _check(L.tg_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.last_stats = None
+
+    def close(self):
+        if self._h:
+            lib().tg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- profiling
+    def set_profiling(self, on=True):
+        _check(lib().tg_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        names = ctypes.create_string_buffer(8192)
+        ms = (ctypes.c_double * 64)()
+        ln = (ctypes.c_uint64 * 64)()
+        n = lib().tg_kernel_times(self._h, names, 8192, ms, ln, 64)
+        keys = names.raw.split(b"\0")[:n]
+        return {keys[i].decode(): (ms[i], ln[i]) for i in range(min(n, 64))}
+
+    def reset_kernel_times(self):
+        lib().tg_reset_kernel_times(self._h)
+
+    # -- device memory
+    def dev_upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = _vp()
+        _check(lib().tg_dev_alloc(self._h, max(arr.nbytes, 1), ctypes.byref(p)))
+        if arr.nbytes:
+            _check(lib().tg_h2d(self._h, p, arr.ctypes.data_as(_vp), arr.nbytes))
+        return p
+
+    def dev_free(self, p):
+        _check(lib().tg_dev_free(self._h, p))
+
+    def sync(self):
+        _check(lib().tg_sync(self._h))
+
+    # -- counting
+    def count(self, bases, off, k, m=21, partitions=16, ci=1, cx=2**64 - 1, fetch=True):
+        bases = _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        p = _params(k, m, partitions, ci, cx)
+        st = TgStats()
+        _check(lib().tg_count_host(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
+                                   _p(off, _u64p), len(off) - 1, ctypes.byref(st)))
+        self.last_stats = st
+        return self.result(k) if fetch else st
+
+    def count_device(self, d_bases, n_bases, d_off, n_chunks, k, m=21, partitions=16, ci=1,
+                     cx=2**64 - 1):
+        p = _params(k, m, partitions, ci, cx)
+        st = TgStats()
+        _check(lib().tg_count_device(self._h, ctypes.byref(p), d_bases, n_bases, d_off,
+                                     n_chunks, ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def result(self, k):
+        nw = 2 if k > 32 else 1
+        n = lib().tg_result_size(self._h)
+        keys = np.zeros(max(n, 1) * nw, dtype=np.uint64)
+        cnt = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(lib().tg_result_copy(self._h, _p(keys, _u64p), _p(cnt, _u32p), n))
+        return keys[: n * nw].reshape(n, nw), cnt[:n]
+
+    def pack(self, bases):
+        bases = _u8(bases)
+        out = np.zeros((len(bases) + 15) // 16 + 1, dtype=np.uint32)
+        _check(lib().tg_pack_host(self._h, _p(bases, _u8p), len(bases), _p(out, _u32p)))
+        return out[: (len(bases) + 15) // 16]
+
+    def superkmers(self, bases, off, k, m, partitions=16):
+        bases = _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        p = _params(k, m, partitions)
+        cap = max(int(off[-1]) if len(off) > 1 else 0, 1)
+        st = np.zeros(cap, np.uint64)
+        ln = np.zeros(cap, np.uint32)
+        mp = np.zeros(cap, np.uint32)
+        pid = np.zeros(cap, np.uint32)
+        nsk = ctypes.c_uint64(0)
+        nsp = ctypes.c_uint32(0)
+        _check(lib().tg_superkmers_host(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
+                                        _p(off, _u64p), len(off) - 1, _p(st, _u64p), _p(ln, _u32p),
+                                        _p(mp, _u32p), _p(pid, _u32p), cap, ctypes.byref(nsk),
+                                        ctypes.byref(nsp)))
+        n = nsk.value
+        return st[:n], ln[:n], mp[:n], pid[:n], nsp.value
+
+    def partitions(self, bases, off, k, m, partitions=16):
+        """Records per reference partition, serialised in the reference format."""
+        bases = _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        p = _params(k, m, partitions)
+        n = partitions
+        K = int(np.maximum(np.diff(off.astype(np.int64)) - k + 1, 0).sum()) if len(off) > 1 else 0
+        bcap = 4 * len(bases) + 16 * K + 64
+        rcap = K + 2
+        part_off = np.zeros(n + 1, np.uint64)
+        rb = np.zeros(bcap, np.uint8)
+        ro = np.zeros(rcap, np.uint64)
+        nsk = ctypes.c_uint64(0)
+        nby = ctypes.c_uint64(0)
+        _check(lib().tg_partitions_host(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
+                                        _p(off, _u64p), len(off) - 1, _p(part_off, _u64p),
+                                        _p(rb, _u8p), bcap, _p(ro, _u64p), rcap,
+                                        ctypes.byref(nsk), ctypes.byref(nby)))
+        parts = []
+        for P in range(n):
+            recs = []
+            for i in range(int(part_off[P]), int(part_off[P + 1])):
+                recs.append(bytes(rb[int(ro[i]):int(ro[i + 1])]))
+            parts.append(recs)
+        return parts
+
+
+_default = None
+
+
+def default_engine():
+    global _default
+    if _default is None:
+        _default = Engine(0)
+    return _default
+
+
+def kmer_label(key, k):
+    """Uppercase ACGT label of a key (first base most significant)."""
+    return "".join("ACGT"[(key >> (2 * (k - 1 - i))) & 3] for i in range(k))
+
+
+def _read_inputs(files):
+    if not files:
+        raise ValueError("tuna::count: no input files")
+    bs, offs, base = [], [], 0
+    for f in files:
+        b, o = parse_file(f)
+        bs.append(b)
+        offs.append(o[1:] + base if base else o[1:])
+        base += len(b)
+    bases = np.concatenate(bs) if bs else np.zeros(0, np.uint8)
+    off = np.concatenate([np.zeros(1, np.uint64)] + offs)
+    return bases, off
+
+
+def count(k, m, files, cb, opts=None):
+    """tuna::count<k,m>(files, cb, opts): cb(label, count) per counted k-mer."""
+    opts = opts or Options()
+    bases, off = _read_inputs(files)
+    keys, cnt = default_engine().count(bases, off, k, m, opts.partitions or 16,
+                                       opts.min_count, opts.max_count)
+    for i in range(len(cnt)):
+        key = int(keys[i, 0]) | (int(keys[i, 1]) << 64 if keys.shape[1] > 1 else 0)
+        cb(kmer_label(key, k), int(cnt[i]))
+
+
+def count_to(k, m, files, opts=None):
+    """tuna::count_to<k,m>(files, opts) -> {label: count}."""
+    out = {}
+    count(k, m, files, lambda s, c: out.__setitem__(s, c), opts)
+    return out
+
+
+def count_to_raw(k, m, files, opts=None):
+    """tuna::count_to_raw<k,m>(files, opts) -> (keys [n, ceil(k/32)] uint64, counts)."""
+    opts = opts or Options()
+    bases, off = _read_inputs(files)
+    return default_engine().count(bases, off, k, m, opts.partitions or 16, opts.min_count,
+                                  opts.max_count)

@@ -1,0 +1,783 @@
+// engine.hip — host orchestration of the gfx950 counting path and its C ABI.
+//
+// One job = Phase 1 (pack, strips, superkmers, routing) + Phase 2 (counting
+// work items, spill drain).  It replaces the in-memory pipeline of
+// src/pipeline.hpp:111-178 (partition_kmers_mem + count_and_write_mem).
+// Device buffers live in the context and are reused across jobs.
+#include "../../include/tuna_gpu.h"
+#include "tg_kernels.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPOK(expr)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(TG_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+uint32_t round_pow2(uint32_t n)
+{
+    // src/setup.hpp:53-59
+    if (n <= 1) return 1;
+    if ((n & (n - 1)) == 0) return n;
+    while (n & (n - 1)) n &= n - 1;
+    return n << 1;
+}
+
+uint32_t ilog2(uint64_t x)
+{
+    uint32_t r = 0;
+    while ((1ull << r) < x) ++r;
+    return r;
+}
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct KTime {
+    double ms = 0;
+    uint64_t n = 0;
+};
+
+}  // namespace
+
+struct tg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::string, Buf> bufs;
+    bool profiling = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, KTime> ktimes;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+    // last job
+    uint64_t n_out = 0;
+    uint32_t nw = 1;
+    uint64_t* d_keys = nullptr;
+    uint32_t* d_cnt = nullptr;
+};
+
+namespace {
+
+int ensure(tg_ctx* c, const char* name, size_t bytes, void** out)
+{
+    Buf& b = c->bufs[name];
+    if (bytes == 0) bytes = 256;
+    if (b.cap < bytes) {
+        if (b.p) HIPOK(hipFree(b.p));
+        b.p = nullptr;
+        size_t want = bytes + bytes / 4;
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            want = bytes;
+            e = hipMalloc(&b.p, want);
+        }
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            b.cap = 0;
+            (void)hipGetLastError();
+            return fail(TG_ENOMEM, "device allocation of %zu bytes for %s failed", bytes, name);
+        }
+        b.cap = want;
+    }
+    *out = b.p;
+    return TG_OK;
+}
+
+template <class T>
+int ensure_t(tg_ctx* c, const char* name, size_t count, T** out)
+{
+    void* p = nullptr;
+    const int rc = ensure(c, name, count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return rc;
+}
+
+hipEvent_t take_event(tg_ctx* c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Kernel timing on the context stream: events bracket each launch.
+struct Prof {
+    tg_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr;
+    Prof(tg_ctx* cc, const char* n) : c(cc), name(n)
+    {
+        if (c->profiling) {
+            a = take_event(c);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~Prof()
+    {
+        if (c->profiling) {
+            hipEvent_t b = take_event(c);
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({name, {a, b}});
+        }
+    }
+};
+
+void collect_prof(tg_ctx* c)
+{
+    if (c->pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& pe : c->pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
+        KTime& kt = c->ktimes[pe.first];
+        kt.ms += ms;
+        kt.n += 1;
+        c->ev_pool.push_back(pe.second.first);
+        c->ev_pool.push_back(pe.second.second);
+    }
+    c->pending.clear();
+}
+
+int check_params(const tg_params* p, uint64_t n_bases, uint64_t n_chunks)
+{
+    if (!p) return fail(TG_EINVAL, "null parameters");
+    if (p->k < 2 || p->k > 64)
+        return fail(TG_EINVAL, "k-mer length (-k %u) must be in [2, 64] on the GPU engine", p->k);
+    if (p->m < 1 || p->m > 32)
+        return fail(TG_EINVAL, "minimizer length (-m %u) must be in [1, 32]", p->m);
+    if (p->m >= p->k)
+        return fail(TG_EINVAL,
+                    "minimizer length (-m %u) must be strictly less than k-mer length (-k %u)",
+                    p->m, p->k);
+    if (n_bases >= (1ull << 32) - 64)
+        return fail(TG_EINVAL, "input of %llu bases exceeds the 2^32 per-job limit",
+                    (unsigned long long)n_bases);
+    if (n_chunks >= (1ull << 32) - 1)
+        return fail(TG_EINVAL, "too many chunks (%llu)", (unsigned long long)n_chunks);
+    return TG_OK;
+}
+
+// ---------------------------------------------------------------------------
+struct Phase1 {
+    uint32_t k = 0, m = 0, W = 0, S = 0;
+    uint32_t n = 0, log2n = 0, nsp = 0, log2s = 0, sb = 0;
+    uint64_t n_strips = 0, n_kmers = 0, n_sk = 0;
+    uint32_t* packed = nullptr;
+    uint2* rec = nullptr;       // strip-slot space
+    uint32_t* nsk = nullptr;
+    uint64_t* strip_slot = nullptr;
+    uint2* sorted = nullptr;    // partition order
+    uint64_t* part_off = nullptr;
+    uint64_t* part_km = nullptr;
+    uint32_t ntiles = 0;
+    uint64_t n_flagged = 0;
+};
+
+int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N,
+               const uint64_t* d_off, uint64_t nch, Phase1& o)
+{
+    hipStream_t st = c->stream;
+    o.k = p->k;
+    o.m = p->m;
+    o.W = p->k - p->m + 1;
+    o.n = round_pow2(p->n_partitions ? p->n_partitions : 16u);
+    if (o.n > (1u << 14)) return fail(TG_EINVAL, "n_partitions %u exceeds 16384 on the GPU engine", o.n);
+    o.log2n = ilog2(o.n);
+    // device sub-partitions: ~16K bases each, at most 2^14 (LDS histograms)
+    uint64_t want = 1;
+    while (want < N / 16384 && want < (1u << 14)) want <<= 1;
+    if (want < o.n) want = o.n;
+    o.nsp = (uint32_t)want;
+    o.log2s = ilog2(o.nsp) - o.log2n;
+    o.sb = tg::HSEL_BITS - ilog2(o.nsp);
+    // strip length: short chunks are one strip; long chunks are cut so that
+    // the look-back (256-k k-mers) stays a small fraction
+    o.S = 2048;
+
+    // -- pack
+    const uint64_t n_words = (N + 15) / 16 + 8;
+    int rc = ensure_t(c, "packed", n_words, &o.packed);
+    if (rc) return rc;
+    {
+        Prof pr(c, "pack");
+        tg::launch_pack(st, d_bases, N, o.packed, n_words);
+    }
+    // -- strips
+    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp;
+    if ((rc = ensure_t(c, "nstr", nch + 1, &nstr)) || (rc = ensure_t(c, "nkm", nch + 1, &nkm)) ||
+        (rc = ensure_t(c, "nstr_x", nch + 1, &nstr_x)) || (rc = ensure_t(c, "nkm_x", nch + 1, &nkm_x)) ||
+        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)))
+        return rc;
+    {
+        Prof pr(c, "strips");
+        tg::launch_chunk_counts(st, d_off, nch, o.k, o.S, nstr, nkm);
+        tg::launch_scan_u64(st, nstr, nch, nstr_x, tmp);
+        tg::launch_scan_u64(st, nkm, nch, nkm_x, tmp);
+    }
+    uint64_t tot[2];
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(&tot[0], nstr_x + nch, 8, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(&tot[1], nkm_x + nch, 8, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    o.n_strips = tot[0];
+    o.n_kmers = tot[1];
+    if (o.n_strips >= (1ull << 32)) return fail(TG_EINVAL, "too many strips");
+    uint32_t *strip_chunk, *strip_tb, *flag_list, *flag_count;
+    const uint64_t ns1 = o.n_strips + 1;
+    if ((rc = ensure_t(c, "strip_chunk", ns1, &strip_chunk)) ||
+        (rc = ensure_t(c, "strip_tb", ns1, &strip_tb)) ||
+        (rc = ensure_t(c, "strip_slot", ns1, &o.strip_slot)) ||
+        (rc = ensure_t(c, "nsk", ns1, &o.nsk)) ||
+        (rc = ensure_t(c, "flag_list", ns1, &flag_list)) ||
+        (rc = ensure_t(c, "flag_count", 64, &flag_count)) ||
+        (rc = ensure_t(c, "rec", o.n_kmers + 1, &o.rec)))
+        return rc;
+    {
+        Prof pr(c, "strips");
+        tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, o.strip_slot);
+        tg::launch_fill_u32(st, flag_count, 0u, 1);
+    }
+    tg::SkArgs a{};
+    a.packed = o.packed;
+    a.off = d_off;
+    a.strip_chunk = strip_chunk;
+    a.strip_tb = strip_tb;
+    a.strip_slot = o.strip_slot;
+    a.list = nullptr;
+    a.n = (uint32_t)o.n_strips;
+    a.k = o.k;
+    a.m = o.m;
+    a.W = o.W;
+    a.S = o.S;
+    a.capk = 256u - o.k;
+    a.lookback = a.capk + 1u;
+    a.log2n = o.log2n;
+    a.log2s = o.log2s;
+    a.sb = o.sb;
+    a.rec = o.rec;
+    a.nsk = o.nsk;
+    a.flag_list = flag_list;
+    a.flag_count = flag_count;
+    {
+        Prof pr(c, "superkmers");
+        tg::launch_superkmers(st, a);
+    }
+    HIPOK(hipGetLastError());
+    uint32_t nflag = 0;
+    HIPOK(hipMemcpyAsync(&nflag, flag_count, 4, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    o.n_flagged = nflag;
+    if (nflag) {
+        // strips whose first run began before the look-back: redo them from
+        // the chunk start (only long low-complexity runs in long chunks)
+        uint32_t* list2;
+        if ((rc = ensure_t(c, "flag_list2", nflag + 1, &list2))) return rc;
+        HIPOK(hipMemcpyAsync(list2, flag_list, nflag * 4ull, hipMemcpyDeviceToDevice, st));
+        a.list = list2;
+        a.n = nflag;
+        a.lookback = tg::LOOKBACK_FULL;
+        Prof pr(c, "superkmers_fixup");
+        tg::launch_superkmers(st, a);
+    }
+
+    // -- routing
+    uint32_t spt = 1024;
+    while ((o.n_strips + spt - 1) / spt > 512) spt <<= 1;
+    o.ntiles = (uint32_t)((o.n_strips + spt - 1) / spt);
+    uint32_t *hsk, *hkm;
+    uint64_t* part_sk;
+    const uint64_t msz = (uint64_t)std::max<uint32_t>(o.ntiles, 1) * o.nsp;
+    if ((rc = ensure_t(c, "hsk", msz, &hsk)) || (rc = ensure_t(c, "hkm", msz, &hkm)) ||
+        (rc = ensure_t(c, "part_sk", o.nsp + 1, &part_sk)) ||
+        (rc = ensure_t(c, "part_km", o.nsp + 1, &o.part_km)) ||
+        (rc = ensure_t(c, "part_off", o.nsp + 1, &o.part_off)) ||
+        (rc = ensure_t(c, "sorted", o.n_kmers + 1, &o.sorted)) ||
+        (rc = ensure_t(c, "scan_tmp2", tg::scan_tmp_words(o.nsp + 1) + 64, &tmp)))
+        return rc;
+    tg::RouteArgs r{};
+    r.strip_slot = o.strip_slot;
+    r.nsk = o.nsk;
+    r.rec = o.rec;
+    r.n_strips = (uint32_t)o.n_strips;
+    r.spt = spt;
+    r.nsp = o.nsp;
+    r.sb = o.sb;
+    r.k = o.k;
+    r.hsk = hsk;
+    r.hkm = hkm;
+    r.part_off = o.part_off;
+    r.dst = o.sorted;
+    {
+        Prof pr(c, "route_hist");
+        tg::launch_route_hist(st, r, o.ntiles);
+    }
+    {
+        Prof pr(c, "route_scan");
+        if (o.ntiles) tg::launch_route_colscan(st, r, o.ntiles, part_sk, o.part_km);
+        else {
+            tg::launch_fill_u64(st, part_sk, 0, o.nsp);
+            tg::launch_fill_u64(st, o.part_km, 0, o.nsp);
+        }
+        tg::launch_scan_u64(st, part_sk, o.nsp, o.part_off, tmp);
+    }
+    {
+        Prof pr(c, "route_scatter");
+        tg::launch_route_scatter(st, r, o.ntiles);
+    }
+    HIPOK(hipGetLastError());
+    HIPOK(hipMemcpyAsync(&o.n_sk, o.part_off + o.nsp, 8, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    return TG_OK;
+}
+
+// Work items: consecutive sub-partitions merged while their k-mers fit one
+// LDS table at ~60% load, large ones split by minimizer-hash bits.
+void build_items(const Phase1& o, const std::vector<uint64_t>& km, const std::vector<uint64_t>& off,
+                 uint32_t cap, double rho, std::vector<uint4>& items)
+{
+    items.clear();
+    const double target = 0.6 * cap;
+    const uint32_t rmax = 1u << o.sb;
+    uint32_t p = 0;
+    while (p < o.nsp) {
+        const double est = (double)km[p] * rho;
+        if (est > target) {
+            uint32_t r = 1;
+            while (r < rmax && est / r > target) r <<= 1;
+            for (uint32_t j = 0; j < r; ++j)
+                items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[p + 1], r - 1, j));
+            ++p;
+            continue;
+        }
+        uint32_t q = p;
+        double acc = 0;
+        while (q < o.nsp) {
+            const double e = (double)km[q] * rho;
+            if (e > target || (q > p && acc + e > target)) break;
+            acc += e;
+            ++q;
+        }
+        if (off[q] > off[p]) items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[q], 0u, 0u));
+        p = q;
+    }
+}
+
+int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
+{
+    hipStream_t st = c->stream;
+    const uint32_t nw = p->k > 32 ? 2u : 1u;
+    const uint32_t cap = nw == 1 ? tg::LDS_CAP_NW1 : tg::LDS_CAP_NW2;
+    std::vector<uint64_t> km(o.nsp), off(o.nsp + 1);
+    HIPOK(hipMemcpyAsync(km.data(), o.part_km, o.nsp * 8ull, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(off.data(), o.part_off, (o.nsp + 1) * 8ull, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    std::vector<uint4> items;
+    build_items(o, km, off, cap, 1.0, items);
+    int rc;
+    uint4* d_items;
+    if ((rc = ensure_t(c, "items", items.size() + 1, &d_items))) return rc;
+    if (!items.empty())
+        HIPOK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    uint64_t* out_keys;
+    uint32_t* out_cnt;
+    unsigned long long* counters;
+    uint32_t* sp_err;
+    if ((rc = ensure_t(c, "out_keys", (o.n_kmers + 1) * nw, &out_keys)) ||
+        (rc = ensure_t(c, "out_cnt", o.n_kmers + 1, &out_cnt)) ||
+        (rc = ensure_t(c, "counters", 8, &counters)) || (rc = ensure_t(c, "sp_err", 8, &sp_err)))
+        return rc;
+    uint64_t sp_slots = 1ull << 16;
+    while (sp_slots < o.n_kmers / 64) sp_slots <<= 1;
+    for (int attempt = 0;; ++attempt) {
+        uint64_t* sp_keys;
+        uint32_t* sp_cnt;
+        if ((rc = ensure_t(c, "sp_keys", sp_slots * nw, &sp_keys)) ||
+            (rc = ensure_t(c, "sp_cnt", sp_slots, &sp_cnt)))
+            return rc;
+        tg::CountArgs a{};
+        a.packed = o.packed;
+        a.rec = o.sorted;
+        a.items = d_items;
+        a.n_items = (uint32_t)items.size();
+        a.k = p->k;
+        a.ci = p->ci;
+        a.cx = p->cx;
+        a.probe_max = 64;
+        a.sp_keys = sp_keys;
+        a.sp_cnt = sp_cnt;
+        a.sp_mask = sp_slots - 1;
+        a.sp_err = sp_err;
+        a.out_keys = out_keys;
+        a.out_cnt = out_cnt;
+        a.counters = counters;
+        a.out_cap = o.n_kmers + 1;
+        {
+            Prof pr(c, "count_init");
+            if (nw == 1) tg::launch_fill_u64(st, sp_keys, ~0ull, sp_slots);
+            tg::launch_fill_u32(st, sp_cnt, 0u, sp_slots);
+            tg::launch_fill_u64(st, (uint64_t*)counters, 0ull, 8);
+            tg::launch_fill_u32(st, sp_err, 0u, 1);
+        }
+        {
+            Prof pr(c, "count");
+            tg::launch_count(st, a, nw);
+        }
+        {
+            Prof pr(c, "spill_drain");
+            tg::launch_spill_drain(st, a, nw, sp_slots);
+        }
+        HIPOK(hipGetLastError());
+        unsigned long long hc[4];
+        uint32_t err = 0;
+        HIPOK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, st));
+        HIPOK(hipMemcpyAsync(&err, sp_err, 4, hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        if (err) {
+            if (attempt >= 4) return fail(TG_ENOMEM, "spill table overflow after %d retries", attempt);
+            sp_slots <<= 2;
+            continue;
+        }
+        c->n_out = hc[0];
+        c->nw = nw;
+        c->d_keys = out_keys;
+        c->d_cnt = out_cnt;
+        if (st_out) {
+            st_out->n_unique = hc[1];
+            st_out->n_out = hc[0];
+            st_out->n_lds_groups = (uint32_t)items.size();
+            st_out->n_hbm_parts = (uint32_t)std::min<unsigned long long>(hc[2], 0xFFFFFFFFull);
+        }
+        break;
+    }
+    return TG_OK;
+}
+
+int count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N,
+                 const uint64_t* d_off, uint64_t nch, tg_stats* st)
+{
+    int rc = check_params(p, N, nch);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    tg_stats s{};
+    s.n_bases = N;
+    s.n_chunks = nch;
+    HIPOK(hipEventRecord(c->ev_a, c->stream));
+    Phase1 o;
+    if ((rc = run_phase1(c, p, d_bases, N, d_off, nch, o))) return rc;
+    HIPOK(hipEventRecord(c->ev_b, c->stream));
+    if ((rc = run_phase2(c, p, o, &s))) return rc;
+    HIPOK(hipEventRecord(c->ev_c, c->stream));
+    HIPOK(hipEventSynchronize(c->ev_c));
+    float m1 = 0, m2 = 0;
+    hipEventElapsedTime(&m1, c->ev_a, c->ev_b);
+    hipEventElapsedTime(&m2, c->ev_b, c->ev_c);
+    s.ms_phase1 = m1;
+    s.ms_phase2 = m2;
+    s.n_kmers = o.n_kmers;
+    s.n_superkmers = o.n_sk;
+    s.n_parts = o.n;
+    s.n_subparts = o.nsp;
+    collect_prof(c);
+    if (st) *st = s;
+    return TG_OK;
+}
+
+int upload_inputs(tg_ctx* c, const uint8_t* bases, uint64_t N, const uint64_t* off, uint64_t nch,
+                  uint8_t** d_b, uint64_t** d_o)
+{
+    int rc;
+    if ((rc = ensure_t(c, "in_bases", N + 64, d_b)) || (rc = ensure_t(c, "in_off", nch + 1, d_o)))
+        return rc;
+    if (N) HIPOK(hipMemcpyAsync(*d_b, bases, N, hipMemcpyHostToDevice, c->stream));
+    HIPOK(hipMemcpyAsync(*d_o, off, (nch + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+    return TG_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI ==
+extern "C" {
+
+const char* tg_last_error(void) { return g_err.c_str(); }
+
+int tg_ctx_create(int device, tg_ctx** out)
+{
+    if (!out) return fail(TG_EINVAL, "null output pointer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(TG_EDEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return fail(TG_EINVAL, "device %d out of range (%d devices)", device, n);
+    tg_ctx* c = new tg_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipEventCreate(&c->ev_a) != hipSuccess || hipEventCreate(&c->ev_b) != hipSuccess ||
+        hipEventCreate(&c->ev_c) != hipSuccess) {
+        delete c;
+        return fail(TG_EDEVICE, "HIP context setup failed");
+    }
+    *out = c;
+    return TG_OK;
+}
+
+void tg_ctx_destroy(tg_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& kv : c->bufs)
+        if (kv.second.p) hipFree(kv.second.p);
+    for (auto& pe : c->pending) {
+        hipEventDestroy(pe.second.first);
+        hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    hipEventDestroy(c->ev_a);
+    hipEventDestroy(c->ev_b);
+    hipEventDestroy(c->ev_c);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int tg_set_profiling(tg_ctx* c, int on)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    c->profiling = on != 0;
+    return TG_OK;
+}
+
+void tg_reset_kernel_times(tg_ctx* c)
+{
+    if (c) c->ktimes.clear();
+}
+
+int tg_kernel_times(tg_ctx* c, char* names, uint64_t cap, double* ms, uint64_t* launches, int n)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    collect_prof(c);
+    int i = 0;
+    uint64_t used = 0;
+    for (auto& kv : c->ktimes) {
+        if (i < n) {
+            if (ms) ms[i] = kv.second.ms;
+            if (launches) launches[i] = kv.second.n;
+            if (names) {
+                const uint64_t L = kv.first.size() + 1;
+                if (used + L <= cap) {
+                    memcpy(names + used, kv.first.c_str(), L);
+                    used += L;
+                }
+            }
+        }
+        ++i;
+    }
+    return i;
+}
+
+int tg_dev_alloc(tg_ctx* c, uint64_t bytes, void** dptr)
+{
+    if (!c || !dptr) return fail(TG_EINVAL, "null argument");
+    HIPOK(hipSetDevice(c->device));
+    if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TG_ENOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+    }
+    return TG_OK;
+}
+
+int tg_dev_free(tg_ctx* c, void* dptr)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    HIPOK(hipFree(dptr));
+    return TG_OK;
+}
+
+int tg_h2d(tg_ctx* c, void* dst, const void* src, uint64_t bytes)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    HIPOK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_d2h(tg_ctx* c, void* dst, const void* src, uint64_t bytes)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    HIPOK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_sync(tg_ctx* c)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                    const uint64_t* d_off, uint64_t n_chunks, tg_stats* st)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    return count_device(c, p, d_bases, n_bases, d_off, n_chunks, st);
+}
+
+int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                  const uint64_t* off, uint64_t n_chunks, tg_stats* st)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    uint8_t* d_b;
+    uint64_t* d_o;
+    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    return count_device(c, p, d_b, n_bases, d_o, n_chunks, st);
+}
+
+uint64_t tg_result_size(tg_ctx* c) { return c ? c->n_out : 0; }
+
+int tg_result_copy(tg_ctx* c, uint64_t* keys, uint32_t* counts, uint64_t cap)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    const uint64_t n = std::min<uint64_t>(cap, c->n_out);
+    if (n == 0) return TG_OK;
+    if (keys) HIPOK(hipMemcpyAsync(keys, c->d_keys, n * c->nw * 8ull, hipMemcpyDeviceToHost, c->stream));
+    if (counts) HIPOK(hipMemcpyAsync(counts, c->d_cnt, n * 4ull, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_pack_host(tg_ctx* c, const uint8_t* bases, uint64_t n_bases, uint32_t* out_words)
+{
+    if (!c || (!bases && n_bases) || !out_words) return fail(TG_EINVAL, "null argument");
+    HIPOK(hipSetDevice(c->device));
+    uint8_t* d_b;
+    uint32_t* d_w;
+    int rc;
+    const uint64_t nw = (n_bases + 15) / 16;
+    if ((rc = ensure_t(c, "in_bases", n_bases + 64, &d_b)) || (rc = ensure_t(c, "packed", nw + 8, &d_w)))
+        return rc;
+    if (n_bases) HIPOK(hipMemcpyAsync(d_b, bases, n_bases, hipMemcpyHostToDevice, c->stream));
+    tg::launch_pack(c->stream, d_b, n_bases, d_w, nw);
+    if (nw) HIPOK(hipMemcpyAsync(out_words, d_w, nw * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_superkmers_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                       const uint64_t* off, uint64_t n_chunks, uint64_t* start, uint32_t* len,
+                       uint32_t* minpos, uint32_t* pid, uint64_t cap, uint64_t* n_sk,
+                       uint32_t* n_subparts)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    uint8_t* d_b;
+    uint64_t* d_o;
+    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    Phase1 o;
+    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o))) return rc;
+    collect_prof(c);
+    if (n_sk) *n_sk = o.n_sk;
+    if (n_subparts) *n_subparts = o.nsp;
+    const uint64_t n = std::min<uint64_t>(o.n_sk, cap);
+    std::vector<uint2> recs(n);
+    if (n) HIPOK(hipMemcpy(recs.data(), o.sorted, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    const uint32_t shift = tg::REC_HSEL_SHIFT + o.sb;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (start) start[i] = recs[i].x;
+        if (len) len[i] = recs[i].y & tg::REC_LEN_MASK;
+        if (minpos) minpos[i] = (recs[i].y >> tg::REC_MINPOS_SHIFT) & 63u;
+        if (pid) pid[i] = recs[i].y >> shift;
+    }
+    return o.n_sk > cap ? fail(TG_ECAP, "superkmer buffer too small (%llu needed)",
+                               (unsigned long long)o.n_sk)
+                        : TG_OK;
+}
+
+int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                       const uint64_t* off, uint64_t n_chunks, uint64_t* part_off,
+                       uint8_t* rec_bytes, uint64_t bytes_cap, uint64_t* rec_off,
+                       uint64_t rec_cap, uint64_t* n_sk, uint64_t* n_bytes)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    uint8_t* d_b;
+    uint64_t* d_o;
+    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    Phase1 o;
+    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o))) return rc;
+    collect_prof(c);
+    std::vector<uint2> recs(o.n_sk);
+    std::vector<uint64_t> poff(o.nsp + 1);
+    if (o.n_sk) HIPOK(hipMemcpy(recs.data(), o.sorted, o.n_sk * sizeof(uint2), hipMemcpyDeviceToHost));
+    HIPOK(hipMemcpy(poff.data(), o.part_off, (o.nsp + 1) * 8ull, hipMemcpyDeviceToHost));
+    if (n_sk) *n_sk = o.n_sk;
+    // reference partition P = sub-partitions [P*s, (P+1)*s)
+    const uint32_t s = 1u << o.log2s;
+    if (part_off)
+        for (uint32_t P = 0; P <= o.n; ++P) part_off[P] = poff[(uint64_t)P * s];
+    // serialise in the reference record format (src/superkmer_io.hpp:154-172);
+    // hdr_t is one byte for every k <= 64 (2k-m <= 127)
+    uint64_t used = 0;
+    for (uint64_t i = 0; i < o.n_sk; ++i) {
+        const uint32_t L = recs[i].y & tg::REC_LEN_MASK;
+        const uint32_t mp = (recs[i].y >> tg::REC_MINPOS_SHIFT) & 63u;
+        const uint64_t need = 2 + (L + 3) / 4;
+        if (rec_off && i < rec_cap) rec_off[i] = used;
+        if (rec_bytes && used + need <= bytes_cap) {
+            uint8_t* d = rec_bytes + used;
+            d[0] = (uint8_t)L;
+            d[1] = (uint8_t)mp;
+            memset(d + 2, 0, (L + 3) / 4);
+            for (uint32_t j = 0; j < L; ++j) {
+                const uint8_t ch = bases[recs[i].x + j];
+                d[2 + j / 4] |= (uint8_t)((((ch >> 2) ^ (ch >> 1)) & 3u) << (6u - 2u * (j % 4u)));
+            }
+        }
+        used += need;
+    }
+    if (rec_off && o.n_sk < rec_cap) rec_off[o.n_sk] = used;
+    if (n_bytes) *n_bytes = used;
+    if (used > bytes_cap || o.n_sk + 1 > rec_cap)
+        return fail(TG_ECAP, "record buffers too small (%llu bytes, %llu records)",
+                    (unsigned long long)used, (unsigned long long)o.n_sk);
+    return TG_OK;
+}
+
+}  // extern "C"
