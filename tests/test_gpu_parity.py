@@ -108,8 +108,10 @@ def test_superkmers_match_oracle(eng, k, m):
     gs, gl, gm, gp, nsp = eng.superkmers(bases, off, k, m, n)
     os_, ol, om, oh, km = O.superkmers(bases, off, k, m)
     log2n, log2s = _ilog2(n), _ilog2(nsp) - _ilog2(n)
+    sb = 18 - _ilog2(nsp)
     oh = oh.astype(np.uint64)
-    opid = ((oh & np.uint64(n - 1)) << np.uint64(log2s)) | ((oh >> np.uint64(log2n)) & np.uint64((1 << log2s) - 1))
+    # sub-partition = reference partition bits, then hash bits above the split bits
+    opid = ((oh & np.uint64(n - 1)) << np.uint64(log2s)) | ((oh >> np.uint64(log2n + sb)) & np.uint64((1 << log2s) - 1))
     a = sorted(zip(gs.tolist(), gl.tolist(), gm.tolist(), gp.tolist()))
     b = sorted(zip(os_.tolist(), ol.tolist(), om.tolist(), opid.astype(np.uint32).tolist()))
     assert len(a) == len(b)

@@ -187,15 +187,13 @@ int check_params(const tg_params* p, uint64_t n_bases, uint64_t n_chunks)
     return TG_OK;
 }
 
+
 // ---------------------------------------------------------------------------
 struct Phase1 {
     uint32_t k = 0, m = 0, W = 0, S = 0;
     uint32_t n = 0, log2n = 0, nsp = 0, log2s = 0, sb = 0;
     uint64_t n_strips = 0, n_kmers = 0, n_sk = 0;
     uint32_t* packed = nullptr;
-    uint2* rec = nullptr;       // strip-slot space
-    uint32_t* nsk = nullptr;
-    uint64_t* strip_slot = nullptr;
     uint2* sorted = nullptr;    // partition order
     uint64_t* part_off = nullptr;
     uint64_t* part_km = nullptr;
@@ -213,13 +211,8 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     o.n = round_pow2(p->n_partitions ? p->n_partitions : 16u);
     if (o.n > (1u << 14)) return fail(TG_EINVAL, "n_partitions %u exceeds 16384 on the GPU engine", o.n);
     o.log2n = ilog2(o.n);
-    // device sub-partitions: ~16K bases each, at most 2^14 (LDS histograms)
-    uint64_t want = 1;
-    while (want < N / 16384 && want < (1u << 14)) want <<= 1;
-    if (want < o.n) want = o.n;
-    o.nsp = (uint32_t)want;
-    o.log2s = ilog2(o.nsp) - o.log2n;
-    o.sb = tg::HSEL_BITS - ilog2(o.nsp);
+    // device sub-partitions are chosen after Phase 1 (records carry 18 hash
+    // bits; the sub-partition id is their top log2(nsp) bits)
     // strip length: short chunks are one strip; long chunks are cut so that
     // the look-back (256-k k-mers) stays a small fraction
     o.S = 2048;
@@ -244,27 +237,35 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         tg::launch_scan_u64(st, nstr, nch, nstr_x, tmp);
         tg::launch_scan_u64(st, nkm, nch, nkm_x, tmp);
     }
-    uint64_t tot[2];
     HIPOK(hipGetLastError());
+    uint64_t tot[2];
     HIPOK(hipMemcpyAsync(&tot[0], nstr_x + nch, 8, hipMemcpyDeviceToHost, st));
     HIPOK(hipMemcpyAsync(&tot[1], nkm_x + nch, 8, hipMemcpyDeviceToHost, st));
     HIPOK(hipStreamSynchronize(st));
     o.n_strips = tot[0];
     o.n_kmers = tot[1];
-    if (o.n_strips >= (1ull << 32)) return fail(TG_EINVAL, "too many strips");
-    uint32_t *strip_chunk, *strip_tb, *flag_list, *flag_count;
+    if (o.n_strips >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
+    const uint64_t n_blocks = (o.n_strips + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
+    uint32_t *strip_chunk, *strip_tb, *flag_list, *flag_count, *run_cnt;
+    uint64_t *strip_slot, *run_off;
+    uint2* rec;
+    uint2** run_ptr;
     const uint64_t ns1 = o.n_strips + 1;
+    // run table: one run per main-pass block, then the fix-up blocks
+    const uint64_t run_cap = n_blocks + n_blocks + 2;
     if ((rc = ensure_t(c, "strip_chunk", ns1, &strip_chunk)) ||
         (rc = ensure_t(c, "strip_tb", ns1, &strip_tb)) ||
-        (rc = ensure_t(c, "strip_slot", ns1, &o.strip_slot)) ||
-        (rc = ensure_t(c, "nsk", ns1, &o.nsk)) ||
+        (rc = ensure_t(c, "strip_slot", ns1, &strip_slot)) ||
         (rc = ensure_t(c, "flag_list", ns1, &flag_list)) ||
         (rc = ensure_t(c, "flag_count", 64, &flag_count)) ||
-        (rc = ensure_t(c, "rec", o.n_kmers + 1, &o.rec)))
+        (rc = ensure_t(c, "run_ptr", run_cap, &run_ptr)) ||
+        (rc = ensure_t(c, "run_cnt", run_cap, &run_cnt)) ||
+        (rc = ensure_t(c, "run_off", run_cap + 1, &run_off)) ||
+        (rc = ensure_t(c, "rec", o.n_kmers + 1, &rec)))
         return rc;
     {
         Prof pr(c, "strips");
-        tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, o.strip_slot);
+        tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, strip_slot);
         tg::launch_fill_u32(st, flag_count, 0u, 1);
     }
     tg::SkArgs a{};
@@ -272,7 +273,7 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     a.off = d_off;
     a.strip_chunk = strip_chunk;
     a.strip_tb = strip_tb;
-    a.strip_slot = o.strip_slot;
+    a.strip_slot = strip_slot;
     a.list = nullptr;
     a.n = (uint32_t)o.n_strips;
     a.k = o.k;
@@ -282,10 +283,10 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     a.capk = 256u - o.k;
     a.lookback = a.capk + 1u;
     a.log2n = o.log2n;
-    a.log2s = o.log2s;
-    a.sb = o.sb;
-    a.rec = o.rec;
-    a.nsk = o.nsk;
+    a.rec = rec;
+    a.run_ptr = run_ptr;
+    a.run_cnt = run_cnt;
+    a.run_base = 0;
     a.flag_list = flag_list;
     a.flag_count = flag_count;
     {
@@ -297,39 +298,74 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     HIPOK(hipMemcpyAsync(&nflag, flag_count, 4, hipMemcpyDeviceToHost, st));
     HIPOK(hipStreamSynchronize(st));
     o.n_flagged = nflag;
+    uint64_t n_runs = n_blocks;
     if (nflag) {
         // strips whose first run began before the look-back: redo them from
         // the chunk start (only long low-complexity runs in long chunks)
+        const uint64_t fb = (nflag + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
         uint32_t* list2;
-        if ((rc = ensure_t(c, "flag_list2", nflag + 1, &list2))) return rc;
+        uint2* rec_fix;
+        if ((rc = ensure_t(c, "flag_list2", nflag + 1, &list2)) ||
+            (rc = ensure_t(c, "rec_fix", fb * tg::SK_BLOCK * o.S, &rec_fix)))
+            return rc;
         HIPOK(hipMemcpyAsync(list2, flag_list, nflag * 4ull, hipMemcpyDeviceToDevice, st));
         a.list = list2;
         a.n = nflag;
         a.lookback = tg::LOOKBACK_FULL;
-        Prof pr(c, "superkmers_fixup");
-        tg::launch_superkmers(st, a);
+        a.rec = rec_fix;
+        a.run_base = (uint32_t)n_blocks;
+        {
+            Prof pr(c, "superkmers_fixup");
+            tg::launch_superkmers(st, a);
+        }
+        HIPOK(hipGetLastError());
+        n_runs += fb;
     }
 
-    // -- routing
-    uint32_t spt = 1024;
-    while ((o.n_strips + spt - 1) / spt > 512) spt <<= 1;
-    o.ntiles = (uint32_t)((o.n_strips + spt - 1) / spt);
+    // -- routing: tiles over the concatenated run space
+    uint64_t* run_cnt64;
+    if ((rc = ensure_t(c, "run_cnt64", n_runs + 1, &run_cnt64)) ||
+        (rc = ensure_t(c, "scan_tmp3", tg::scan_tmp_words(n_runs + 1) + 64, &tmp)))
+        return rc;
+    {
+        Prof pr(c, "route_scan");
+        tg::launch_u32_to_u64(st, run_cnt, n_runs, run_cnt64);
+        tg::launch_scan_u64(st, run_cnt64, n_runs, run_off, tmp);
+    }
+    HIPOK(hipMemcpyAsync(&o.n_sk, run_off + n_runs, 8, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    uint64_t tile = 16384;
+    while ((o.n_sk + tile - 1) / tile > 512) tile <<= 1;
+    o.ntiles = (uint32_t)((o.n_sk + tile - 1) / tile);
+    // sub-partitions: ~16K k-mers each (N/16384), at least the reference n,
+    // at least 2, at most 2^14 (LDS histograms)
+    {
+        uint64_t want = 2;
+        while (want < o.n_kmers / 16384 && want < (1u << 14)) want <<= 1;
+        if (want < o.n) want = o.n;
+        o.nsp = (uint32_t)want;
+        o.log2s = ilog2(o.nsp) - o.log2n;
+        o.sb = tg::HSEL_BITS - ilog2(o.nsp);
+    }
     uint32_t *hsk, *hkm;
     uint64_t* part_sk;
+    uint2* route_tmp;
     const uint64_t msz = (uint64_t)std::max<uint32_t>(o.ntiles, 1) * o.nsp;
     if ((rc = ensure_t(c, "hsk", msz, &hsk)) || (rc = ensure_t(c, "hkm", msz, &hkm)) ||
         (rc = ensure_t(c, "part_sk", o.nsp + 1, &part_sk)) ||
         (rc = ensure_t(c, "part_km", o.nsp + 1, &o.part_km)) ||
         (rc = ensure_t(c, "part_off", o.nsp + 1, &o.part_off)) ||
-        (rc = ensure_t(c, "sorted", o.n_kmers + 1, &o.sorted)) ||
+        (rc = ensure_t(c, "sorted", o.n_sk + 1, &o.sorted)) ||
+        (rc = ensure_t(c, "route_tmp", o.n_sk + 1, &route_tmp)) ||
         (rc = ensure_t(c, "scan_tmp2", tg::scan_tmp_words(o.nsp + 1) + 64, &tmp)))
         return rc;
     tg::RouteArgs r{};
-    r.strip_slot = o.strip_slot;
-    r.nsk = o.nsk;
-    r.rec = o.rec;
-    r.n_strips = (uint32_t)o.n_strips;
-    r.spt = spt;
+    r.run_ptr = run_ptr;
+    r.run_cnt = run_cnt;
+    r.run_off = run_off;
+    r.n_runs = (uint32_t)n_runs;
+    r.n_rec = o.n_sk;
+    r.tile = (uint32_t)tile;
     r.nsp = o.nsp;
     r.sb = o.sb;
     r.k = o.k;
@@ -352,44 +388,71 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     }
     {
         Prof pr(c, "route_scatter");
-        tg::launch_route_scatter(st, r, o.ntiles);
+        tg::launch_route_scatter(st, r, o.ntiles, route_tmp);
     }
     HIPOK(hipGetLastError());
-    HIPOK(hipMemcpyAsync(&o.n_sk, o.part_off + o.nsp, 8, hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
     return TG_OK;
 }
 
-// Work items: consecutive sub-partitions merged while their k-mers fit one
-// LDS table at ~60% load, large ones split by minimizer-hash bits.
+// Work items over partitions [p_begin, p_end): consecutive sub-partitions
+// merged while their estimated distinct k-mers (k-mers x rho) fit one LDS table
+// at ~60% load; larger ones split by minimizer-hash bits.
 void build_items(const Phase1& o, const std::vector<uint64_t>& km, const std::vector<uint64_t>& off,
-                 uint32_t cap, double rho, std::vector<uint4>& items)
+                 uint32_t cap, double rho, const std::vector<uint8_t>* skip, uint32_t p_begin,
+                 uint32_t p_end, uint32_t stride, std::vector<uint4>& items)
 {
-    items.clear();
-    const double target = 0.6 * cap;
+    const double target = 0.75 * cap;
     const uint32_t rmax = 1u << o.sb;
-    uint32_t p = 0;
-    while (p < o.nsp) {
+    uint32_t p = p_begin;
+    while (p < p_end) {
+        if (skip && (*skip)[p]) {
+            p += 1;
+            continue;
+        }
         const double est = (double)km[p] * rho;
         if (est > target) {
             uint32_t r = 1;
             while (r < rmax && est / r > target) r <<= 1;
             for (uint32_t j = 0; j < r; ++j)
                 items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[p + 1], r - 1, j));
-            ++p;
+            p += stride;
             continue;
         }
         uint32_t q = p;
         double acc = 0;
-        while (q < o.nsp) {
+        const uint32_t q_end = stride > 1 ? std::min(p + 1, p_end) : p_end;
+        while (q < q_end && !(skip && (*skip)[q])) {
             const double e = (double)km[q] * rho;
             if (e > target || (q > p && acc + e > target)) break;
             acc += e;
             ++q;
         }
+        if (q == p) q = p + 1;
         if (off[q] > off[p]) items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[q], 0u, 0u));
-        p = q;
+        p = stride > 1 ? p + stride : q;
     }
+}
+
+int launch_items(tg_ctx* c, const char* buf, const std::vector<uint4>& items, tg::CountArgs& a,
+                 uint32_t nw, uint32_t** item_uniq)
+{
+    if (items.empty()) return TG_OK;
+    int rc;
+    uint4* d_items;
+    if ((rc = ensure_t(c, buf, items.size() + 1, &d_items))) return rc;
+    HIPOK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                         c->stream));
+    if (item_uniq) {
+        if ((rc = ensure_t(c, "item_uniq", items.size() + 1, item_uniq))) return rc;
+        a.item_uniq = *item_uniq;
+    } else {
+        a.item_uniq = nullptr;
+    }
+    a.items = d_items;
+    a.n_items = (uint32_t)items.size();
+    Prof pr(c, item_uniq ? "count_sample" : "count");
+    tg::launch_count(c->stream, a, nw);
+    return TG_OK;
 }
 
 int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
@@ -401,13 +464,7 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
     HIPOK(hipMemcpyAsync(km.data(), o.part_km, o.nsp * 8ull, hipMemcpyDeviceToHost, st));
     HIPOK(hipMemcpyAsync(off.data(), o.part_off, (o.nsp + 1) * 8ull, hipMemcpyDeviceToHost, st));
     HIPOK(hipStreamSynchronize(st));
-    std::vector<uint4> items;
-    build_items(o, km, off, cap, 1.0, items);
     int rc;
-    uint4* d_items;
-    if ((rc = ensure_t(c, "items", items.size() + 1, &d_items))) return rc;
-    if (!items.empty())
-        HIPOK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
     uint64_t* out_keys;
     uint32_t* out_cnt;
     unsigned long long* counters;
@@ -427,8 +484,6 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
         tg::CountArgs a{};
         a.packed = o.packed;
         a.rec = o.sorted;
-        a.items = d_items;
-        a.n_items = (uint32_t)items.size();
         a.k = p->k;
         a.ci = p->ci;
         a.cx = p->cx;
@@ -448,10 +503,32 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
             tg::launch_fill_u64(st, (uint64_t*)counters, 0ull, 8);
             tg::launch_fill_u32(st, sp_err, 0u, 1);
         }
-        {
-            Prof pr(c, "count");
-            tg::launch_count(st, a, nw);
+        // Calibration (after src/count.hpp:905-938, which sizes tables from the
+        // first partition's distinct count): count a spread sample of
+        // sub-partitions first, estimate distinct/k-mers, size the rest.
+        std::vector<uint8_t> done(o.nsp, 0);
+        std::vector<uint4> items;
+        double rho = 1.0;
+        const uint32_t stride = o.nsp >= 256 ? o.nsp / 64 : 0;
+        if (stride) {
+            build_items(o, km, off, cap, 1.0, nullptr, 0, o.nsp, stride, items);
+            uint32_t* item_uniq = nullptr;
+            if ((rc = launch_items(c, "items_cal", items, a, nw, &item_uniq))) return rc;
+            std::vector<uint32_t> uq(items.size());
+            if (!items.empty())
+                HIPOK(hipMemcpyAsync(uq.data(), item_uniq, items.size() * 4, hipMemcpyDeviceToHost, st));
+            HIPOK(hipStreamSynchronize(st));
+            uint64_t su = 0, sk = 0;
+            for (size_t i = 0; i < uq.size(); ++i) su += uq[i];
+            for (uint32_t q = 0; q < o.nsp; q += stride) {
+                sk += km[q];
+                done[q] = 1;
+            }
+            if (sk) rho = std::min(1.0, std::max(0.02, 1.15 * (double)su / (double)sk));
         }
+        items.clear();
+        build_items(o, km, off, cap, rho, stride ? &done : nullptr, 0, o.nsp, 1, items);
+        if ((rc = launch_items(c, "items", items, a, nw, nullptr))) return rc;
         {
             Prof pr(c, "spill_drain");
             tg::launch_spill_drain(st, a, nw, sp_slots);

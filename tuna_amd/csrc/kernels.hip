@@ -196,6 +196,20 @@ void launch_scan_u64(hipStream_t st, const uint64_t* in, uint64_t n, uint64_t* o
     k_scan_apply<<<(unsigned)nb, SCAN_T, 0, st>>>(in, n, soff, out);
 }
 
+__global__ void k_u32_to_u64(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
+void launch_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t n, uint64_t* out)
+{
+    if (!n) return;
+    uint64_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    k_u32_to_u64<<<(unsigned)b, 256, 0, st>>>(in, n, out);
+}
+
 // ======================================================== superkmers ====
 // Lane state for one strip: rolling ntHash over the packed stream plus the
 // superkmer cut state.  The cut rule follows src/partition_hash.hpp:78-93: a
@@ -213,9 +227,8 @@ struct SkLane {
     uint32_t ts, tb, te, K;
     uint32_t t0, P0;
     uint64_t H0, prevH;
-    bool have, done, flagged;
-    uint32_t nrec;
-    uint2* out;
+    uint32_t hsel;
+    bool have, done, flagged, tail;
     uint64_t cb;
 };
 
@@ -236,49 +249,85 @@ __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uin
     L.rv = rotr1(L.rv) ^ tr[ix];
 }
 
-__device__ __forceinline__ uint32_t route_hsel(uint64_t H, uint32_t log2n, uint32_t log2s, uint32_t sb)
+// Routing field of a record (18 bits of the minimizer hash H): the reference
+// partition H & (n-1) on top (src/partition_hash.hpp:549-553), then the next
+// 18-log2(n) bits of H.  Its top log2(NSP) bits are the device sub-partition,
+// the low SB bits split large sub-partitions in Phase 2.
+__device__ __forceinline__ uint32_t route_hsel(uint64_t H, uint32_t log2n)
 {
-    const uint64_t n1 = (1ull << log2n) - 1u, s1 = (1ull << log2s) - 1u;
-    const uint32_t pid = (uint32_t)(((H & n1) << log2s) | ((H >> log2n) & s1));
-    const uint32_t split = (uint32_t)(H >> (log2n + log2s)) & ((1u << sb) - 1u);
-    return (pid << sb) | split;
+    const uint32_t h = (uint32_t)H;
+    const uint32_t rest = HSEL_BITS - log2n;
+    return ((h & ((1u << log2n) - 1u)) << rest) | ((h >> log2n) & ((1u << rest) - 1u));
 }
 
-__device__ __forceinline__ void sk_emit(SkLane& L, const SkArgs& a, uint32_t tend)
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 {
-    const uint32_t len = tend - L.t0 + a.k - 1u;
-    const uint32_t mp = L.P0 - L.t0;
-    const uint32_t hsel = route_hsel(L.H0, a.log2n, a.log2s, a.sb);
-    L.out[L.nrec++] = make_uint2((uint32_t)(L.cb + L.t0),
-                                 len | (mp << REC_MINPOS_SHIFT) | (hsel << REC_HSEL_SHIFT));
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Decision for k-mer t with window minimum (Hc, Pc).
-__device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, uint32_t t, uint64_t Hc, uint32_t Pc)
+// Per-wave staging of records: the active lanes append with a ballot, and
+// when the buffer nears full they copy it to the workgroup's run (reserved by
+// one LDS atomic), so global writes are contiguous.
+struct SkStage {
+    uint2* buf;        // this wave's buffer (LDS)
+    uint32_t* wn;      // this wave's fill count (LDS)
+    uint32_t* wg_n;    // workgroup run fill (LDS)
+    uint2* run;        // workgroup run (global)
+};
+
+__device__ __forceinline__ void sk_flush(const SkStage& S, uint32_t cnt)
 {
-    if (t == L.ts) {
-        if (L.ts == 0) {
-            L.t0 = 0; L.H0 = Hc; L.P0 = Pc; L.have = true;
-        }
-    } else {
-        const bool cut = (Hc != L.prevH) || (L.have && (t - L.t0 >= a.capk));
-        if (cut) {
-            if (L.have && L.t0 >= L.tb) sk_emit(L, a, t);
-            if (t >= L.te) {
-                L.done = true;
-            } else {
-                L.t0 = t; L.H0 = Hc; L.P0 = Pc; L.have = true;
-            }
-        } else if (t == L.tb && !L.have) {
-            L.flagged = true;
-            L.done = true;
-        }
-    }
+    const uint64_t act = __ballot(1);
+    const uint32_t nact = (uint32_t)__popcll(act), rank = lane_rank(act);
+    uint32_t base = 0;
+    if (rank == 0) base = atomicAdd(S.wg_n, cnt);
+    base = __builtin_amdgcn_readfirstlane(base);
+    for (uint32_t i = rank; i < cnt; i += nact) S.run[base + i] = S.buf[i];
+    __builtin_amdgcn_wave_barrier();
+    *S.wn = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void sk_push(const SkStage& S, bool e, uint2 rec)
+{
+    const uint64_t mask = __ballot(e);
+    if (!mask) return;
+    const uint32_t base = __builtin_amdgcn_readfirstlane(*S.wn);
+    // lanes that do not emit write the scratch slot past the buffer end
+    const uint32_t slot = e ? base + lane_rank(mask) : SK_WAVE_BUF + (threadIdx.x & 63u);
+    S.buf[slot] = rec;
+    const uint32_t cnt = base + (uint32_t)__popcll(mask);
+    __builtin_amdgcn_wave_barrier();
+    *S.wn = cnt;
+    __builtin_amdgcn_wave_barrier();
+    if (cnt > SK_WAVE_BUF - 64u) sk_flush(S, cnt);
+}
+
+// Decision for k-mer t with window minimum (Hc, Pc), written without branches
+// so the active lanes stay converged for the one ballot in sk_push.
+__device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, const SkStage& S, uint32_t t,
+                                        uint64_t Hc, uint32_t Pc)
+{
+    const bool first = t == L.ts;
+    const bool cut = !first && ((Hc != L.prevH) || (L.have && (t - L.t0 >= a.capk)));
+    const bool emit = cut && L.have && L.t0 >= L.tb;
+    const bool flag = !first && !cut && t == L.tb && !L.have;
+    const uint2 rec = make_uint2((uint32_t)L.cb + L.t0,
+                                 (t - L.t0 + a.k - 1u) | ((L.P0 - L.t0) << REC_MINPOS_SHIFT) |
+                                     (L.hsel << REC_HSEL_SHIFT));
+    const bool stop = (cut && t >= L.te) || flag;
+    const bool start = (first && L.ts == 0) || (cut && t < L.te);
+    L.t0 = start ? t : L.t0;
+    L.H0 = start ? Hc : L.H0;
+    L.P0 = start ? Pc : L.P0;
+    L.hsel = start ? route_hsel(Hc, a.log2n) : L.hsel;
+    L.have = L.have || start;
     L.prevH = Hc;
-    if (!L.done && t == L.K - 1u) {
-        if (L.have && L.t0 >= L.tb) sk_emit(L, a, L.K);
-        L.done = true;
-    }
+    const bool last = !stop && t == L.K - 1u;
+    L.tail = last && L.have && L.t0 >= L.tb;   // [t0, K) is emitted after the loop
+    L.flagged = L.flagged || flag;
+    L.done = stop || last;
+    sk_push(S, emit, rec);
 }
 
 __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, const uint64_t* s_init)
@@ -291,8 +340,6 @@ __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, 
     L.te = min(L.tb + a.S, L.K);
     L.ts = 0;
     if (a.lookback != LOOKBACK_FULL && L.tb > a.lookback) L.ts = ((L.tb - a.lookback) / a.W) * a.W;
-    L.out = a.rec + a.strip_slot[s];
-    L.nrec = 0;
     L.packed = a.packed;
     const uint64_t gq = L.cb + L.ts;
     L.widx = gq >> 4;
@@ -303,7 +350,8 @@ __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, 
     L.lmer = 0;
     L.osh = 2u * (a.m - 1u);
     L.t0 = 0; L.P0 = 0; L.H0 = 0; L.prevH = 0;
-    L.have = false; L.done = false; L.flagged = false;
+    L.hsel = 0;
+    L.have = false; L.done = false; L.flagged = false; L.tail = false;
 }
 
 __device__ __forceinline__ void sk_tables(const SkArgs& a, uint64_t* tf, uint64_t* tr, uint64_t* s_init)
@@ -353,105 +401,195 @@ __device__ __forceinline__ void sk_backward(uint64_t (&sv)[WMAX], uint8_t (&sp)[
 }
 
 template <int WMAX>
-__global__ __launch_bounds__(256) void k_superkmers(SkArgs a)
+__global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
 {
     __shared__ uint64_t tf[16], tr[16], s_init[2];
+    __shared__ uint2 s_buf[SK_BLOCK / 64][SK_WAVE_BUF + 64];
+    __shared__ uint32_t s_wn[SK_BLOCK / 64];
+    __shared__ uint32_t s_wgn;
     sk_tables(a, tf, tr, s_init);
+    const uint32_t wv = threadIdx.x >> 6;
+    if (threadIdx.x < SK_BLOCK / 64) s_wn[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_wgn = 0;
     __syncthreads();
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= a.n) return;
-    const uint32_t s = a.list ? a.list[idx] : idx;
-    SkLane L;
-    sk_init(L, a, s, s_init);
-    const uint32_t W = a.W;
 
-    for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, tf, tr);
-    uint64_t sv[WMAX];
-    uint8_t sp[WMAX];
-#pragma unroll
-    for (int r = 0; r < WMAX; ++r) {
-        if ((uint32_t)r < W) {
-            sk_roll(L, tf, tr);
-            sv[r] = L.f ^ L.rv;
-        }
-    }
-    sk_backward<WMAX>(sv, sp, W);
-    uint32_t bs = L.ts;
-    uint64_t pv = 0;
-    uint32_t pp = 0;
-    while (!L.done) {
+    const uint32_t idx = blockIdx.x * SK_BLOCK + threadIdx.x;
+    SkStage S;
+    S.buf = s_buf[wv];
+    S.wn = &s_wn[wv];
+    S.wg_n = &s_wgn;
+    // main pass: the run is the slot region of this block's strips; fix-up:
+    // a private region of SK_BLOCK*S records per block
+    S.run = a.list ? a.rec + (uint64_t)blockIdx.x * SK_BLOCK * a.S
+                   : a.rec + a.strip_slot[blockIdx.x * SK_BLOCK];
+    bool tail = false;
+    uint2 tail_rec = make_uint2(0u, 0u);
+    if (idx < a.n) {
+        const uint32_t s = a.list ? a.list[idx] : idx;
+        SkLane L;
+        sk_init(L, a, s, s_init);
+        const uint32_t W = a.W;
+        for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, tf, tr);
+        uint64_t sv[WMAX];
+        uint8_t sp[WMAX];
 #pragma unroll
         for (int r = 0; r < WMAX; ++r) {
-            if ((uint32_t)r < W && !L.done) {
-                const uint32_t t = bs + (uint32_t)r;
-                uint64_t Hc;
-                uint32_t Pc;
-                if (r == 0) {
-                    Hc = sv[0];
-                    Pc = bs + sp[0];
-                } else if (sv[r] < pv) {
-                    Hc = sv[r];
-                    Pc = bs + sp[r];
-                } else {
-                    Hc = pv;
-                    Pc = bs + W + pp;
-                }
-                sk_kmer(L, a, t, Hc, Pc);
-                if (!L.done) {
-                    sk_roll(L, tf, tr);
-                    const uint64_t h = L.f ^ L.rv;
-                    sv[r] = h;
-                    if (r == 0 || h < pv) {
-                        pv = h;
-                        pp = (uint32_t)r;
+            if ((uint32_t)r < W) {
+                sk_roll(L, tf, tr);
+                sv[r] = L.f ^ L.rv;
+            }
+        }
+        sk_backward<WMAX>(sv, sp, W);
+        uint32_t bs = L.ts;
+        uint64_t pv = 0;
+        uint32_t pp = 0;
+        while (!L.done) {
+#pragma unroll
+            for (int r = 0; r < WMAX; ++r) {
+                if ((uint32_t)r < W && !L.done) {
+                    const uint32_t t = bs + (uint32_t)r;
+                    uint64_t Hc;
+                    uint32_t Pc;
+                    if (r == 0) {
+                        Hc = sv[0];
+                        Pc = bs + sp[0];
+                    } else if (sv[r] < pv) {
+                        Hc = sv[r];
+                        Pc = bs + sp[r];
+                    } else {
+                        Hc = pv;
+                        Pc = bs + W + pp;
+                    }
+                    sk_kmer(L, a, S, t, Hc, Pc);
+                    if (!L.done) {
+                        sk_roll(L, tf, tr);
+                        const uint64_t h = L.f ^ L.rv;
+                        sv[r] = h;
+                        if (r == 0 || h < pv) {
+                            pv = h;
+                            pp = (uint32_t)r;
+                        }
                     }
                 }
             }
+            if (!L.done) {
+                sk_backward<WMAX>(sv, sp, W);
+                bs += W;
+            }
         }
-        if (!L.done) {
-            sk_backward<WMAX>(sv, sp, W);
-            bs += W;
+        if (L.flagged) {
+            const uint32_t slot = atomicAdd(a.flag_count, 1u);
+            a.flag_list[slot] = s;
         }
+        tail = L.tail;
+        tail_rec = make_uint2((uint32_t)L.cb + L.t0, (L.K - L.t0 + a.k - 1u) |
+                                                         ((L.P0 - L.t0) << REC_MINPOS_SHIFT) |
+                                                         (L.hsel << REC_HSEL_SHIFT));
     }
-    a.nsk[s] = L.flagged ? 0u : L.nrec;
-    if (L.flagged) {
-        const uint32_t slot = atomicAdd(a.flag_count, 1u);
-        a.flag_list[slot] = s;
+    // every lane of the wave is here again: the chunk-end records, then flush
+    sk_push(S, tail, tail_rec);
+    const uint32_t left = s_wn[wv];
+    if (left) sk_flush(S, left);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.run_ptr[a.run_base + blockIdx.x] = S.run;
+        a.run_cnt[a.run_base + blockIdx.x] = s_wgn;
     }
 }
 
 void launch_superkmers(hipStream_t st, const SkArgs& a)
 {
     if (a.n == 0) return;
-    const unsigned blocks = (a.n + 255u) / 256u;
-    if (a.W <= 8) k_superkmers<8><<<blocks, 256, 0, st>>>(a);
-    else if (a.W <= 16) k_superkmers<16><<<blocks, 256, 0, st>>>(a);
-    else if (a.W <= 32) k_superkmers<32><<<blocks, 256, 0, st>>>(a);
-    else k_superkmers<64><<<blocks, 256, 0, st>>>(a);
+    const unsigned blocks = (a.n + SK_BLOCK - 1) / SK_BLOCK;
+    if (a.W <= 8) k_superkmers<8><<<blocks, SK_BLOCK, 0, st>>>(a);
+    else if (a.W <= 16) k_superkmers<16><<<blocks, SK_BLOCK, 0, st>>>(a);
+    else if (a.W <= 32) k_superkmers<32><<<blocks, SK_BLOCK, 0, st>>>(a);
+    else k_superkmers<64><<<blocks, SK_BLOCK, 0, st>>>(a);
 }
 
 // =========================================================== routing ====
+// Records are read run by run; a tile is a fixed range of the concatenated
+// run space.  Histograms live in LDS; every tile writes one row of the
+// [tile][sub-partition] matrix.
 constexpr uint32_t ROUTE_T = 1024;
+
+constexpr uint32_t ROUTE_RUNS = 1024;  // run descriptors cached per tile (LDS: 16 KiB)
+
+// Calls f(record) for every record of this block's tile.  The tile's run
+// descriptors are cached in LDS; each wave walks 64*U consecutive records at
+// a time and every lane finds its run by a short forward scan.
+template <class F>
+__device__ __forceinline__ void for_tile_records(const RouteArgs& a, uint64_t* s_off, uint2** s_ptr,
+                                                 uint32_t* s_meta, F&& f)
+{
+    const uint64_t lo = (uint64_t)blockIdx.x * a.tile;
+    const uint64_t hi = min(lo + a.tile, a.n_rec);
+    if (threadIdx.x == 0) {
+        uint32_t l = 0, h = a.n_runs;
+        while (l < h) {  // first run with run_off[r+1] > lo
+            const uint32_t mid = (l + h) >> 1;
+            if (a.run_off[mid + 1] > lo) h = mid;
+            else l = mid + 1;
+        }
+        uint32_t e = l;
+        while (e < a.n_runs && a.run_off[e] < hi && e - l < ROUTE_RUNS - 1) ++e;
+        s_meta[0] = l;
+        s_meta[1] = e - l;  // runs cached
+        s_meta[2] = (e < a.n_runs && a.run_off[e] < hi) ? 1u : 0u;  // more runs than cached
+    }
+    __syncthreads();
+    const uint32_t r0 = s_meta[0], nr = s_meta[1];
+    for (uint32_t i = threadIdx.x; i <= nr; i += blockDim.x) {
+        s_off[i] = a.run_off[r0 + i];
+        if (i < nr) s_ptr[i] = a.run_ptr[r0 + i] - a.run_off[r0 + i];
+    }
+    __syncthreads();
+    if (s_meta[2]) {
+        // many tiny runs: walk them one by one
+        for (uint32_t r = r0; r < a.n_runs && a.run_off[r] < hi; ++r) {
+            const uint64_t b = max(a.run_off[r], lo), e = min(a.run_off[r + 1], hi);
+            const uint2* p = a.run_ptr[r] - a.run_off[r];
+            for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) f(p[i]);
+        }
+        return;
+    }
+    constexpr uint32_t U = 8;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    uint32_t rr = 0;  // run of this lane's last record (non-decreasing)
+    for (uint64_t base = lo + (uint64_t)wv * 64u * U; base < hi; base += (uint64_t)nwv * 64u * U) {
+        uint2 v[U];
+        bool ok[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint64_t i = base + u * 64u + lane;
+            ok[u] = i < hi;
+            if (ok[u]) {
+                while (s_off[rr + 1] <= i) ++rr;
+                v[u] = s_ptr[rr][i];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (ok[u]) f(v[u]);
+    }
+}
 
 __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
 {
     extern __shared__ uint32_t sh[];  // [0, nsp): superkmers, [nsp, 2nsp): k-mers
+    __shared__ uint64_t s_off[ROUTE_RUNS + 1];
+    __shared__ uint2* s_ptr[ROUTE_RUNS];
+    __shared__ uint32_t s_meta[4];
     const uint32_t nsp = a.nsp;
     for (uint32_t i = threadIdx.x; i < 2u * nsp; i += blockDim.x) sh[i] = 0;
     __syncthreads();
-    const uint32_t s0 = blockIdx.x * a.spt;
-    const uint32_t s1 = min(s0 + a.spt, a.n_strips);
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
-    for (uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-        const uint32_t n = a.nsk[s];
-        const uint2* r = a.rec + a.strip_slot[s];
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t y = r[i].y;
-            const uint32_t pid = y >> shift;
-            atomicAdd(&sh[pid], 1u);
-            atomicAdd(&sh[nsp + pid], (y & REC_LEN_MASK) - a.k + 1u);
-        }
-    }
+    const uint32_t k1 = a.k - 1u;
+    for_tile_records(a, s_off, s_ptr, s_meta, [&](uint2 v) {
+        const uint32_t pid = v.y >> shift;
+        atomicAdd(&sh[pid], 1u);
+        atomicAdd(&sh[nsp + pid], (v.y & REC_LEN_MASK) - k1);
+    });
     __syncthreads();
     uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * nsp;
     uint32_t* hk = a.hkm + (uint64_t)blockIdx.x * nsp;
@@ -469,19 +607,38 @@ void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles)
     k_route_hist<<<ntiles, ROUTE_T, 2u * a.nsp * sizeof(uint32_t), st>>>(a);
 }
 
+// Per sub-partition: tile-exclusive offsets (in place) and totals.  Eight
+// tiles per step keep eight loads in flight per thread.
 __global__ __launch_bounds__(256) void k_route_colscan(RouteArgs a, uint32_t ntiles,
                                                      uint64_t* __restrict__ part_sk,
                                                      uint64_t* __restrict__ part_km)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nsp) return;
+    uint32_t* __restrict__ hs = a.hsk;
+    const uint32_t* __restrict__ hk = a.hkm;
     uint64_t run = 0, km = 0;
-    for (uint32_t t = 0; t < ntiles; ++t) {
+    uint32_t t = 0;
+    for (; t + 8 <= ntiles; t += 8) {
+        uint32_t v[8], w[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v[j] = hs[(uint64_t)(t + j) * a.nsp + p];
+            w[j] = hk[(uint64_t)(t + j) * a.nsp + p];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hs[(uint64_t)(t + j) * a.nsp + p] = (uint32_t)run;
+            run += v[j];
+            km += w[j];
+        }
+    }
+    for (; t < ntiles; ++t) {
         const uint64_t i = (uint64_t)t * a.nsp + p;
-        const uint32_t v = a.hsk[i];
-        a.hsk[i] = (uint32_t)run;  // tile-exclusive offset inside partition p
+        const uint32_t v = hs[i];
+        hs[i] = (uint32_t)run;
         run += v;
-        km += a.hkm[i];
+        km += hk[i];
     }
     part_sk[p] = run;
     part_km[p] = km;
@@ -493,33 +650,73 @@ void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     k_route_colscan<<<(a.nsp + 255u) / 256u, 256, 0, st>>>(a, ntiles, part_sk, part_km);
 }
 
-__global__ __launch_bounds__(ROUTE_T) void k_route_scatter(RouteArgs a)
+// Scatter, level 1: every tile writes its records into coarse buckets of F
+// consecutive sub-partitions.  A tile's runs are long (tile/nc records) and
+// the lines being filled at once fit in L2, so partial lines merge there.
+__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint2* tmp)
 {
-    extern __shared__ uint32_t cur[];  // nsp cursors
-    const uint32_t nsp = a.nsp;
-    const uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * nsp;
-    for (uint32_t i = threadIdx.x; i < nsp; i += blockDim.x) cur[i] = (uint32_t)a.part_off[i] + hs[i];
+    extern __shared__ uint32_t cur[];  // nc cursors
+    __shared__ uint64_t s_off[ROUTE_RUNS + 1];
+    __shared__ uint2* s_ptr[ROUTE_RUNS];
+    __shared__ uint32_t s_meta[4];
+    const uint32_t nc = a.nsp >> log2f, F = 1u << log2f;
+    const uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * a.nsp;
+    for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        uint32_t o = (uint32_t)a.part_off[c * F];
+        for (uint32_t f = 0; f < F; ++f) o += hs[c * F + f];
+        cur[c] = o;
+    }
     __syncthreads();
-    const uint32_t s0 = blockIdx.x * a.spt;
-    const uint32_t s1 = min(s0 + a.spt, a.n_strips);
+    const uint32_t shift = REC_HSEL_SHIFT + a.sb + log2f;
+    for_tile_records(a, s_off, s_ptr, s_meta, [&](uint2 v) {
+        const uint32_t pos = atomicAdd(&cur[v.y >> shift], 1u);
+        tmp[pos] = v;
+    });
+}
+
+// Scatter, level 2: one workgroup per coarse bucket reads its region
+// (contiguous) and places records by sub-partition inside the same region.
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint2* tmp)
+{
+    extern __shared__ uint32_t cur[];  // F cursors
+    const uint32_t F = 1u << log2f, c = blockIdx.x;
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[c * F + f];
+    __syncthreads();
+    const uint64_t b = a.part_off[c * F], e = a.part_off[(c + 1) * F];
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
-    for (uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-        const uint32_t n = a.nsk[s];
-        const uint2* r = a.rec + a.strip_slot[s];
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint2 v = r[i];
-            const uint32_t pos = atomicAdd(&cur[v.y >> shift], 1u);
-            a.dst[pos] = v;
+    const uint32_t fm = F - 1u;
+    constexpr uint32_t U = 8;
+    uint64_t i = b + threadIdx.x;
+    for (; i + (U - 1) * (uint64_t)blockDim.x < e; i += U * (uint64_t)blockDim.x) {
+        uint2 v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) v[u] = tmp[i + u * blockDim.x];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t pos = atomicAdd(&cur[(v[u].y >> shift) & fm], 1u);
+            a.dst[pos] = v[u];
         }
+    }
+    for (; i < e; i += blockDim.x) {
+        const uint2 v = tmp[i];
+        const uint32_t pos = atomicAdd(&cur[(v.y >> shift) & fm], 1u);
+        a.dst[pos] = v;
     }
 }
 
-void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles)
+void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint2* tmp)
 {
     if (ntiles == 0) return;
-    (void)hipFuncSetAttribute((const void*)k_route_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(a.nsp * sizeof(uint32_t)));
-    k_route_scatter<<<ntiles, ROUTE_T, a.nsp * sizeof(uint32_t), st>>>(a);
+    // coarse buckets: up to 256, F = nsp / nc sub-partitions each
+    uint32_t log2f = 0;
+    while ((a.nsp >> log2f) > 256u) ++log2f;
+    const uint32_t nc = a.nsp >> log2f;
+    (void)hipFuncSetAttribute((const void*)k_route_coarse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(nc * sizeof(uint32_t)));
+    k_route_coarse<<<ntiles, ROUTE_T, nc * sizeof(uint32_t), st>>>(a, log2f, tmp);
+    (void)hipFuncSetAttribute((const void*)k_route_fine, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((1u << log2f) * sizeof(uint32_t)));
+    k_route_fine<<<nc, ROUTE_T, (1u << log2f) * sizeof(uint32_t), st>>>(a, log2f, tmp);
 }
 
 // ============================================================== fill ====
@@ -552,7 +749,7 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n)
 constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 
 // Global spill table, one 64-bit word per key: claim by CAS on the key.
-__device__ bool spill_insert1(const CountArgs& a, uint64_t x)
+__device__ __noinline__ bool spill_insert1(const CountArgs& a, uint64_t x)
 {
     uint64_t slot = mix64(x ^ 0x9e3779b97f4a7c15ull) & a.sp_mask;
     for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16); ++p) {
@@ -577,7 +774,7 @@ __device__ bool spill_insert1(const CountArgs& a, uint64_t x)
 
 // Global spill table, two words per key: the count word is the slot state
 // (0 empty, LOCKED32 being written, otherwise published count).
-__device__ bool spill_insert2(const CountArgs& a, uint64_t hi, uint64_t lo)
+__device__ __noinline__ bool spill_insert2(const CountArgs& a, uint64_t hi, uint64_t lo)
 {
     uint64_t slot = mix64(lo ^ mix64(hi ^ 0x9e3779b97f4a7c15ull)) & a.sp_mask;
     for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16);) {
@@ -606,31 +803,33 @@ __device__ bool spill_insert2(const CountArgs& a, uint64_t hi, uint64_t lo)
     return false;
 }
 
+// Slot of a key in an LDS table: multiplicative hash of the folded key (slot
+// placement never affects results).
+__device__ __forceinline__ uint32_t slot_hash(uint64_t x)
+{
+    const uint32_t h = (uint32_t)x ^ (uint32_t)(x >> 29);
+    return (h * 0x9E3779B1u) ^ ((h * 0x85EBCA77u) >> 15);
+}
+
 // LDS table, one word per key.  Slots move EMPTY -> key exactly once, so a key
 // absent from a full probe window can never enter the table later: spilling
 // it keeps every occurrence of a key in exactly one table.
 __device__ __forceinline__ bool lds_insert1(uint64_t* keys, uint32_t* cnt, uint32_t mask,
                                             uint64_t x, uint32_t probe_max)
 {
-    uint32_t slot = (uint32_t)mix64(x) & mask;
-    for (uint32_t p = 0; p < probe_max; ++p) {
-        const uint64_t cur = __hip_atomic_load((unsigned long long*)&keys[slot], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (cur == x) {
-            atomicAdd(&cnt[slot], 1u);
-            return true;
-        }
-        if (cur == EMPTY64) {
-            const uint64_t old = atomicCAS((unsigned long long*)&keys[slot],
-                                           (unsigned long long)EMPTY64, (unsigned long long)x);
-            if (old == EMPTY64 || old == x) {
-                atomicAdd(&cnt[slot], 1u);
-                return true;
-            }
-        }
+    // one compare-and-swap per probe: it claims an empty slot or returns the
+    // key already there
+    uint32_t slot = slot_hash(x) & mask;
+    uint32_t p = 0;
+    for (;;) {
+        const uint64_t old = atomicCAS((unsigned long long*)&keys[slot], (unsigned long long)EMPTY64,
+                                       (unsigned long long)x);
+        if (old == EMPTY64 || old == x) break;
+        if (++p == probe_max) return false;
         slot = (slot + 1) & mask;
     }
-    return false;
+    atomicAdd(&cnt[slot], 1u);
+    return true;
 }
 
 // LDS table, two words per key, count word as slot state.
@@ -638,7 +837,7 @@ __device__ __forceinline__ bool lds_insert2(uint64_t* khi, uint64_t* klo, uint32
                                             uint32_t mask, uint64_t hi, uint64_t lo,
                                             uint32_t probe_max)
 {
-    uint32_t slot = (uint32_t)mix64(lo ^ mix64(hi)) & mask;
+    uint32_t slot = slot_hash(lo ^ (hi * 0x9e3779b97f4a7c15ull)) & mask;
     for (uint32_t p = 0; p < probe_max;) {
         const uint32_t c = __hip_atomic_load(&cnt[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (c == 0u) {
@@ -697,9 +896,96 @@ __device__ __forceinline__ void emit_out(const CountArgs& a, uint64_t pos, uint6
     a.out_cnt[pos] = c;
 }
 
-// One workgroup per work item.  Waves flatten 64 records at a time into a
-// k-mer index space (wave scan of k-mer counts) and every lane takes k-mers
-// lane, lane+64, ... so superkmers of any length load-balance.
+// Rolling k-mer of one lane over words staged in LDS: forward and reverse
+// complement plus a reader positioned at the next base.
+template <int NW>
+struct KRoll {
+    uint64_t fh, fl, rh, rl;   // forward / reverse complement (hi used when NW == 2)
+    uint32_t wi;               // next word (stage index)
+    uint32_t wrd, rem;
+};
+
+// 2k-bit k-mer whose first base is at bit offset 2*sh of stage word wi.
+template <int NW>
+__device__ __forceinline__ void kroll_init(KRoll<NW>& R, const uint32_t* src, uint32_t wi,
+                                           uint32_t sh, uint32_t k)
+{
+    const uint32_t bsh = 2u * sh;
+    uint64_t a0 = ((uint64_t)src[wi] << 32) | src[wi + 1];
+    const uint32_t c2 = src[wi + 2];
+    if (NW == 1) {
+        const uint64_t top = bsh ? (a0 << bsh) | ((uint64_t)c2 >> (32u - bsh)) : a0;
+        R.fl = top >> (64u - 2u * k);
+        R.fh = 0;
+        R.rl = (~rev_pairs64(R.fl)) >> (64u - 2u * k);
+        R.rh = 0;
+    } else {
+        uint64_t b0 = ((uint64_t)c2 << 32) | src[wi + 3];
+        const uint32_t c4 = src[wi + 4];
+        if (bsh) {
+            a0 = (a0 << bsh) | (b0 >> (64u - bsh));
+            b0 = (b0 << bsh) | ((uint64_t)c4 >> (32u - bsh));
+        }
+        const uint32_t r = 128u - 2u * k;
+        if (r == 0) { R.fh = a0; R.fl = b0; }
+        else { R.fl = (b0 >> r) | (a0 << (64u - r)); R.fh = a0 >> r; }
+        uint64_t rh = ~rev_pairs64(R.fl), rl = ~rev_pairs64(R.fh);
+        if (r) { rl = (rl >> r) | (rh << (64u - r)); rh >>= r; }
+        R.rh = rh;
+        R.rl = rl;
+    }
+    const uint32_t q = sh + k;  // next base, relative to word wi
+    R.wi = wi + (q >> 4);
+    R.wrd = src[R.wi] << (2u * (q & 15u));
+    R.rem = 16u - (q & 15u);
+}
+
+template <int NW>
+__device__ __forceinline__ void kroll_step(KRoll<NW>& R, const uint32_t* src, uint32_t k)
+{
+    {
+        const uint32_t nxt = src[R.wi + 1u];
+        const bool ld = R.rem == 0;
+        R.wrd = ld ? nxt : R.wrd;
+        R.wi += ld ? 1u : 0u;
+        R.rem = ld ? 16u : R.rem;
+    }
+    const uint64_t b = R.wrd >> 30;
+    R.wrd <<= 2;
+    --R.rem;
+    const uint64_t cb = 3u - b;
+    if (NW == 1) {
+        const uint64_t mask = k == 32 ? ~0ull : ((1ull << (2u * k)) - 1u);
+        R.fl = ((R.fl << 2) | b) & mask;
+        R.rl = (R.rl >> 2) | (cb << (2u * k - 2u));
+    } else {
+        const uint32_t hb = 2u * k - 64u;  // valid bits of the high word
+        const uint64_t hmask = hb == 64 ? ~0ull : ((1ull << hb) - 1u);
+        R.fh = ((R.fh << 2) | (R.fl >> 62)) & hmask;
+        R.fl = (R.fl << 2) | b;
+        R.rl = (R.rl >> 2) | (R.rh << 62);
+        R.rh = (R.rh >> 2) | (cb << (hb - 2u));
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
+{
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
+    }
+    return v;
+}
+
+constexpr uint32_t COUNT_STAGE = 512;  // packed words staged per wave
+
+// One workgroup per work item.  A wave takes 64 records at a time from a
+// workgroup dispenser, copies the packed words they cover into its LDS stage
+// (one round of global loads), and then their k-mers form a flat index space
+// (wave scan) cut into 64 equal pieces, one per lane.  A lane finds its first
+// record once (binary search) and rolls through consecutive k-mers from LDS,
+// re-seeding only when it crosses into the next record.
 template <int NW, uint32_t CAP>
 __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 {
@@ -707,9 +993,13 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ uint64_t s_k0[CAP];
     __shared__ uint64_t s_k1[NW == 2 ? CAP : 1];
     __shared__ uint32_t s_cnt[CAP];
+    __shared__ uint32_t s_stage[NWAVE][COUNT_STAGE];
     __shared__ uint32_t s_incl[NWAVE][64];
-    __shared__ uint32_t s_start[NWAVE][64];
+    __shared__ uint32_t s_wb[NWAVE][64];   // stage word of the record's first base word
+    __shared__ uint32_t s_st[NWAVE][64];   // first base of the record (global index)
+    __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint32_t s_red[NWAVE];
+    __shared__ uint32_t s_next;
     __shared__ unsigned long long s_base;
 
     const uint4 it = a.items[blockIdx.x];
@@ -717,60 +1007,127 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         if (NW == 1) s_k0[i] = EMPTY64;
         s_cnt[i] = 0;
     }
+    if (threadIdx.x == 0) s_next = it.x;
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t k = a.k;
-    const uint32_t r0 = it.x, r1 = it.y, smask = it.z, sval = it.w;
-    const uint32_t split_mask_bits = 0;  // split bits sit at REC_HSEL_SHIFT
-    (void)split_mask_bits;
+    const uint32_t r1 = it.y, smask = it.z, sval = it.w;
+    const uint32_t pad = NW == 1 ? 3u : 5u;  // words read past a record's last base word
+    const uint32_t* stage = s_stage[wv];
     uint32_t spilled = 0;
-    for (uint32_t rb = r0 + wv * 64u; rb < r1; rb += NWAVE * 64u) {
+    for (;;) {
+        uint32_t rb = 0;
+        if (lane == 0) rb = atomicAdd(&s_next, 64u);
+        rb = __shfl(rb, 0, 64);
+        if (rb >= r1) break;
         const uint32_t i = rb + lane;
-        uint32_t nk = 0, st = 0;
+        uint32_t nk = 0, st = 0, len = 0;
         if (i < r1) {
             const uint2 rr = a.rec[i];
             if (((rr.y >> REC_HSEL_SHIFT) & smask) == sval) {
-                nk = (rr.y & REC_LEN_MASK) - k + 1u;
+                len = rr.y & REC_LEN_MASK;
+                nk = len - k + 1u;
                 st = rr.x;
             }
         }
-        uint32_t incl = nk;
-#pragma unroll
-        for (uint32_t d = 1; d < 64; d <<= 1) {
-            const uint32_t y = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += y;
-        }
-        s_incl[wv][lane] = incl;
-        s_start[wv][lane] = st;
-        const uint32_t total = __shfl(incl, 63, 64);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t j = lane; j < total; j += 64u) {
-            uint32_t lo = 0;
-#pragma unroll
-            for (uint32_t step = 32; step; step >>= 1)
-                if (s_incl[wv][lo + step - 1u] <= j) lo += step;
-            const uint32_t ex = lo ? s_incl[wv][lo - 1u] : 0u;
-            const uint64_t g = (uint64_t)s_start[wv][lo] + (j - ex);
-            if (NW == 1) {
-                const uint64_t x = canon64(load_kmer64(a.packed, g, k), k);
-                if (!lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max)) {
-                    spill_insert1(a, x);
-                    ++spilled;
+        bool pending = nk > 0;
+        while (__ballot(pending)) {
+            const uint32_t w0 = st >> 4;
+            const uint32_t nwd = pending ? ((st + len - 1u) >> 4) - w0 + pad : 0u;
+            const uint32_t wincl = wave_incl_u32(nwd, lane);
+            const bool take = pending && wincl <= COUNT_STAGE;
+            const uint32_t wex = wincl - nwd;
+            if (take) {
+                const uint32_t* g = a.packed + w0;
+                uint32_t j = 0;
+                for (; j + 4 <= nwd; j += 4) {
+                    const uint32_t v0 = g[j], v1 = g[j + 1], v2 = g[j + 2], v3 = g[j + 3];
+                    s_stage[wv][wex + j] = v0;
+                    s_stage[wv][wex + j + 1] = v1;
+                    s_stage[wv][wex + j + 2] = v2;
+                    s_stage[wv][wex + j + 3] = v3;
                 }
-            } else {
-                uint64_t hi, lo2;
-                load_kmer128(a.packed, g, k, hi, lo2);
-                canon128(hi, lo2, k);
-                if (!lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, hi, lo2, a.probe_max)) {
-                    spill_insert2(a, hi, lo2);
-                    ++spilled;
+                for (; j < nwd; ++j) s_stage[wv][wex + j] = g[j];
+            }
+            const uint32_t incl = wave_incl_u32(take ? nk : 0u, lane);
+            s_incl[wv][lane] = incl;
+            s_wb[wv][lane] = wex;
+            s_st[wv][lane] = st;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                // seed of this lane's own record, computed by every lane at once so
+                // that crossing into a record later is two LDS reads
+                KRoll<NW> Q;
+                kroll_init<NW>(Q, stage, take ? wex : 0u, take ? (st & 15u) : 0u, k);
+                s_seed[wv][lane][0] = Q.fl;
+                s_seed[wv][lane][1] = Q.rl;
+                if (NW == 2) {
+                    s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
+                    s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
                 }
             }
+            const uint32_t total = __shfl(incl, 63, 64);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t q = (total + 63u) >> 6;
+            const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
+            if (j0 < j1) {
+                uint32_t s = 0;
+#pragma unroll
+                for (uint32_t step = 32; step; step >>= 1)
+                    if (s_incl[wv][s + step - 1u] <= j0) s += step;
+                const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
+                uint32_t end = s_incl[wv][s];
+                KRoll<NW> R;
+                {
+                    const uint32_t rs = s_st[wv][s], gb = rs + (j0 - ex);
+                    kroll_init<NW>(R, stage, s_wb[wv][s] + (gb >> 4) - (rs >> 4), gb & 15u, k);
+                }
+                for (uint32_t j = j0;;) {
+                    bool ok;
+                    if (NW == 1) {
+                        const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
+                        ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
+                        if (!ok) spill_insert1(a, x);
+                    } else {
+                        const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
+                        const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
+                        ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
+                        if (!ok) spill_insert2(a, xh, xl);
+                    }
+                    spilled += ok ? 0u : 1u;
+                    if (++j == j1) break;
+                    if (j == end) {
+                        // next record with k-mers (records not taken this round have none)
+                        do {
+                            ++s;
+                            end = s_incl[wv][s];
+                        } while (end == j);
+                        const uint32_t rs = s_st[wv][s];
+                        R.fl = s_seed[wv][s][0];
+                        R.rl = s_seed[wv][s][1];
+                        if (NW == 2) {
+                            R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
+                            R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
+                        }
+                        const uint32_t q2 = (rs & 15u) + k;
+                        R.wi = s_wb[wv][s] + (q2 >> 4);
+                        R.wrd = stage[R.wi] << (2u * (q2 & 15u));
+                        R.rem = 16u - (q2 & 15u);
+                    } else {
+                        kroll_step<NW>(R, stage, k);
+                    }
+                }
+            }
+            pending = pending && !take;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
 
@@ -791,6 +1148,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         s_base = tot_out ? atomicAdd(&a.counters[0], (unsigned long long)tot_out) : 0ull;
         if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
         if (tot_sp) atomicAdd(&a.counters[2], (unsigned long long)tot_sp);
+        if (a.item_uniq) a.item_uniq[blockIdx.x] = tot_uniq + tot_sp;
     }
     __syncthreads();
     uint64_t pos = s_base + my_off;
