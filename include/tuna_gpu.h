@@ -134,6 +134,29 @@ int tg_partitions_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
 int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases,
                         uint64_t* out_off, uint64_t off_cap);
 
+/* Read a whole input file into a malloc'd buffer (gzip-compressed when the
+ * name ends in ".gz", as src/seq_source.hpp:196-204).  Errors follow the
+ * reference messages: "Cannot open file: <path>", "Empty file: <path>". */
+int tg_read_file(const char* path, uint8_t** data, uint64_t* len);
+void tg_free(void* p);
+
+/* Reference auto-tuned partition count (src/setup.hpp:82-196): depends on the
+ * host's L2/L3 sizes, RLIMIT_NOFILE, the input sizes, k and threads.  Not yet
+ * rounded to a power of two (src/main.cpp:47-52 does that). */
+uint32_t tg_auto_partitions(const char* const* files, uint32_t n_files, uint32_t k,
+                            uint32_t threads);
+
+/* ---- output formats (A10) ---------------------------------------------- */
+/* TSV lines "<k-mer>\t<count>\n" (src/count.hpp:180-216) for n pairs; returns
+ * bytes written or TG_ECAP.  cap >= n*(k+12) always suffices. */
+int64_t tg_format_tsv(uint32_t k, const uint64_t* keys, const uint32_t* counts, uint64_t n,
+                      char* out, uint64_t cap);
+/* KFF file with one raw section (src/kff_output.hpp:20-57): encoding A0 C1 G2 T3,
+ * unique, canonical, k / max=1 / data_size=4, each k-mer as ceil(k/4) bytes
+ * MSB-first followed by its count as 4 big-endian bytes. */
+int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint32_t* counts,
+                 uint64_t n);
+
 /* ---- synthetic inputs (seeded, deterministic) ------------------------- */
 /* Generate the FASTA/FASTQ text of a named workload into a library-owned
  * buffer: "genome_fa" (uniform genome), "reads_fq" (short reads from a uniform

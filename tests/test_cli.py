@@ -1,0 +1,307 @@
+"""The `tuna` CLI and the C++ library header (include/tuna/tuna.hpp).
+
+CPU tests: argument validation (exit codes and messages of src/CLI.cpp),
+output formatters, gzip input.  GPU tests: end-to-end counts against the
+oracle, KFF output, @list input, -tp, and -kt partition files against the
+reference's own partition bytes (tests/golden)."""
+import gzip
+import json
+import os
+import random
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tuna_amd as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CLI = os.path.join(ROOT, "tuna_amd", "tuna")
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))
+
+
+def run_cli(*args, cwd=None):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=300)
+
+
+def rs(rng, n, alph="ACGT"):
+    return "".join(rng.choice(alph) for _ in range(n))
+
+
+# ------------------------------------------------------------------ CPU ----
+@pytest.mark.parametrize("args,msg", [
+    (["-k", "300", "a", "b"], "k-mer length (-k 300) must be in [2, 256]"),
+    (["-k", "31", "-m", "31", "a", "b"], "must be strictly less than k-mer length"),
+    (["-m", "40", "a", "b"], "minimizer length (-m 40) must be in [1, 32]"),
+    (["-t", "0", "a", "b"], "number of threads (-t) must be > 0"),
+    (["-x", "a", "b"], "unknown option: -x"),
+    (["only_one"], "need at least one input file and an output file"),
+    (["-k", "abc", "a", "b"], "invalid integer for -k: abc"),
+    (["-k"], "-k requires an argument"),
+    (["@/nonexistent/list", "out"], "cannot open input list"),
+])
+def test_cli_rejects_bad_arguments(args, msg):
+    r = run_cli(*args)
+    assert r.returncode == 1
+    assert "tuna: error: " in r.stderr and msg in r.stderr
+
+
+def test_cli_help_and_version():
+    r = run_cli("--help")
+    assert r.returncode == 1 and "Usage:" in r.stderr
+    r = run_cli("-v")
+    assert r.returncode == 0 and r.stderr.startswith("tuna 1.4.0")
+
+
+def test_cli_rejects_k_above_engine_limit(tmp_path):
+    f = tmp_path / "x.fa"
+    f.write_text(">a\nACGT\n")
+    r = run_cli("-k", "127", "-m", "21", str(f), str(tmp_path / "o.tsv"))
+    assert r.returncode == 1 and "unsupported combination k=127 m=21" in r.stderr
+
+
+def test_tsv_formatter_matches_oracle():
+    rng = random.Random(3)
+    for k in (5, 31, 32, 33, 63):
+        nw = 2 if k > 32 else 1
+        keys, counts, lines = [], [], []
+        for _ in range(50):
+            key = rng.getrandbits(2 * k)
+            c = rng.randint(1, 10**9)
+            keys.append([key & (2**64 - 1), key >> 64][:nw])
+            counts.append(c)
+            lines.append(O.tsv_line(key, k, c))
+        ka = np.array(keys, dtype=np.uint64).reshape(-1)
+        ca = np.array(counts, dtype=np.uint32)
+        buf = np.zeros(50 * (k + 12), dtype=np.uint8)
+        n = T.lib().tg_format_tsv(k, ka.ctypes.data_as(T._u64p), ca.ctypes.data_as(T._u32p), 50,
+                                  buf.ctypes.data_as(T.ctypes.c_char_p), len(buf))
+        assert bytes(buf[:n]).decode() == "".join(lines)
+
+
+def parse_kff(path):
+    """Minimal KFF reader for files written by tg_write_kff."""
+    d = open(path, "rb").read()
+    assert d[:3] == b"KFF" and d[-3:] == b"KFF"
+    assert d[5] == 0x1B and d[6] == 1 and d[7] == 1
+    p = 8
+    msz = struct.unpack(">I", d[p:p + 4])[0]
+    p += 4 + msz
+    assert d[p:p + 1] == b"v"
+    nv = struct.unpack(">Q", d[p + 1:p + 9])[0]
+    p += 9
+    var = {}
+    for _ in range(nv):
+        e = d.index(b"\0", p)
+        var[d[p:e].decode()] = struct.unpack(">Q", d[e + 1:e + 9])[0]
+        p = e + 9
+    assert d[p:p + 1] == b"r"
+    nb = struct.unpack(">Q", d[p + 1:p + 9])[0]
+    p += 9
+    k = var["k"]
+    kb = (k + 3) // 4
+    out = {}
+    for _ in range(nb):
+        seq = d[p:p + kb]
+        cnt = struct.unpack(">I", d[p + kb:p + kb + 4])[0]
+        p += kb + 4
+        lab = "".join("ACGT"[(seq[i // 4] >> (6 - 2 * (i % 4))) & 3] for i in range(k))
+        out[lab] = cnt
+    assert var["max"] == 1 and var["data_size"] == 4
+    return k, out
+
+
+def test_kff_writer_round_trip(tmp_path):
+    rng = random.Random(4)
+    for k in (7, 31, 33, 63):
+        nw = 2 if k > 32 else 1
+        exp, keys, counts = {}, [], []
+        for _ in range(40):
+            key = rng.getrandbits(2 * k)
+            c = rng.randint(1, 2**32 - 1)
+            keys.append([key & (2**64 - 1), key >> 64][:nw])
+            counts.append(c)
+            exp[O.tsv_line(key, k, 1).split("\t")[0]] = c
+        ka = np.array(keys, dtype=np.uint64).reshape(-1)
+        ca = np.array(counts, dtype=np.uint32)
+        path = str(tmp_path / f"o{k}.kff")
+        assert T.lib().tg_write_kff(path.encode(), k, ka.ctypes.data_as(T._u64p),
+                                    ca.ctypes.data_as(T._u32p), 40) == 0
+        kk, got = parse_kff(path)
+        assert kk == k and got == exp
+
+
+def test_gzip_input_reads_like_plain(tmp_path):
+    text = GOLD["inputs"]["fastq_reads"].encode()
+    p = tmp_path / "r.fq.gz"
+    with gzip.open(p, "wb") as f:
+        f.write(text)
+    data = T.ctypes.POINTER(T.ctypes.c_uint8)()
+    n = T.ctypes.c_uint64(0)
+    assert T.lib().tg_read_file(str(p).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) == 0
+    got = bytes(data[:n.value])
+    T.lib().tg_free(data)
+    assert got == text
+
+
+def test_empty_and_missing_inputs_report_reference_messages(tmp_path):
+    e = tmp_path / "empty.fa"
+    e.write_bytes(b"")
+    data = T.ctypes.POINTER(T.ctypes.c_uint8)()
+    n = T.ctypes.c_uint64(0)
+    assert T.lib().tg_read_file(str(e).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) != 0
+    assert T.lib().tg_last_error().decode() == f"Empty file: {e}"
+    m = tmp_path / "missing.fa"
+    assert T.lib().tg_read_file(str(m).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) != 0
+    assert T.lib().tg_last_error().decode() == f"Cannot open file: {m}"
+
+
+def test_auto_partitions_bounds(tmp_path):
+    f = tmp_path / "g.fa"
+    f.write_text(">a\n" + "ACGT" * 1000000 + "\n")
+    arr = (T.ctypes.c_char_p * 1)(str(f).encode())
+    T.lib().tg_auto_partitions.restype = T.ctypes.c_uint32
+    n = T.lib().tg_auto_partitions(arr, 1, 31, 1)
+    assert 16 <= n <= 32768 and n & (n - 1) == 0
+
+
+# ------------------------------------------------------------------ GPU ----
+def oracle_tsv(files_text, k, ci=1, cx=2**64 - 1):
+    bs, offs, base = [], [], 0
+    for t in files_text:
+        b, o = O.parse(t)
+        bs.append(b)
+        offs.append(o[1:] + base)
+        base += len(b)
+    bases = np.concatenate(bs)
+    off = np.concatenate([np.zeros(1, np.uint64)] + offs)
+    keys, cnt, _, _ = O.count(bases, off, k, ci, cx)
+    out = {}
+    for i in range(len(cnt)):
+        key = int(keys[i, 0]) | (int(keys[i, 1]) << 64 if keys.shape[1] > 1 else 0)
+        out[O.tsv_line(key, k, 1).split("\t")[0]] = int(cnt[i])
+    return out
+
+
+def read_tsv(path):
+    out = {}
+    for line in open(path):
+        a, b = line.rstrip("\n").split("\t")
+        assert a not in out
+        out[a] = int(b)
+    return out
+
+
+@pytest.mark.gpu
+def test_cli_counts_match_oracle(tmp_path):
+    text = T.synth("kind=reads_fq genome=40000 seed=2 cov=15 rl=150 sub=0.005 nrate=0.0002")
+    fq = tmp_path / "r.fq"
+    fq.write_bytes(bytes(text))
+    out = tmp_path / "o.tsv"
+    r = run_cli("-k", "31", "-m", "21", "-n", "64", "-ci", "2", str(fq), str(out))
+    assert r.returncode == 0, r.stderr
+    exp = oracle_tsv([bytes(text)], 31, 2)
+    assert read_tsv(out) == exp
+    stats = dict(l.split(": ") for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+    assert int(stats["unique_kmers"]) == len(exp) and int(stats["n_parts"]) == 64
+    b, o = O.parse(bytes(text))
+    assert int(stats["superkmers"]) == len(O.superkmers(b, o, 31, 21)[0])
+
+
+@pytest.mark.gpu
+def test_cli_kff_list_gzip_and_multi_file(tmp_path):
+    rng = random.Random(6)
+    texts = [">a\n" + rs(rng, 3000) + "\n", ">b\n" + rs(rng, 2000) + "N" + rs(rng, 500) + "\n"]
+    paths = []
+    for i, t in enumerate(texts):
+        p = tmp_path / f"f{i}.fa.gz"
+        with gzip.open(p, "wb") as f:
+            f.write(t.encode())
+        paths.append(str(p))
+    lst = tmp_path / "list.txt"
+    lst.write_text("\n".join(paths) + "\n")
+    out = tmp_path / "o.kff"
+    r = run_cli("-k", "25", "-m", "11", "-hp", "@" + str(lst), str(out))
+    assert r.returncode == 0, r.stderr
+    k, got = parse_kff(str(out))
+    assert k == 25 and got == oracle_tsv([t.encode() for t in texts], 25)
+
+
+@pytest.mark.gpu
+def test_cli_partition_only_and_kept_partition_files(tmp_path):
+    name = "homopolymer_cap"
+    f = tmp_path / "h.fa"
+    f.write_text(GOLD["inputs"][name])
+    wd = tmp_path / "wd"
+    r = run_cli("-k", "31", "-m", "21", "-n", "16", "-tp", "-kt", "-w", str(wd), str(f), str(tmp_path / "o.tsv"))
+    assert r.returncode == 0, r.stderr
+    g = GOLD["records"][f"{name}|31|21|16"]
+    assert f"superkmers: {g['superkmers']}" in r.stderr
+    for p in range(16):
+        blob = (wd / f"hash_{p}.superkmers").read_bytes()
+        mine, q = [], 0
+        while q < len(blob):
+            L = blob[q]
+            mine.append(blob[q:q + 2 + (L + 3) // 4])
+            q += 2 + (L + 3) // 4
+        ref, q = [], 0
+        rb = bytes.fromhex(g["parts"][p])
+        while q < len(rb):
+            L = rb[q]
+            ref.append(rb[q:q + 2 + (L + 3) // 4])
+            q += 2 + (L + 3) // 4
+        assert sorted(mine) == sorted(ref)
+
+
+@pytest.mark.gpu
+def test_cli_empty_input_error(tmp_path):
+    e = tmp_path / "e.fa"
+    e.write_bytes(b"")
+    r = run_cli(str(e), str(tmp_path / "o.tsv"))
+    assert r.returncode == 1 and f"tuna: error: Empty file: {e}" in r.stderr
+
+
+CPP_TEST = r'''
+#include "tuna/tuna.hpp"
+#include <cstdio>
+#include <map>
+int main(int argc, char** argv) {
+    std::vector<std::string> files(argv + 1, argv + argc);
+    tuna::Options o; o.min_count = 2; o.partitions = 32;
+    auto m = tuna::count_to<31, 21>(files, o);
+    auto raw = tuna::count_to_raw<31, 21>(files, o);
+    std::map<std::string, uint32_t> sorted(m.begin(), m.end());
+    if (raw.size() != m.size()) { fprintf(stderr, "size mismatch\n"); return 2; }
+    for (auto& kv : raw) { std::string s; kv.first.get_label(s); if (m.at(s) != kv.second) return 3; }
+    for (auto& kv : sorted) printf("%s\t%u\n", kv.first.c_str(), kv.second);
+    try { tuna::count<31>({}, [](std::string_view, uint32_t) {}); return 4; }
+    catch (const std::invalid_argument& e) { fprintf(stderr, "%s\n", e.what()); }
+    return 0;
+}
+'''
+
+
+@pytest.mark.gpu
+def test_cpp_api_count_to_and_raw(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text(CPP_TEST)
+    exe = tmp_path / "t"
+    lib = os.path.join(ROOT, "tuna_amd")
+    c = subprocess.run(["g++", "-std=c++17", "-O1", str(src), "-I", os.path.join(ROOT, "include"),
+                        "-L", lib, "-ltuna_gpu", "-Wl,-rpath," + lib, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert c.returncode == 0, c.stderr
+    text = T.synth("kind=reads_fq genome=20000 seed=7 cov=10 rl=150 sub=0.005")
+    fq = tmp_path / "r.fq"
+    fq.write_bytes(bytes(text))
+    r = subprocess.run([str(exe), str(fq)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "tuna::count: no input files" in r.stderr
+    got = {}
+    for line in r.stdout.splitlines():
+        a, b = line.split("\t")
+        got[a] = int(b)
+    assert got == oracle_tsv([bytes(text)], 31, 2)

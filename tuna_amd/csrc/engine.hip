@@ -17,8 +17,12 @@
 #include <vector>
 
 namespace {
-
 thread_local std::string g_err;
+}
+
+void set_host_error(const std::string& m) { g_err = m; }
+
+namespace {
 
 int fail(int code, const char* fmt, ...)
 {

@@ -6,7 +6,13 @@
 // every byte outside [ACGTacgt] ends a chunk.
 #include "../../include/tuna_gpu.h"
 
+#include <sys/resource.h>
+#include <sys/stat.h>
+#include <zlib.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <fstream>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -320,7 +326,262 @@ std::string spec_str(const std::string& spec, const char* key, const char* dflt)
 
 }  // namespace
 
+// thread-local message shared with the engine (tg_last_error)
+void set_host_error(const std::string& m);
+
+namespace {
+
+// Cache size of a level from sysfs, as the reference's sysfs_cache_size
+// (src/setup.hpp:16-46): cpu0 index entries, unified or data, K/M suffixes.
+size_t cache_size(int level, size_t fallback)
+{
+    for (int idx = 0; idx <= 5; ++idx) {
+        const std::string base = "/sys/devices/system/cpu/cpu0/cache/index" + std::to_string(idx);
+        std::ifstream lf(base + "/level");
+        int lv = 0;
+        if (!lf || !(lf >> lv) || lv != level) continue;
+        std::ifstream tf(base + "/type");
+        std::string type;
+        if (tf && (tf >> type) && type == "Instruction") continue;
+        std::ifstream sf(base + "/size");
+        std::string sz;
+        if (!sf || !(sf >> sz) || sz.empty()) continue;
+        char* end = nullptr;
+        size_t v = strtoul(sz.c_str(), &end, 10);
+        if (*end == 'K' || *end == 'k') v <<= 10;
+        else if (*end == 'M' || *end == 'm') v <<= 20;
+        if (v) return v;
+    }
+    return fallback;
+}
+
+void put_be(std::string& o, uint64_t v, int bytes)
+{
+    for (int i = bytes - 1; i >= 0; --i) o += (char)((v >> (8 * i)) & 0xFF);
+}
+
+}  // namespace
+
 extern "C" {
+
+int tg_read_file(const char* path, uint8_t** data, uint64_t* len)
+{
+    if (!path || !data || !len) return TG_EINVAL;
+    *data = nullptr;
+    *len = 0;
+    const std::string p(path);
+    const bool gz = p.size() > 3 && p.compare(p.size() - 3, 3, ".gz") == 0;
+    if (!gz) {
+        FILE* f = fopen(path, "rb");
+        if (!f) {
+            set_host_error("Cannot open file: " + p);
+            return TG_EINPUT;
+        }
+        struct stat st {};
+        fstat(fileno(f), &st);
+        const uint64_t n = (uint64_t)st.st_size;
+        if (n == 0) {
+            fclose(f);
+            set_host_error("Empty file: " + p);
+            return TG_EINPUT;
+        }
+        uint8_t* b = (uint8_t*)malloc(n);
+        if (!b) {
+            fclose(f);
+            set_host_error("out of host memory reading " + p);
+            return TG_ENOMEM;
+        }
+        const size_t got = fread(b, 1, n, f);
+        fclose(f);
+        if (got != n) {
+            free(b);
+            set_host_error("short read: " + p);
+            return TG_EINPUT;
+        }
+        *data = b;
+        *len = n;
+        return TG_OK;
+    }
+    gzFile g = gzopen(path, "rb");
+    if (!g) {
+        set_host_error("Cannot open: " + p);
+        return TG_EINPUT;
+    }
+    gzbuffer(g, 1u << 20);
+    uint64_t cap = 1u << 24, n = 0;
+    uint8_t* b = (uint8_t*)malloc(cap);
+    for (;;) {
+        if (n == cap) {
+            cap *= 2;
+            uint8_t* nb = (uint8_t*)realloc(b, cap);
+            if (!nb) {
+                free(b);
+                gzclose(g);
+                set_host_error("out of host memory reading " + p);
+                return TG_ENOMEM;
+            }
+            b = nb;
+        }
+        const uint64_t want = std::min<uint64_t>(cap - n, 1u << 30);
+        const int r = gzread(g, b + n, (unsigned)want);
+        if (r <= 0) break;
+        n += (uint64_t)r;
+    }
+    gzclose(g);
+    if (n == 0) {
+        free(b);
+        set_host_error("Cannot read: " + p);
+        return TG_EINPUT;
+    }
+    *data = b;
+    *len = n;
+    return TG_OK;
+}
+
+void tg_free(void* p) { free(p); }
+
+// src/setup.hpp:82-196, restated: n = max(n_L3, n_throughput) capped by the
+// fd limit and the writer-cache cap, floor 16.
+uint32_t tg_auto_partitions(const char* const* files, uint32_t n_files, uint32_t k, uint32_t threads)
+{
+    struct rlimit rl {};
+    size_t fd_limit = 1024;
+    if (getrlimit(RLIMIT_NOFILE, &rl) == 0 && rl.rlim_cur != RLIM_INFINITY) fd_limit = rl.rlim_cur;
+    const size_t max_parts = fd_limit > 32 ? fd_limit - 32 : 16;
+    const size_t writer_bytes = 32;  // sizeof the reference's SuperkmerWriter
+    const size_t ratio = cache_size(2, 256u << 10) / writer_bytes;
+    size_t writer_cap = 4096;
+    while ((writer_cap << 1) <= ratio) writer_cap <<= 1;
+    writer_cap = std::min(writer_cap, (size_t)32768);
+    uint64_t total = 0, max_single = 0;
+    for (uint32_t i = 0; i < n_files; ++i) {
+        struct stat st {};
+        if (stat(files[i], &st) != 0) continue;
+        uint64_t sz = (uint64_t)st.st_size;
+        const std::string f(files[i]);
+        if (f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0) sz *= 6;
+        total += sz;
+        max_single = std::max(max_single, sz);
+    }
+    uint64_t est_bytes = total;
+    if (n_files > 1 && total / n_files < (50ull << 20)) est_bytes = max_single;
+    const uint64_t est_unique = est_bytes * 76 / 100;
+    const size_t words = (k + 31) / 32;
+    const size_t per_slot = words * 8 + 4 + 2;
+    const size_t per_kmer = (per_slot * 5 + 3) / 4;
+    const size_t l3_usable = cache_size(3, 30u << 20) * 3 / 4;
+    const uint64_t num = (uint64_t)std::max<uint32_t>(threads, 1) * est_unique * per_kmer;
+    const size_t n_l3_raw = (size_t)((num + l3_usable - 1) / l3_usable);
+    size_t n_l3 = 16;
+    while (n_l3 < n_l3_raw) n_l3 <<= 1;
+    const size_t n_tp_raw = (size_t)std::max<uint64_t>(1, total >> 21);
+    size_t n_tp = 16;
+    while (n_tp < n_tp_raw) n_tp <<= 1;
+    size_t n = std::max(n_l3, n_tp);
+    n = std::min(n, std::min(max_parts, writer_cap));
+    n = std::max(n, (size_t)16);
+    return (uint32_t)n;
+}
+
+int64_t tg_format_tsv(uint32_t k, const uint64_t* keys, const uint32_t* counts, uint64_t n, char* out,
+                      uint64_t cap)
+{
+    static const char B[4] = {'A', 'C', 'G', 'T'};
+    const uint32_t nw = k > 32 ? 2 : 1;
+    uint64_t o = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (o + k + 12 > cap) return TG_ECAP;
+        const uint64_t* w = keys + i * nw;
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t bit = 2u * (k - 1u - j);
+            out[o + j] = B[(w[bit >> 6] >> (bit & 63u)) & 3u];
+        }
+        o += k;
+        out[o++] = '\t';
+        char tmp[12];
+        int t = 0;
+        uint32_t v = counts[i];
+        do {
+            tmp[t++] = (char)('0' + v % 10u);
+            v /= 10u;
+        } while (v);
+        while (t) out[o++] = tmp[--t];
+        out[o++] = '\n';
+    }
+    return (int64_t)o;
+}
+
+int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint32_t* counts, uint64_t n)
+{
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        set_host_error(std::string("cannot open output file: ") + path);
+        return TG_EINPUT;
+    }
+    std::string h;
+    h += "KFF";
+    h += (char)1;                                   // version 1.0
+    h += (char)0;
+    h += (char)((0 << 6) | (1 << 4) | (2 << 2) | 3);  // A=0 C=1 G=2 T=3
+    h += (char)1;                                   // unique
+    h += (char)1;                                   // canonical
+    put_be(h, 0, 4);                                // no metadata
+    // global variables: k, max = 1, data_size = 4
+    h += 'v';
+    put_be(h, 3, 8);
+    for (const auto& kv : {std::make_pair("k", (uint64_t)k), std::make_pair("max", (uint64_t)1),
+                           std::make_pair("data_size", (uint64_t)4)}) {
+        h += kv.first;
+        h += '\0';
+        put_be(h, kv.second, 8);
+    }
+    // raw section: one block per k-mer (max = 1: no per-block count)
+    h += 'r';
+    put_be(h, n, 8);
+    fwrite(h.data(), 1, h.size(), f);
+    const uint32_t kb = (k + 3) / 4, nw = k > 32 ? 2 : 1;
+    std::vector<uint8_t> buf;
+    buf.reserve(1u << 20);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t* w = keys + i * nw;
+        const size_t at = buf.size();
+        buf.resize(at + kb + 4, 0);
+        for (uint32_t j = 0; j < k; ++j) {  // first base in the top bits of byte 0
+            const uint32_t bit = 2u * (k - 1u - j);
+            buf[at + j / 4] |= (uint8_t)(((w[bit >> 6] >> (bit & 63u)) & 3u) << (6u - 2u * (j % 4u)));
+        }
+        const uint32_t c = counts[i];
+        buf[at + kb] = (uint8_t)(c >> 24);
+        buf[at + kb + 1] = (uint8_t)(c >> 16);
+        buf[at + kb + 2] = (uint8_t)(c >> 8);
+        buf[at + kb + 3] = (uint8_t)c;
+        if (buf.size() >= (1u << 20)) {
+            fwrite(buf.data(), 1, buf.size(), f);
+            buf.clear();
+        }
+    }
+    if (!buf.empty()) fwrite(buf.data(), 1, buf.size(), f);
+    // footer: a variables section locating the first section, then the magic
+    std::string t;
+    t += 'v';
+    put_be(t, 2, 8);
+    t += "first_index";
+    t += '\0';
+    put_be(t, 0, 8);
+    t += "footer_size";
+    t += '\0';
+    const uint64_t footer = 1 + 8 + 12 + 8 + 12 + 8 + 3;
+    put_be(t, footer, 8);
+    t += "KFF";
+    fwrite(t.data(), 1, t.size(), f);
+    const bool ok = ferror(f) == 0;
+    fclose(f);
+    if (!ok) {
+        set_host_error(std::string("failed while writing output file: ") + path);
+        return TG_EINPUT;
+    }
+    return TG_OK;
+}
 
 int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases, uint64_t* out_off,
                         uint64_t off_cap)

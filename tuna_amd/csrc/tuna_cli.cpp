@@ -1,0 +1,441 @@
+// tuna_cli.cpp — the `tuna` command line over the gfx950 engine.
+//
+// Flags, messages, exit codes and structured stderr lines follow the
+// reference CLI (src/CLI.cpp:10-237, src/main.cpp:23-98,
+// src/pipeline.hpp:64-263).  Phase 1 (parse + superkmers + routing) and
+// Phase 2 (counting) run on the GPU; output is TSV or KFF.
+#include "../../include/tuna_gpu.h"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#ifndef TUNA_VERSION
+#define TUNA_VERSION "1.4.0"
+#endif
+
+namespace {
+
+struct Config {  // src/Config.hpp:10-26
+    std::vector<std::string> input_files;
+    std::string output_file;
+    std::string work_dir;
+    uint16_t k = 31;
+    uint16_t l = 21;
+    uint32_t num_partitions = 0;
+    uint32_t num_threads = 1;
+    uint32_t ci = 1;
+    uint64_t cx = std::numeric_limits<uint64_t>::max();
+    bool hide_progress = false;
+    bool keep_tmp = false;
+    bool partition_only = false;
+    bool debug_stats = false;
+    bool output_kff = false;
+    uint64_t ram_budget_bytes = 0;
+};
+
+void print_usage(const char* prog)
+{
+    std::cerr << "tuna — streaming k-mer counter (MI355X engine)\n"
+                 "\n"
+                 "Usage:\n"
+                 "  " << prog << " [options] <input1.fa [input2.fa ...]> <output_file>\n"
+                 "  " << prog << " [options] @<input_list_file>          <output_file>\n"
+                 "\n"
+                 "Options:\n"
+                 "  -k  <int>   k-mer length               [default: 31]\n"
+                 "              this engine accepts any value in [2,64]\n"
+                 "  -m  <int>   minimizer length            [default: 21]\n"
+                 "              any value in [1, min(k-1, 32)]\n"
+                 "  -n  <int>   number of partitions        [default: auto, ~2 MB input/partition]\n"
+                 "  -t  <int>   worker threads              [default: 1]\n"
+                 "  -ci <int>   minimum k-mer count         [default: 1]\n"
+                 "  -cx <int>   maximum k-mer count         [default: max]\n"
+                 "  -ram <int>  RAM budget in GB            [accepted; device memory is managed]\n"
+                 "  -w  <dir>   working directory for partition files written by -kt\n"
+                 "              [default: tuna_tmp/ next to output file]\n"
+                 "\n"
+                 "  -hp         hide progress messages\n"
+                 "  -kt         keep partition files (hash_<p>.superkmers in the work dir)\n"
+                 "  -tp         stop after partitioning (phase 1 only, for benchmarking)\n"
+                 "  -dbg        debug stats: per-phase device timings\n"
+                 "  -kff        write output in KFF binary format (auto-detected from .kff extension)\n"
+                 "  -h/--help   show this help\n"
+                 "  -v/--version print version\n"
+                 "\n"
+                 "Output: plain text TSV (<kmer>\\t<count>) by default; use -kff for KFF binary.\n";
+}
+
+bool parse_args(int argc, char* argv[], Config& cfg)
+{
+    if (argc < 2) return false;
+    std::vector<std::string> positionals;
+    for (int i = 1; i < argc; ++i) {
+        const std::string arg = argv[i];
+        if (arg == "-h" || arg == "--help") return false;
+        if (arg == "-v" || arg == "--version") {
+            std::cerr << "tuna " << TUNA_VERSION << "\n";
+            std::exit(0);
+        }
+        auto next_val = [&](const char* flag) -> const char* {
+            if (i + 1 >= argc) {
+                std::cerr << "tuna: error: " << flag << " requires an argument\n";
+                return nullptr;
+            }
+            return argv[++i];
+        };
+        auto parse_u64 = [&](const char* flag, const char* txt, uint64_t max, uint64_t& out) -> bool {
+            try {
+                const std::string s(txt);
+                size_t pos = 0;
+                const unsigned long long v = std::stoull(s, &pos, 10);
+                if (pos != s.size()) {
+                    std::cerr << "tuna: error: invalid integer for " << flag << ": " << s << "\n";
+                    return false;
+                }
+                if (v > max) {
+                    std::cerr << "tuna: error: value out of range for " << flag << ": " << s << "\n";
+                    return false;
+                }
+                out = v;
+                return true;
+            } catch (const std::exception&) {
+                std::cerr << "tuna: error: invalid integer for " << flag << ": " << txt << "\n";
+                return false;
+            }
+        };
+        uint64_t v = 0;
+        if (arg == "-k") {
+            const char* t = next_val("-k");
+            if (!t || !parse_u64("-k", t, std::numeric_limits<uint16_t>::max(), v)) return false;
+            cfg.k = (uint16_t)v;
+        } else if (arg == "-m") {
+            const char* t = next_val("-m");
+            if (!t || !parse_u64("-m", t, std::numeric_limits<uint16_t>::max(), v)) return false;
+            cfg.l = (uint16_t)v;
+        } else if (arg == "-n") {
+            const char* t = next_val("-n");
+            if (!t || !parse_u64("-n", t, std::numeric_limits<uint32_t>::max(), v)) return false;
+            cfg.num_partitions = (uint32_t)v;
+        } else if (arg == "-ci") {
+            const char* t = next_val("-ci");
+            if (!t || !parse_u64("-ci", t, std::numeric_limits<uint32_t>::max(), v)) return false;
+            cfg.ci = (uint32_t)v;
+        } else if (arg == "-cx") {
+            const char* t = next_val("-cx");
+            if (!t || !parse_u64("-cx", t, std::numeric_limits<uint64_t>::max(), v)) return false;
+            cfg.cx = v;
+        } else if (arg == "-t") {
+            const char* t = next_val("-t");
+            if (!t || !parse_u64("-t", t, std::numeric_limits<uint32_t>::max(), v)) return false;
+            cfg.num_threads = (uint32_t)v;
+        } else if (arg == "-ram") {
+            const char* t = next_val("-ram");
+            if (!t || !parse_u64("-ram", t, std::numeric_limits<uint64_t>::max() >> 30, v)) return false;
+            cfg.ram_budget_bytes = v << 30;
+        } else if (arg == "-w") {
+            const char* t = next_val("-w");
+            if (!t) return false;
+            cfg.work_dir = t;
+        } else if (arg == "-dbg") {
+            cfg.debug_stats = true;
+        } else if (arg == "-kff") {
+            cfg.output_kff = true;
+        } else if (arg == "-hp") {
+            cfg.hide_progress = true;
+        } else if (arg == "-kt") {
+            cfg.keep_tmp = true;
+        } else if (arg == "-tp") {
+            cfg.partition_only = true;
+        } else if (!arg.empty() && arg[0] == '-') {
+            std::cerr << "tuna: error: unknown option: " << arg << "\n";
+            return false;
+        } else {
+            positionals.push_back(arg);
+        }
+    }
+    if (positionals.size() < 2) {
+        std::cerr << "tuna: error: need at least one input file and an output file\n";
+        return false;
+    }
+    cfg.output_file = positionals.back();
+    positionals.pop_back();
+    if (positionals.size() == 1 && positionals[0][0] == '@') {
+        std::string list_path = positionals[0].substr(1);
+        if (list_path.size() >= 2 && list_path[0] == '~' && list_path[1] == '/') {
+            const char* home = std::getenv("HOME");
+            if (home) list_path = std::string(home) + list_path.substr(1);
+        }
+        std::ifstream lf(list_path);
+        if (!lf) {
+            std::cerr << "tuna: error: cannot open input list: " << list_path << "\n";
+            return false;
+        }
+        std::string line;
+        while (std::getline(lf, line))
+            if (!line.empty()) cfg.input_files.push_back(line);
+    } else {
+        cfg.input_files = std::move(positionals);
+    }
+    if (cfg.input_files.empty()) {
+        std::cerr << "tuna: error: no input files\n";
+        return false;
+    }
+    if (!cfg.output_kff) {
+        const auto& of = cfg.output_file;
+        if (of.size() >= 4 && of.compare(of.size() - 4, 4, ".kff") == 0) cfg.output_kff = true;
+    }
+    if (cfg.k < 2 || cfg.k > 256) {
+        std::cerr << "tuna: error: k-mer length (-k " << cfg.k << ") must be in [2, 256]\n";
+        return false;
+    }
+    if (cfg.l < 1 || cfg.l > 32) {
+        std::cerr << "tuna: error: minimizer length (-m " << cfg.l << ") must be in [1, 32]\n";
+        return false;
+    }
+    if (cfg.l >= cfg.k) {
+        std::cerr << "tuna: error: minimizer length (-m " << cfg.l
+                  << ") must be strictly less than k-mer length (-k " << cfg.k << ")\n";
+        return false;
+    }
+    if (cfg.num_threads == 0) {
+        std::cerr << "tuna: error: number of threads (-t) must be > 0\n";
+        return false;
+    }
+    return true;
+}
+
+uint32_t round_pow2(uint32_t n)  // src/setup.hpp:53-59
+{
+    if (n <= 1) return 1;
+    if ((n & (n - 1)) == 0) return n;
+    while (n & (n - 1)) n &= n - 1;
+    return n << 1;
+}
+
+double since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::string fmt_s(double s)
+{
+    char b[32];
+    snprintf(b, sizeof(b), "%.3fs", s);
+    return b;
+}
+
+struct HostChunks {
+    std::vector<uint8_t> bases;
+    std::vector<uint64_t> off{0};
+    uint64_t seqs = 0;
+};
+
+void read_inputs(const Config& cfg, HostChunks& h)
+{
+    for (const auto& f : cfg.input_files) {
+        uint8_t* data = nullptr;
+        uint64_t len = 0;
+        if (tg_read_file(f.c_str(), &data, &len) != TG_OK) throw std::runtime_error(tg_last_error());
+        std::unique_ptr<uint8_t, void (*)(void*)> hold(data, tg_free);
+        std::vector<uint8_t> b(len);
+        std::vector<uint64_t> o(len / 2 + 2);
+        const int64_t n = tg_parse_buffer(data, len, b.data(), o.data(), o.size());
+        if (n < 0) throw std::runtime_error("cannot parse input: " + f);
+        const uint64_t base = h.bases.size();
+        h.bases.insert(h.bases.end(), b.begin(), b.begin() + (std::ptrdiff_t)o[n]);
+        for (int64_t i = 1; i <= n; ++i) h.off.push_back(base + o[i]);
+        h.seqs += (uint64_t)n;
+    }
+}
+
+void check(int rc)
+{
+    if (rc != TG_OK) throw std::runtime_error(tg_last_error());
+}
+
+int write_partition_files(const Config& cfg, tg_ctx* ctx, const tg_params& p, const HostChunks& h)
+{
+    // -kt: the reference keeps hash_<p>.superkmers (src/superkmer_io.hpp:30-33)
+    const uint64_t K = h.bases.size();
+    std::vector<uint64_t> part_off(cfg.num_partitions + 1);
+    std::vector<uint8_t> bytes(4 * K + 64);
+    std::vector<uint64_t> rec_off(K + 2);
+    uint64_t nsk = 0, nb = 0;
+    check(tg_partitions_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(), h.off.size() - 1,
+                             part_off.data(), bytes.data(), bytes.size(), rec_off.data(), rec_off.size(),
+                             &nsk, &nb));
+    std::filesystem::create_directories(cfg.work_dir);
+    for (uint32_t q = 0; q < cfg.num_partitions; ++q) {
+        const std::string path = cfg.work_dir + "hash_" + std::to_string(q) + ".superkmers";
+        std::ofstream o(path, std::ios::binary);
+        if (!o) {
+            std::cerr << "tuna: error: cannot open partition file for writing: " << path << "\n";
+            return 1;
+        }
+        const uint64_t b = rec_off[part_off[q]], e = rec_off[part_off[q + 1]];
+        o.write((const char*)bytes.data() + b, (std::streamsize)(e - b));
+    }
+    return 0;
+}
+
+int run(Config& cfg)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    tg_ctx* ctx = nullptr;
+    check(tg_ctx_create(0, &ctx));
+    std::unique_ptr<tg_ctx, void (*)(tg_ctx*)> hold(ctx, tg_ctx_destroy);
+    if (cfg.debug_stats) tg_set_profiling(ctx, 1);
+
+    if (!cfg.hide_progress)
+        std::cerr << "[1/2] partitioning  (" << cfg.num_threads << " thread"
+                  << (cfg.num_threads > 1 ? "s" : "") << ", in-memory, gfx950) ...\n";
+    const auto t_part = std::chrono::steady_clock::now();
+    HostChunks h;
+    read_inputs(cfg, h);
+    tg_params p{cfg.k, cfg.l, cfg.num_partitions, cfg.ci, cfg.cx};
+    if (cfg.partition_only) {
+        uint64_t nsk = 0;
+        uint32_t nsp = 0;
+        // counts only (no output buffers): TG_ECAP just reports that none were copied
+        const int rc = tg_superkmers_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(),
+                                          h.off.size() - 1, nullptr, nullptr, nullptr, nullptr, 0, &nsk, &nsp);
+        if (rc != TG_OK && rc != TG_ECAP) check(rc);
+        if (cfg.keep_tmp && write_partition_files(cfg, ctx, p, h)) return 1;
+        const double t1 = since(t_part);
+        uint64_t K = 0;
+        for (size_t i = 0; i + 1 < h.off.size(); ++i) {
+            const uint64_t L = h.off[i + 1] - h.off[i];
+            if (L >= cfg.k) K += L - cfg.k + 1;
+        }
+        if (!cfg.hide_progress) std::cerr << "      " << h.seqs << " seqs  " << K << " k-mers  " << fmt_s(t1) << "\n";
+        std::cerr << "phase1: " << t1 << "s\n"
+                  << "superkmers: " << nsk << "\n";
+        if (!cfg.hide_progress) std::cerr << "done  (partition only)\n";
+        return 0;
+    }
+    tg_stats st{};
+    check(tg_count_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(), h.off.size() - 1, &st));
+    // phase split on the wall clock: parse + device phase 1 | device phase 2 + output
+    const double t_all = since(t_part);
+    const double t_phase1 = std::max(0.0, t_all - st.ms_phase2 * 1e-3);
+    if (!cfg.hide_progress) {
+        std::cerr << "      " << h.seqs << " seqs  " << st.n_kmers << " k-mers  " << fmt_s(t_phase1) << "\n";
+        std::cerr << "[2/2] counting  (gfx950, in-memory) ...\n";
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    const uint64_t n = tg_result_size(ctx);
+    const uint32_t nw = cfg.k > 32 ? 2 : 1;
+    std::vector<uint64_t> keys(n * nw + 1);
+    std::vector<uint32_t> counts(n + 1);
+    check(tg_result_copy(ctx, keys.data(), counts.data(), n));
+    if (cfg.output_kff) {
+        check(tg_write_kff(cfg.output_file.c_str(), cfg.k, keys.data(), counts.data(), n));
+    } else {
+        std::ofstream out(cfg.output_file, std::ios::binary);
+        if (!out) {
+            std::cerr << "tuna: error: cannot open output file: " << cfg.output_file << "\n";
+            return 1;
+        }
+        constexpr uint64_t BATCH = 1u << 16;
+        std::vector<char> buf(BATCH * (cfg.k + 12));
+        for (uint64_t i = 0; i < n; i += BATCH) {
+            const uint64_t m = std::min(BATCH, n - i);
+            const int64_t w = tg_format_tsv(cfg.k, keys.data() + i * nw, counts.data() + i, m, buf.data(), buf.size());
+            if (w < 0) throw std::runtime_error("TSV formatting failed");
+            out.write(buf.data(), w);
+        }
+        if (!out) {
+            std::cerr << "tuna: error: failed while writing output file: " << cfg.output_file << "\n";
+            return 1;
+        }
+    }
+    if (cfg.keep_tmp && write_partition_files(cfg, ctx, p, h)) return 1;
+    const double t_phase2 = st.ms_phase2 * 1e-3 + since(t2);
+    if (!cfg.hide_progress)
+        std::cerr << "      " << st.n_kmers << " k-mers in  " << st.n_out << " unique out  " << fmt_s(t_phase2) << "\n";
+    std::cerr << "phase1: " << t_phase1 << "s\n"
+              << "phase2: " << t_phase2 << "s\n"
+              << "superkmers: " << st.n_superkmers << "\n"
+              << "n_parts: " << cfg.num_partitions << "\n"
+              << "unique_kmers: " << st.n_out << "\n";
+    if (cfg.debug_stats) {
+        char names[4096];
+        double ms[64];
+        uint64_t cnt[64];
+        const int nk = tg_kernel_times(ctx, names, sizeof(names), ms, cnt, 64);
+        const char* q = names;
+        std::cerr << "[dbg] device phase1 " << st.ms_phase1 << " ms, phase2 " << st.ms_phase2
+                  << " ms, sub-partitions " << st.n_subparts << ", work items " << st.n_lds_groups
+                  << ", spilled k-mers " << st.n_hbm_parts << "\n";
+        for (int i = 0; i < nk && i < 64; ++i) {
+            std::cerr << "[dbg] kernel " << q << " " << ms[i] << " ms x" << cnt[i] << "\n";
+            q += strlen(q) + 1;
+        }
+    }
+    if (!cfg.hide_progress) std::cerr << "total: " << fmt_s(since(t_start)) << "\n";
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[])
+{
+    Config cfg;
+    if (!parse_args(argc, argv, cfg)) {
+        print_usage(argv[0]);
+        return 1;
+    }
+    if (cfg.k > 64) {
+        // src/pipeline.hpp:545-551 (unsupported combination), engine limit
+        std::cerr << "tuna: error: unsupported combination k=" << cfg.k << " m=" << cfg.l << "\n"
+                  << "  The gfx950 engine supports k in [2,64] with m in [1,min(k-1,32)].\n";
+        return 1;
+    }
+    if (cfg.num_partitions == 0) {
+        std::vector<const char*> fp;
+        for (const auto& f : cfg.input_files) fp.push_back(f.c_str());
+        cfg.num_partitions = tg_auto_partitions(fp.data(), (uint32_t)fp.size(), cfg.k, cfg.num_threads);
+    }
+    cfg.num_partitions = round_pow2(cfg.num_partitions);
+    if (cfg.num_partitions > (1u << 14)) {
+        std::cerr << "tuna: error: -n " << cfg.num_partitions
+                  << " exceeds the 16384 partitions of the gfx950 engine\n";
+        return 1;
+    }
+    if (cfg.work_dir.empty()) {
+        std::filesystem::path parent = std::filesystem::path(cfg.output_file).parent_path();
+        if (parent.empty()) parent = std::filesystem::current_path();
+        cfg.work_dir = (parent / "tuna_tmp").string();
+    }
+    if (cfg.work_dir.back() != '/') cfg.work_dir += '/';
+    if (!cfg.hide_progress) {
+        std::cerr << "tuna  k=" << cfg.k << "  m=" << cfg.l << "  n=" << cfg.num_partitions
+                  << "  t=" << cfg.num_threads;
+        if (cfg.ci > 1 || cfg.cx != std::numeric_limits<uint64_t>::max())
+            std::cerr << "  ci=" << cfg.ci << "  cx=" << cfg.cx;
+        std::cerr << "\n";
+    }
+    int rc = 1;
+    try {
+        rc = run(cfg);
+    } catch (const std::exception& e) {
+        std::cerr << "tuna: error: " << e.what() << "\n";
+        rc = 1;
+    } catch (...) {
+        std::cerr << "tuna: error: unexpected failure\n";
+        rc = 1;
+    }
+    return rc;
+}
