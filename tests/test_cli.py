@@ -139,24 +139,18 @@ def test_gzip_input_reads_like_plain(tmp_path):
     p = tmp_path / "r.fq.gz"
     with gzip.open(p, "wb") as f:
         f.write(text)
-    data = T.ctypes.POINTER(T.ctypes.c_uint8)()
-    n = T.ctypes.c_uint64(0)
-    assert T.lib().tg_read_file(str(p).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) == 0
-    got = bytes(data[:n.value])
-    T.lib().tg_free(data)
-    assert got == text
+    assert T.read_file(p) == text
+    assert np.array_equal(T.parse_file(p)[1], T.parse_buffer(text)[1])
 
 
 def test_empty_and_missing_inputs_report_reference_messages(tmp_path):
     e = tmp_path / "empty.fa"
     e.write_bytes(b"")
-    data = T.ctypes.POINTER(T.ctypes.c_uint8)()
-    n = T.ctypes.c_uint64(0)
-    assert T.lib().tg_read_file(str(e).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) != 0
-    assert T.lib().tg_last_error().decode() == f"Empty file: {e}"
+    with pytest.raises(T.TunaGpuError, match=f"Empty file: {e}"):
+        T.read_file(e)
     m = tmp_path / "missing.fa"
-    assert T.lib().tg_read_file(str(m).encode(), T.ctypes.byref(data), T.ctypes.byref(n)) != 0
-    assert T.lib().tg_last_error().decode() == f"Cannot open file: {m}"
+    with pytest.raises(T.TunaGpuError, match=f"Cannot open file: {m}"):
+        T.read_file(m)
 
 
 def test_auto_partitions_bounds(tmp_path):

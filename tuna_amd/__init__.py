@@ -28,7 +28,8 @@ EXPORTS = [
     "tg_ctx_create", "tg_ctx_destroy", "tg_last_error", "tg_set_profiling", "tg_kernel_times",
     "tg_reset_kernel_times", "tg_dev_alloc", "tg_dev_free", "tg_h2d", "tg_d2h", "tg_sync",
     "tg_count_host", "tg_count_device", "tg_result_size", "tg_result_copy", "tg_pack_host",
-    "tg_superkmers_host", "tg_partitions_host", "tg_parse_buffer", "tg_synth", "tg_synth_copy",
+    "tg_superkmers_host", "tg_partitions_host", "tg_parse_buffer", "tg_read_file", "tg_free",
+    "tg_auto_partitions", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
     "tg_synth_free",
 ]
 
@@ -92,6 +93,16 @@ def lib():
                                      ctypes.c_uint64, _u64p, _u64p]
     L.tg_parse_buffer.restype = ctypes.c_int64
     L.tg_parse_buffer.argtypes = [_u8p, ctypes.c_uint64, _u8p, _u64p, ctypes.c_uint64]
+    L.tg_read_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p), _u64p]
+    L.tg_free.argtypes = [_vp]
+    L.tg_free.restype = None
+    L.tg_auto_partitions.restype = ctypes.c_uint32
+    L.tg_auto_partitions.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32, ctypes.c_uint32,
+                                     ctypes.c_uint32]
+    L.tg_format_tsv.restype = ctypes.c_int64
+    L.tg_format_tsv.argtypes = [ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, ctypes.c_char_p,
+                                ctypes.c_uint64]
+    L.tg_write_kff.argtypes = [ctypes.c_char_p, ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64]
     L.tg_synth.restype = ctypes.c_int64
     L.tg_synth.argtypes = [ctypes.c_char_p]
     L.tg_synth_copy.argtypes = [_u8p, ctypes.c_uint64]
@@ -131,12 +142,22 @@ def parse_buffer(data):
     return out[: int(off[-1])].copy(), off
 
 
+def read_file(path):
+    """Whole input file (gzip-decompressed for *.gz) as bytes."""
+    L = lib()
+    data = _u8p()
+    n = ctypes.c_uint64(0)
+    rc = L.tg_read_file(str(path).encode(), ctypes.byref(data), ctypes.byref(n))
+    if rc != 0:
+        raise TunaGpuError(L.tg_last_error().decode())
+    try:
+        return ctypes.string_at(data, n.value)
+    finally:
+        L.tg_free(data)
+
+
 def parse_file(path):
-    with open(path, "rb") as f:
-        data = f.read()
-    if not data:
-        raise TunaGpuError(f"Empty file: {path}")
-    return parse_buffer(data)
+    return parse_buffer(read_file(path))
 
 
 def synth(spec):
