@@ -126,6 +126,61 @@ int tg_partitions_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
                        uint64_t* rec_off, uint64_t rec_cap, uint64_t* n_sk,
                        uint64_t* n_bytes);
 
+/* ---- many independent jobs per launch sequence ------------------------ */
+/* The reference counts one sample per run (src/main.cpp:77 ->
+ * src/pipeline.hpp:111-178); counting many samples is one pipeline each.  These
+ * calls count many jobs together: job j is chunks [job_chunk[j], job_chunk[j+1])
+ * of the input, and every job gets its own result, identical to counting it
+ * alone with tg_count_host.  Jobs share k, m, ci, cx (n_partitions is not used:
+ * the result does not depend on it). */
+typedef struct {
+    uint64_t n_bases;      /* ACGT bases of the job */
+    uint64_t n_kmers;      /* reference total_inserted */
+    uint64_t n_superkmers; /* reference "superkmers:" */
+    uint64_t n_unique;     /* distinct canonical k-mers before ci/cx */
+    uint64_t n_out;        /* reference "unique_kmers:" */
+} tg_job_stats;
+
+typedef struct {
+    uint32_t n_jobs, n_batches;
+    uint32_t n_graph_launches; /* batches run by replaying a captured hipGraph */
+    uint32_t n_captures;       /* graphs captured (one per capacity class) */
+    uint32_t n_retries;        /* batches re-run with larger overflow buffers */
+    uint64_t n_kmers, n_unique, n_out;
+    double ms_wall;            /* host wall time of the call */
+} tg_many_stats;
+
+/* Tunables of the batched path (tg_autotune_many picks them). */
+typedef struct {
+    uint32_t kmers_per_bucket; /* a job of B bases gets pow2(B / this) routing buckets */
+    uint32_t max_tiles;        /* routing tiles of a batch */
+    uint64_t batch_bases;      /* bases per batch (a larger job runs alone) */
+    float load_factor;         /* LDS table fill target when splitting work items */
+    uint32_t use_graph;        /* 1: replay one captured hipGraph per batch; 0: plain launches */
+    float rho;                 /* distinct/k-mers estimate for splits; 0 = learned per batch */
+} tg_batch_config;
+
+int tg_get_batch_config(tg_ctx* ctx, tg_batch_config* cfg);
+int tg_set_batch_config(tg_ctx* ctx, const tg_batch_config* cfg);
+
+/* Host inputs: bases/off as for tg_count_host plus the job table (n_jobs+1
+ * entries).  st (n_jobs entries) and ms are optional. */
+int tg_count_many(tg_ctx* ctx, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                  const uint64_t* off, uint64_t n_chunks, const uint64_t* job_chunk,
+                  uint32_t n_jobs, tg_job_stats* st, tg_many_stats* ms);
+/* Device-resident inputs.  job_chunk and job_base (n_jobs+1 entries each) are
+ * host arrays: job_base[j] = off[job_chunk[j]], so batches are planned without
+ * reading device memory. */
+int tg_count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                         const uint64_t* d_off, uint64_t n_chunks, const uint64_t* job_chunk,
+                         const uint64_t* job_base, uint32_t n_jobs, tg_job_stats* st,
+                         tg_many_stats* ms);
+/* Results of the last tg_count_many* call, per job (unspecified order). */
+uint64_t tg_many_result_size(tg_ctx* ctx, uint32_t job);
+int tg_many_result_copy(tg_ctx* ctx, uint32_t job, uint64_t* keys, uint32_t* counts, uint64_t cap);
+int tg_many_result_device(tg_ctx* ctx, uint32_t job, const uint64_t** d_keys,
+                          const uint32_t** d_counts, uint64_t* n);
+
 /* ---- host input path (A1) --------------------------------------------- */
 /* Parse FASTA/FASTQ bytes (format by first byte, '@' = FASTQ) into ACGT
  * chunks with the reference parser's rules (src/seq_source.hpp:33-39,169-185).

@@ -1,0 +1,205 @@
+"""Batched path (many independent jobs per launch sequence): every job's
+result must equal counting it alone (oracle and single-job path), bit-exact,
+whatever the batching, graph replay or overflow handling."""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tuna_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = T.Engine(0)
+    yield e
+    e.close()
+
+
+def _sorted(keys, cnt):
+    if keys.shape[1] == 1:
+        order = np.lexsort((keys[:, 0],))
+    else:
+        order = np.lexsort((keys[:, 0], keys[:, 1]))
+    return keys[order], cnt[order]
+
+
+def concat(parts):
+    """parts: list of (bases u8, off u64) -> bases, off, job_chunk."""
+    bases, offs, jc = [], [np.zeros(1, dtype=np.uint64)], [0]
+    base = 0
+    nch = 0
+    for b, o in parts:
+        bases.append(np.asarray(b, dtype=np.uint8))
+        offs.append(np.asarray(o[1:], dtype=np.uint64) + base)
+        base += len(b)
+        nch += len(o) - 1
+        jc.append(nch)
+    allb = np.concatenate(bases) if bases else np.zeros(0, dtype=np.uint8)
+    return allb, np.concatenate(offs), np.array(jc, dtype=np.uint64)
+
+
+def rand_fa(rng, n, name="g"):
+    return (">" + name + "\n" + "".join(rng.choice("ACGT") for _ in range(n)) + "\n").encode()
+
+
+def job_inputs(seed):
+    rng = random.Random(seed)
+    texts = [
+        rand_fa(rng, 5000),
+        T.synth("kind=reads_fq genome=40000 seed=11 cov=8 rl=150 sub=0.005 nrate=0.001").tobytes(),
+        b">empty\n",                                 # no chunks at all
+        b">short\nACGTN" + b"ACG" * 3 + b"\n",       # only chunks shorter than k
+        rand_fa(rng, 21, "exact"),                   # one k-mer
+        T.synth("kind=samples_fq samples=2000 seed=4 index=3 cov=5 rl=150 sub=0.005").tobytes(),
+        rand_fa(rng, 120000),
+        b">lc\n" + b"A" * 9000 + b"CG" * 3000 + b"ACGTTGCA" * 900 + b"\n",  # low complexity
+        T.synth("kind=reads_fq genome=25000 seed=12 cov=12 rl=100 sub=0.01 nrate=0.0").tobytes(),
+    ]
+    return [T.parse_buffer(t) for t in texts]
+
+
+def check_jobs(eng, parts, k, m, ci=1, cx=2**64 - 1, label=""):
+    bases, off, jc = concat(parts)
+    say(f"count_many {label} k={k} m={m} jobs={len(parts)} N={len(bases)} chunks={len(off) - 1}")
+    st = eng.count_many(bases, off, jc, k, m, ci, cx)
+    for j, (b, o) in enumerate(parts):
+        gk, gc = eng.many_result(j, k)
+        ok, oc, tot, nu = O.count(b, o, k, ci, cx)
+        assert st[j]["n_bases"] == len(b), j
+        assert st[j]["n_kmers"] == tot, j
+        assert st[j]["n_unique"] == nu, j
+        assert st[j]["n_out"] == len(oc) == len(gc), j
+        if len(o) > 1:
+            nsk = len(O.superkmers(b, o, k, m)[0])
+            assert st[j]["n_superkmers"] == nsk, j
+        gk, gc = _sorted(gk, gc)
+        ok, oc = _sorted(ok, oc)
+        assert np.array_equal(gk, ok), j
+        assert np.array_equal(gc, oc), j
+    return st
+
+
+def test_many_jobs_match_oracle(eng):
+    check_jobs(eng, job_inputs(1), 21, 15, label="mixed")
+
+
+def test_many_jobs_filters(eng):
+    check_jobs(eng, job_inputs(2), 31, 21, ci=2, cx=40, label="ci2 cx40")
+
+
+def test_many_jobs_long_k(eng):
+    check_jobs(eng, job_inputs(3), 63, 21, ci=2, label="k63")
+    check_jobs(eng, job_inputs(4), 45, 9, label="k45")
+
+
+def test_many_odd_k(eng):
+    check_jobs(eng, job_inputs(5), 32, 16, label="k32")
+    check_jobs(eng, job_inputs(6), 5, 3, label="k5")
+
+
+def test_graph_replay_equals_plain_launches(eng):
+    parts = job_inputs(7)
+    bases, off, jc = concat(parts)
+    eng.set_batch_config(use_graph=0)
+    try:
+        say("plain launches")
+        eng.count_many(bases, off, jc, 25, 11)
+        assert eng.last_many.n_graph_launches == 0
+        plain = [_sorted(*eng.many_result(j, 25)) for j in range(len(parts))]
+    finally:
+        eng.set_batch_config(use_graph=1)
+    say("graph replay x3")
+    for rep in range(3):
+        eng.count_many(bases, off, jc, 25, 11)
+        ms = eng.last_many
+        assert ms.n_graph_launches == ms.n_batches >= 1
+        for j in range(len(parts)):
+            gk, gc = _sorted(*eng.many_result(j, 25))
+            assert np.array_equal(gk, plain[j][0]) and np.array_equal(gc, plain[j][1]), (rep, j)
+    # the class and its graph are reused by the later calls
+    assert eng.last_many.n_captures == 0
+
+
+def test_many_batches_in_one_call(eng):
+    parts = job_inputs(8)
+    old = eng.batch_config()
+    eng.set_batch_config(batch_bases=1 << 16)
+    try:
+        check_jobs(eng, parts, 21, 15, label="small batches")
+        assert eng.last_many.n_batches > 3
+    finally:
+        eng.set_batch_config(batch_bases=old.batch_bases)
+
+
+def test_spill_and_retry_path(eng):
+    # one bucket per job and a tiny distinct estimate: work items far larger
+    # than an LDS table, so most keys go through the tagged spill table, which
+    # overflows and is regrown
+    rng = random.Random(9)
+    texts = [rand_fa(rng, 300000), rand_fa(rng, 80000), rand_fa(rng, 3000)]
+    parts = [T.parse_buffer(t) for t in texts]
+    old = eng.batch_config()
+    eng.set_batch_config(kmers_per_bucket=1 << 20, rho=0.02)
+    try:
+        check_jobs(eng, parts, 21, 15, label="spill")
+        check_jobs(eng, parts, 63, 21, label="spill k63")
+    finally:
+        eng.set_batch_config(kmers_per_bucket=old.kmers_per_bucket, rho=0.0)
+
+
+def test_many_samples_workload(eng):
+    # jobs of the many-sample workload (log-normal genome sizes, 10x reads)
+    parts = []
+    for i in range(6):
+        t = T.synth(f"kind=samples_fq samples=2000 seed=4 index={i} cov=10 rl=150 sub=0.005 hi=400000")
+        parts.append(T.parse_buffer(t.tobytes()))
+    check_jobs(eng, parts, 21, 15, label="samples")
+
+
+def test_results_equal_single_job_path(eng):
+    parts = job_inputs(10)
+    bases, off, jc = concat(parts)
+    st = eng.count_many(bases, off, jc, 31, 21)
+    for j, (b, o) in enumerate(parts):
+        if len(o) < 2:
+            continue
+        sk, sc = _sorted(*eng.count(b, o, 31, 21))
+        s1 = eng.last_stats
+        assert st[j]["n_superkmers"] == s1.n_superkmers
+        mk, mc = _sorted(*eng.many_result(j, 31))
+        assert np.array_equal(sk, mk) and np.array_equal(sc, mc), j
+
+
+def test_device_offsets_must_match_job_table(eng):
+    parts = job_inputs(11)[:2]
+    bases, off, jc = concat(parts)
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    try:
+        jb = off[jc.astype(np.int64)].copy()
+        eng.count_many_device(d_b, len(bases), d_o, len(off) - 1, jc, jb, 21, 15)
+        jb2 = jb.copy()
+        jb2[-1] -= 7  # the last job claims fewer bases than its chunks cover
+        with pytest.raises(T.TunaGpuError, match="disagree"):
+            eng.count_many_device(d_b, len(bases), d_o, len(off) - 1, jc, jb2, 21, 15)
+    finally:
+        eng.dev_free(d_b)
+        eng.dev_free(d_o)
+
+
+def test_batch_config_validation(eng):
+    with pytest.raises(T.TunaGpuError):
+        eng.set_batch_config(kmers_per_bucket=1000)
+    with pytest.raises(T.TunaGpuError):
+        eng.set_batch_config(load_factor=0.0)
+    c = eng.batch_config()
+    assert c.kmers_per_bucket >= 256 and c.use_graph in (0, 1)

@@ -30,7 +30,8 @@ EXPORTS = [
     "tg_count_host", "tg_count_device", "tg_result_size", "tg_result_copy", "tg_pack_host",
     "tg_superkmers_host", "tg_partitions_host", "tg_parse_buffer", "tg_read_file", "tg_free",
     "tg_auto_partitions", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
-    "tg_synth_free",
+    "tg_synth_free", "tg_get_batch_config", "tg_set_batch_config", "tg_count_many",
+    "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
 ]
 
 
@@ -46,6 +47,35 @@ class TgStats(ctypes.Structure):
                 ("n_parts", ctypes.c_uint32), ("n_subparts", ctypes.c_uint32),
                 ("n_lds_groups", ctypes.c_uint32), ("n_hbm_parts", ctypes.c_uint32),
                 ("ms_phase1", ctypes.c_double), ("ms_phase2", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TgJobStats(ctypes.Structure):
+    _fields_ = [("n_bases", ctypes.c_uint64), ("n_kmers", ctypes.c_uint64),
+                ("n_superkmers", ctypes.c_uint64), ("n_unique", ctypes.c_uint64),
+                ("n_out", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TgManyStats(ctypes.Structure):
+    _fields_ = [("n_jobs", ctypes.c_uint32), ("n_batches", ctypes.c_uint32),
+                ("n_graph_launches", ctypes.c_uint32), ("n_captures", ctypes.c_uint32),
+                ("n_retries", ctypes.c_uint32), ("n_kmers", ctypes.c_uint64),
+                ("n_unique", ctypes.c_uint64), ("n_out", ctypes.c_uint64),
+                ("ms_wall", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TgBatchConfig(ctypes.Structure):
+    _fields_ = [("kmers_per_bucket", ctypes.c_uint32), ("max_tiles", ctypes.c_uint32),
+                ("batch_bases", ctypes.c_uint64), ("load_factor", ctypes.c_float),
+                ("use_graph", ctypes.c_uint32), ("rho", ctypes.c_float)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -107,6 +137,19 @@ def lib():
     L.tg_synth.argtypes = [ctypes.c_char_p]
     L.tg_synth_copy.argtypes = [_u8p, ctypes.c_uint64]
     L.tg_synth_free.restype = None
+    L.tg_get_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
+    L.tg_set_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
+    L.tg_count_many.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
+                                ctypes.c_uint64, _u64p, ctypes.c_uint32, ctypes.POINTER(TgJobStats),
+                                ctypes.POINTER(TgManyStats)]
+    L.tg_count_many_device.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
+                                       ctypes.c_uint64, _u64p, _u64p, ctypes.c_uint32,
+                                       ctypes.POINTER(TgJobStats), ctypes.POINTER(TgManyStats)]
+    L.tg_many_result_size.argtypes = [_vp, ctypes.c_uint32]
+    L.tg_many_result_size.restype = ctypes.c_uint64
+    L.tg_many_result_copy.argtypes = [_vp, ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64]
+    L.tg_many_result_device.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                        _u64p]
     _lib = L
     return L
 
@@ -266,6 +309,57 @@ class Engine:
         keys = np.zeros(max(n, 1) * nw, dtype=np.uint64)
         cnt = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib().tg_result_copy(self._h, _p(keys, _u64p), _p(cnt, _u32p), n))
+        return keys[: n * nw].reshape(n, nw), cnt[:n]
+
+    # -- many jobs per launch sequence
+    def batch_config(self):
+        c = TgBatchConfig()
+        _check(lib().tg_get_batch_config(self._h, ctypes.byref(c)))
+        return c
+
+    def set_batch_config(self, **kw):
+        c = self.batch_config()
+        for key, v in kw.items():
+            setattr(c, key, v)
+        _check(lib().tg_set_batch_config(self._h, ctypes.byref(c)))
+        return c
+
+    def count_many(self, bases, off, job_chunk, k, m=21, ci=1, cx=2**64 - 1):
+        """Count every job (chunks [job_chunk[j], job_chunk[j+1])) separately in
+        shared launches; returns per-job stats (results via many_result)."""
+        bases = _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        jc = np.ascontiguousarray(job_chunk, dtype=np.uint64)
+        nj = len(jc) - 1
+        p = _params(k, m, 16, ci, cx)
+        st = (TgJobStats * max(nj, 1))()
+        ms = TgManyStats()
+        _check(lib().tg_count_many(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
+                                   _p(off, _u64p), len(off) - 1, _p(jc, _u64p), nj, st,
+                                   ctypes.byref(ms)))
+        self.last_many = ms
+        return [st[j].as_dict() for j in range(nj)]
+
+    def count_many_device(self, d_bases, n_bases, d_off, n_chunks, job_chunk, job_base, k, m=21,
+                          ci=1, cx=2**64 - 1, stats=True):
+        jc = np.ascontiguousarray(job_chunk, dtype=np.uint64)
+        jb = np.ascontiguousarray(job_base, dtype=np.uint64)
+        nj = len(jc) - 1
+        p = _params(k, m, 16, ci, cx)
+        st = (TgJobStats * max(nj, 1))()
+        ms = TgManyStats()
+        _check(lib().tg_count_many_device(self._h, ctypes.byref(p), d_bases, n_bases, d_off,
+                                          n_chunks, _p(jc, _u64p), _p(jb, _u64p), nj, st,
+                                          ctypes.byref(ms)))
+        self.last_many = ms
+        return [st[j].as_dict() for j in range(nj)] if stats else None
+
+    def many_result(self, job, k):
+        nw = 2 if k > 32 else 1
+        n = lib().tg_many_result_size(self._h, job)
+        keys = np.zeros(max(n, 1) * nw, dtype=np.uint64)
+        cnt = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(lib().tg_many_result_copy(self._h, job, _p(keys, _u64p), _p(cnt, _u32p), n))
         return keys[: n * nw].reshape(n, nw), cnt[:n]
 
     def pack(self, bases):

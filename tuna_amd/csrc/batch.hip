@@ -1,0 +1,717 @@
+// batch.hip — many independent counting jobs per launch sequence.
+//
+// The reference counts one sample per invocation (src/main.cpp:77 ->
+// src/pipeline.hpp:111-178); counting many samples means one full pipeline per
+// sample.  Here a batch concatenates the chunks of many jobs and runs ONE
+// launch sequence over all of them: every job owns a private range of routing
+// buckets, so records, work items, LDS tables and spill entries (tagged with
+// the job) never mix two jobs, and the per-job results are gathered into job
+// order on the device.  The sequence needs no host round trip: every size the
+// host does not know (strips, flagged strips, records, work items) is read by
+// the kernels from device memory, grids cover a capacity class, and the whole
+// sequence is captured once per class as a hipGraph and replayed per batch.
+#include "engine_impl.h"
+
+#include <chrono>
+
+namespace tgi {
+
+namespace {
+
+constexpr uint32_t STRIP = 2048;            // k-mers per strip (as the single-job path)
+constexpr uint32_t ITEM_CAP = 1u << tg::HSEL_BITS;  // items of a batch: buckets x splits
+constexpr uint32_t MAX_BUCKETS = 1u << 14;  // LDS histogram bound of the routing kernels
+constexpr uint32_t GATHER_BLOCKS = 1024;
+
+uint64_t pow2_ceil64(uint64_t x)
+{
+    uint64_t r = 1;
+    while (r < x) r <<= 1;
+    return r;
+}
+
+// Everything a captured launch sequence bakes in.
+struct BatchClass {
+    uint32_t k = 0, m = 0, ci = 0, nw = 0;
+    uint64_t cx = 0;
+    uint64_t cap_bases = 0, cap_ch = 0, cap_strips = 0;
+    uint32_t cap_jobs = 0, nsp = 0, xb = 0, max_tiles = 0, fixb = 0;
+    uint64_t sp_slots = 0;
+    bool operator==(const BatchClass& o) const
+    {
+        return k == o.k && m == o.m && ci == o.ci && nw == o.nw && cx == o.cx &&
+               cap_bases == o.cap_bases && cap_ch == o.cap_ch && cap_strips == o.cap_strips &&
+               cap_jobs == o.cap_jobs && nsp == o.nsp && xb == o.xb && max_tiles == o.max_tiles &&
+               fixb == o.fixb && sp_slots == o.sp_slots;
+    }
+    // every capacity of o fits in this class (same parameters)
+    bool covers(const BatchClass& o) const
+    {
+        return k == o.k && m == o.m && ci == o.ci && nw == o.nw && cx == o.cx &&
+               cap_bases >= o.cap_bases && cap_ch >= o.cap_ch && cap_strips >= o.cap_strips &&
+               cap_jobs >= o.cap_jobs && nsp >= o.nsp && max_tiles == o.max_tiles &&
+               fixb >= o.fixb && sp_slots >= o.sp_slots;
+    }
+};
+
+// Device buffers of one class (fixed addresses: the captured graph uses them).
+struct BatchBufs {
+    uint8_t* tab = nullptr;  // BatchDesc | job_chunk | job_route
+    tg::BatchDesc* desc = nullptr;
+    uint32_t* job_chunk = nullptr;
+    uint32_t* job_route = nullptr;
+    uint32_t* packed = nullptr;
+    uint64_t cap_words = 0;
+    uint64_t *boff, *nstr, *nkm, *nstr_x, *nkm_x, *scan_tmp;
+    uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_cnt, *flags;
+    uint64_t *strip_slot, *run_cnt64, *run_off;
+    uint2 **run_ptr;
+    uint2 *rec, *rec_fix, *sorted, *route_tmp;
+    uint32_t *hsk, *hkm, *bucket_job, *item_job;
+    uint64_t *part_sk, *part_km, *part_off, *bucket_item;
+    uint4* items;
+    uint64_t *seg_pos, *seg_n, *seg_x;
+    uint64_t* sp_keys;
+    uint32_t* sp_cnt;
+    unsigned long long* counters;
+    uint64_t *out_keys, *fin_keys;
+    uint32_t *out_cnt, *fin_cnt;
+    unsigned long long* job_acc;  // [0,J): distinct, [J,2J): spill reported, [2J,3J): spill cursor
+    uint64_t *job_tot, *job_item_tot, *job_nsk, *job_nkm, *job_off;
+    uint64_t n_blocks = 0, n_runs = 0;
+};
+
+// flags block (u32): [0] flagged strips, [1] fix-up overflow, [2] spill
+// overflow, [3] work items, [4] item queue
+enum { F_FLAGS = 0, F_SKERR = 1, F_SPERR = 2, F_NITEMS = 3, F_ITEMQ = 4, F_WORDS = 16 };
+
+}  // namespace
+
+struct BatchState {
+    BatchClass cls;
+    bool have_cls = false;
+    BatchBufs b{};
+    std::vector<void*> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint8_t* h_tab = nullptr;  // pinned host copy of the per-batch tables
+    uint8_t* h_back = nullptr; // pinned readback
+    size_t h_tab_bytes = 0, h_back_bytes = 0;
+    uint32_t count_blocks = 0;
+    tg_batch_config cfg{};
+    double rho = 0.3;          // distinct / k-mers, updated from every batch
+    // results
+    uint64_t* store_keys = nullptr;
+    uint32_t* store_cnt = nullptr;
+    uint64_t store_cap = 0, store_n = 0;
+    uint32_t store_nw = 1;
+    std::vector<uint64_t> res_off, res_n;
+    tg_many_stats last{};
+};
+
+void destroy_batch(BatchState* s);
+
+namespace {
+
+void free_class(BatchState* s)
+{
+    if (s->exec) hipGraphExecDestroy(s->exec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+    for (void* p : s->allocs) hipFree(p);
+    s->allocs.clear();
+    s->b = BatchBufs{};
+    s->have_cls = false;
+}
+
+template <class T>
+int balloc(BatchState* s, T** p, uint64_t count)
+{
+    void* q = nullptr;
+    const uint64_t bytes = std::max<uint64_t>(count * sizeof(T), 256);
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TG_ENOMEM, "device allocation of %llu bytes for a batch failed",
+                    (unsigned long long)bytes);
+    }
+    s->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return TG_OK;
+}
+
+int alloc_class(BatchState* s, const BatchClass& c)
+{
+    free_class(s);
+    BatchBufs& b = s->b;
+    int rc;
+    const uint64_t tab_bytes = 256 + 4ull * (c.cap_jobs + 1) + 4ull * c.cap_jobs;
+    if ((rc = balloc(s, &b.tab, tab_bytes))) return rc;
+    b.desc = reinterpret_cast<tg::BatchDesc*>(b.tab);
+    b.job_chunk = reinterpret_cast<uint32_t*>(b.tab + 256);
+    b.job_route = b.job_chunk + c.cap_jobs + 1;
+    if (s->h_tab_bytes < tab_bytes) {
+        if (s->h_tab) hipHostFree(s->h_tab);
+        s->h_tab = nullptr;
+        if (hipHostMalloc((void**)&s->h_tab, tab_bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(TG_ENOMEM, "pinned host allocation failed");
+        s->h_tab_bytes = tab_bytes;
+    }
+    const uint64_t back_bytes = 8ull * (4ull * c.cap_jobs + 1) + 4ull * F_WORDS + 64;
+    if (s->h_back_bytes < back_bytes) {
+        if (s->h_back) hipHostFree(s->h_back);
+        s->h_back = nullptr;
+        if (hipHostMalloc((void**)&s->h_back, back_bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(TG_ENOMEM, "pinned host allocation failed");
+        s->h_back_bytes = back_bytes;
+    }
+    b.cap_words = c.cap_bases / 16 + 16;
+    const uint64_t nch1 = c.cap_ch + 1, ns1 = c.cap_strips + 1;
+    b.n_blocks = (c.cap_strips + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
+    b.n_runs = b.n_blocks + c.fixb;
+    const uint64_t scan_words =
+        tg::scan_tmp_words(std::max<uint64_t>({nch1, b.n_runs + 1, c.nsp + 1ull, ITEM_CAP + 1ull,
+                                               c.cap_jobs + 1ull})) + 64;
+    const uint64_t recs = c.cap_bases + 1;
+    if ((rc = balloc(s, &b.packed, b.cap_words)) || (rc = balloc(s, &b.boff, nch1)) ||
+        (rc = balloc(s, &b.nstr, nch1)) || (rc = balloc(s, &b.nkm, nch1)) ||
+        (rc = balloc(s, &b.nstr_x, nch1)) || (rc = balloc(s, &b.nkm_x, nch1)) ||
+        (rc = balloc(s, &b.scan_tmp, scan_words)) || (rc = balloc(s, &b.chunk_job, nch1)) ||
+        (rc = balloc(s, &b.strip_chunk, ns1)) || (rc = balloc(s, &b.strip_tb, ns1)) ||
+        (rc = balloc(s, &b.strip_slot, ns1)) || (rc = balloc(s, &b.flag_list, ns1)) ||
+        (rc = balloc(s, &b.flags, F_WORDS)) || (rc = balloc(s, &b.run_ptr, b.n_runs + 1)) ||
+        (rc = balloc(s, &b.run_cnt, b.n_runs + 1)) || (rc = balloc(s, &b.run_cnt64, b.n_runs + 1)) ||
+        (rc = balloc(s, &b.run_off, b.n_runs + 2)) || (rc = balloc(s, &b.rec, recs)) ||
+        (rc = balloc(s, &b.rec_fix, (uint64_t)c.fixb * tg::SK_BLOCK * STRIP)) ||
+        (rc = balloc(s, &b.sorted, recs)) || (rc = balloc(s, &b.route_tmp, recs)) ||
+        (rc = balloc(s, &b.hsk, (uint64_t)c.max_tiles * c.nsp)) ||
+        (rc = balloc(s, &b.hkm, (uint64_t)c.max_tiles * c.nsp)) ||
+        (rc = balloc(s, &b.part_sk, c.nsp + 1)) || (rc = balloc(s, &b.part_km, c.nsp + 1)) ||
+        (rc = balloc(s, &b.part_off, c.nsp + 1)) || (rc = balloc(s, &b.bucket_job, c.nsp)) ||
+        (rc = balloc(s, &b.bucket_item, c.nsp + 1)) || (rc = balloc(s, &b.items, ITEM_CAP)) ||
+        (rc = balloc(s, &b.item_job, ITEM_CAP)) || (rc = balloc(s, &b.seg_pos, ITEM_CAP)) ||
+        (rc = balloc(s, &b.seg_n, ITEM_CAP + 1)) || (rc = balloc(s, &b.seg_x, ITEM_CAP + 2)) ||
+        (rc = balloc(s, &b.sp_keys, c.sp_slots * (c.nw + 1))) ||
+        (rc = balloc(s, &b.sp_cnt, c.sp_slots)) || (rc = balloc(s, &b.counters, 8)) ||
+        (rc = balloc(s, &b.out_keys, recs * c.nw)) || (rc = balloc(s, &b.out_cnt, recs)) ||
+        (rc = balloc(s, &b.fin_keys, recs * c.nw)) || (rc = balloc(s, &b.fin_cnt, recs)) ||
+        (rc = balloc(s, &b.job_acc, 3ull * c.cap_jobs)) ||
+        (rc = balloc(s, &b.job_tot, c.cap_jobs + 1)) ||
+        (rc = balloc(s, &b.job_item_tot, c.cap_jobs)) || (rc = balloc(s, &b.job_nsk, c.cap_jobs)) ||
+        (rc = balloc(s, &b.job_nkm, c.cap_jobs)) || (rc = balloc(s, &b.job_off, c.cap_jobs + 2)))
+        return rc;
+    s->cls = c;
+    s->have_cls = true;
+    return TG_OK;
+}
+
+// The launch sequence of one batch (no host synchronisation inside).
+void enqueue_batch(BatchState* s, hipStream_t st)
+{
+    const BatchClass& c = s->cls;
+    BatchBufs& b = s->b;
+    const tg::BatchJobs jobs{b.job_chunk, b.job_route};
+    const uint32_t J = c.cap_jobs;
+    // clear state (kernels only: the sequence may be captured)
+    tg::launch_fill_u32(st, b.flags, 0u, F_WORDS);
+    tg::launch_fill_u64(st, (uint64_t*)b.counters, 0ull, 8);
+    tg::launch_fill_u32(st, b.sp_cnt, 0u, c.sp_slots);
+    tg::launch_fill_u64(st, b.seg_n, 0ull, ITEM_CAP + 1);
+    tg::launch_fill_u64(st, (uint64_t*)b.job_acc, 0ull, 3ull * J);
+    // inputs
+    tg::launch_batch_setup(st, b.desc, jobs, J, b.boff, b.chunk_job, c.cap_ch, b.bucket_job, c.nsp,
+                           b.flags + F_SKERR);
+    tg::launch_pack_desc(st, b.desc, b.packed, b.cap_words);
+    tg::launch_chunk_counts(st, b.boff, c.cap_ch, c.k, STRIP, b.nstr, b.nkm);
+    tg::launch_scan_u64(st, b.nstr, c.cap_ch, b.nstr_x, b.scan_tmp);
+    tg::launch_scan_u64(st, b.nkm, c.cap_ch, b.nkm_x, b.scan_tmp);
+    tg::launch_fill_strips(st, c.cap_ch, b.nstr_x, b.nkm_x, b.nkm, STRIP, b.strip_chunk, b.strip_tb,
+                           b.strip_slot);
+    // superkmers: main pass over the strip capacity, fix-up over FIXB blocks
+    tg::SkArgs a{};
+    a.packed = b.packed;
+    a.off = b.boff;
+    a.strip_chunk = b.strip_chunk;
+    a.strip_tb = b.strip_tb;
+    a.strip_slot = b.strip_slot;
+    a.list = nullptr;
+    a.n = (uint32_t)c.cap_strips;
+    a.k = c.k;
+    a.m = c.m;
+    a.W = c.k - c.m + 1;
+    a.S = STRIP;
+    a.capk = 256u - c.k;
+    a.lookback = a.capk + 1u;
+    a.log2n = 0;
+    a.xb = c.xb;
+    a.chunk_job = b.chunk_job;
+    a.job_route = b.job_route;
+    a.n_dev = b.nstr_x + c.cap_ch;
+    a.fix_dev = 0;
+    a.err = b.flags + F_SKERR;
+    a.rec = b.rec;
+    a.run_ptr = b.run_ptr;
+    a.run_cnt = b.run_cnt;
+    a.run_base = 0;
+    a.flag_list = b.flag_list;
+    a.flag_count = b.flags + F_FLAGS;
+    tg::launch_superkmers(st, a);
+    a.list = b.flag_list;
+    a.n = c.fixb * tg::SK_BLOCK;
+    a.n_dev = nullptr;
+    a.fix_dev = 1;
+    a.lookback = tg::LOOKBACK_FULL;
+    a.rec = b.rec_fix;
+    a.run_base = (uint32_t)b.n_blocks;
+    tg::launch_superkmers(st, a);
+    // routing into job buckets
+    tg::launch_u32_to_u64(st, b.run_cnt, b.n_runs, b.run_cnt64);
+    tg::launch_scan_u64(st, b.run_cnt64, b.n_runs, b.run_off, b.scan_tmp);
+    tg::RouteArgs r{};
+    r.run_ptr = b.run_ptr;
+    r.run_cnt = b.run_cnt;
+    r.run_off = b.run_off;
+    r.n_runs = (uint32_t)b.n_runs;
+    r.nsp = c.nsp;
+    r.sb = c.xb;
+    r.k = c.k;
+    r.hsk = b.hsk;
+    r.hkm = b.hkm;
+    r.part_off = b.part_off;
+    r.dst = b.sorted;
+    r.n_rec_dev = b.run_off + b.n_runs;
+    r.max_tiles = c.max_tiles;
+    tg::launch_route_hist(st, r, c.max_tiles);
+    tg::launch_route_colscan(st, r, c.max_tiles, b.part_sk, b.part_km);
+    tg::launch_scan_u64(st, b.part_sk, c.nsp, b.part_off, b.scan_tmp);
+    tg::launch_route_scatter(st, r, c.max_tiles, b.route_tmp);
+    // work items and counting
+    tg::launch_build_items(st, b.part_km, b.part_off, c.nsp, c.xb, b.bucket_job, &b.desc->scale,
+                           b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS);
+    tg::CountArgs ca{};
+    ca.packed = b.packed;
+    ca.rec = b.sorted;
+    ca.items = b.items;
+    ca.k = c.k;
+    ca.ci = c.ci;
+    ca.cx = c.cx;
+    ca.probe_max = 64;
+    ca.sp_keys = b.sp_keys;
+    ca.sp_cnt = b.sp_cnt;
+    ca.sp_mask = c.sp_slots - 1;
+    ca.sp_err = b.flags + F_SPERR;
+    ca.out_keys = b.out_keys;
+    ca.out_cnt = b.out_cnt;
+    ca.counters = b.counters;
+    ca.out_cap = c.cap_bases + 1;
+    ca.item_uniq = nullptr;
+    ca.n_items_dev = b.flags + F_NITEMS;
+    ca.item_q = b.flags + F_ITEMQ;
+    ca.item_job = b.item_job;
+    ca.seg_pos = b.seg_pos;
+    ca.seg_n = b.seg_n;
+    ca.job_uniq = b.job_acc;
+    tg::launch_count_batch(st, ca, c.nw, s->count_blocks);
+    // per-job results in job order
+    tg::launch_spill_jobs(st, ca, c.nw, c.sp_slots, b.job_acc + J);
+    tg::launch_scan_u64(st, b.seg_n, ITEM_CAP, b.seg_x, b.scan_tmp);
+    tg::launch_job_totals(st, b.desc, jobs, J, b.bucket_item, b.seg_x, b.part_off, b.job_acc + J,
+                          b.nkm_x, b.job_tot, b.job_item_tot, b.job_nsk, b.job_nkm);
+    tg::launch_scan_u64(st, b.job_tot, J, b.job_off, b.scan_tmp);
+    tg::launch_gather_items(st, ca, c.nw, jobs, b.bucket_item, b.seg_x, b.job_off, b.fin_keys,
+                            b.fin_cnt, GATHER_BLOCKS);
+    tg::launch_spill_place(st, ca, c.nw, c.sp_slots, b.job_off, b.job_item_tot, b.job_acc + 2ull * J,
+                           b.fin_keys, b.fin_cnt);
+}
+
+int ensure_graph(BatchState* s, hipStream_t st)
+{
+    if (s->exec) return TG_OK;
+    // kernel attributes are set outside the capture
+    HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue_batch(s, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return fail(TG_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    s->graph = g;
+    HIPOK(hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0));
+    s->last.n_captures += 1;
+    return TG_OK;
+}
+
+int grow_store(BatchState* s, hipStream_t st, uint64_t need, uint32_t nw)
+{
+    if (s->store_cap >= need && s->store_nw == nw) return TG_OK;
+    uint64_t cap = std::max<uint64_t>(need, s->store_cap + s->store_cap / 2);
+    cap = std::max<uint64_t>(cap, 1u << 20);
+    uint64_t* k2 = nullptr;
+    uint32_t* c2 = nullptr;
+    if (hipMalloc(&k2, cap * nw * 8ull) != hipSuccess || hipMalloc(&c2, cap * 4ull) != hipSuccess) {
+        (void)hipGetLastError();
+        if (k2) hipFree(k2);
+        return fail(TG_ENOMEM, "result store of %llu k-mers does not fit in device memory",
+                    (unsigned long long)cap);
+    }
+    if (s->store_n) {
+        HIPOK(hipMemcpyAsync(k2, s->store_keys, s->store_n * nw * 8ull, hipMemcpyDeviceToDevice, st));
+        HIPOK(hipMemcpyAsync(c2, s->store_cnt, s->store_n * 4ull, hipMemcpyDeviceToDevice, st));
+    }
+    HIPOK(hipStreamSynchronize(st));
+    if (s->store_keys) hipFree(s->store_keys);
+    if (s->store_cnt) hipFree(s->store_cnt);
+    s->store_keys = k2;
+    s->store_cnt = c2;
+    s->store_cap = cap;
+    s->store_nw = nw;
+    return TG_OK;
+}
+
+struct JobTab {
+    const uint64_t* job_chunk;
+    const uint64_t* job_base;
+    uint32_t n;
+};
+
+uint32_t job_buckets(uint64_t bases, uint32_t kb)
+{
+    const uint64_t want = std::max<uint64_t>(1, (bases + kb - 1) / kb);
+    return (uint32_t)std::min<uint64_t>(pow2_ceil64(want), MAX_BUCKETS);
+}
+
+// Greedy batches in job order: bases within the configured budget (a larger
+// job goes alone), buckets within MAX_BUCKETS.
+void plan_batches(const JobTab& jt, const tg_batch_config& cfg, std::vector<uint32_t>& cuts)
+{
+    cuts.clear();
+    cuts.push_back(0);
+    uint64_t bases = 0;
+    uint32_t buckets = 0;
+    for (uint32_t j = 0; j < jt.n; ++j) {
+        const uint64_t nb = jt.job_base[j + 1] - jt.job_base[j];
+        const uint32_t nbk = job_buckets(nb, cfg.kmers_per_bucket);
+        const bool empty = j == cuts.back();
+        if (!empty && (bases + nb > cfg.batch_bases || buckets + nbk > MAX_BUCKETS)) {
+            cuts.push_back(j);
+            bases = 0;
+            buckets = 0;
+        }
+        bases += nb;
+        buckets += nbk;
+    }
+    if (cuts.back() != jt.n) cuts.push_back(jt.n);
+}
+
+BatchClass class_for(const tg_params* p, const JobTab& jt, const std::vector<uint32_t>& cuts,
+                     const tg_batch_config& cfg)
+{
+    BatchClass c;
+    c.k = p->k;
+    c.m = p->m;
+    c.ci = p->ci;
+    c.cx = p->cx;
+    c.nw = p->k > 32 ? 2u : 1u;
+    c.max_tiles = cfg.max_tiles;
+    c.fixb = tg::FIX_BLOCKS;
+    uint64_t bases = 0, ch = 0;
+    uint32_t jobs = 0, buckets = 0;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        const uint32_t j0 = cuts[i], j1 = cuts[i + 1];
+        bases = std::max(bases, jt.job_base[j1] - jt.job_base[j0]);
+        ch = std::max(ch, jt.job_chunk[j1] - jt.job_chunk[j0]);
+        jobs = std::max(jobs, j1 - j0);
+        uint32_t bk = 0;
+        for (uint32_t j = j0; j < j1; ++j)
+            bk += job_buckets(jt.job_base[j + 1] - jt.job_base[j], cfg.kmers_per_bucket);
+        buckets = std::max(buckets, bk);
+    }
+    c.cap_bases = pow2_ceil64(std::max<uint64_t>(bases, 1u << 16));
+    c.cap_ch = pow2_ceil64(std::max<uint64_t>(ch, 1024));
+    c.cap_strips = c.cap_ch + c.cap_bases / STRIP + 1;
+    c.cap_jobs = (uint32_t)pow2_ceil64(std::max<uint32_t>(jobs, 64));
+    c.nsp = (uint32_t)pow2_ceil64(std::max<uint32_t>(buckets, 2));
+    c.xb = tg::HSEL_BITS - ilog2(c.nsp);
+    c.sp_slots = pow2_ceil64(std::max<uint64_t>(c.cap_bases / 64, 1u << 16));
+    return c;
+}
+
+}  // namespace
+
+void destroy_batch(BatchState* s)
+{
+    if (!s) return;
+    free_class(s);
+    if (s->h_tab) hipHostFree(s->h_tab);
+    if (s->h_back) hipHostFree(s->h_back);
+    if (s->store_keys) hipFree(s->store_keys);
+    if (s->store_cnt) hipFree(s->store_cnt);
+    delete s;
+}
+
+namespace {
+
+BatchState* state(tg_ctx* c)
+{
+    if (!c->batch) {
+        c->batch = new BatchState();
+        tg_batch_config& g = c->batch->cfg;
+        g.kmers_per_bucket = 16384;
+        g.max_tiles = tg::ROUTE_MAX_TILES;
+        g.batch_bases = 1ull << 28;
+        g.load_factor = 0.75f;
+        g.use_graph = 1;
+        g.rho = 0.0f;
+    }
+    return c->batch;
+}
+
+int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                      const uint64_t* d_off, uint64_t n_chunks, const JobTab& jt,
+                      tg_job_stats* js)
+{
+    int rc = check_params(p, 0, 0);
+    if (rc) return rc;
+    if (!jt.job_chunk || !jt.job_base) return fail(TG_EINVAL, "null job table");
+    if (jt.job_chunk[jt.n] > n_chunks || jt.job_base[jt.n] > n_bases)
+        return fail(TG_EINVAL, "job table exceeds the input (%llu chunks, %llu bases)",
+                    (unsigned long long)n_chunks, (unsigned long long)n_bases);
+    for (uint32_t j = 0; j < jt.n; ++j) {
+        if (jt.job_chunk[j + 1] < jt.job_chunk[j] || jt.job_base[j + 1] < jt.job_base[j])
+            return fail(TG_EINVAL, "job table is not non-decreasing at job %u", j);
+        if (jt.job_base[j + 1] - jt.job_base[j] >= (1ull << 31))
+            return fail(TG_EINVAL, "job %u has %llu bases; a job is limited to 2^31 in a batch", j,
+                        (unsigned long long)(jt.job_base[j + 1] - jt.job_base[j]));
+    }
+    HIPOK(hipSetDevice(ctx->device));
+    BatchState* s = state(ctx);
+    hipStream_t st = ctx->stream;
+    const auto t_start = std::chrono::steady_clock::now();
+    s->last = tg_many_stats{};
+    const uint32_t nw = p->k > 32 ? 2u : 1u;
+    s->count_blocks = tg::count_persistent_blocks(nw);
+    tg::set_kernel_attributes();
+    std::vector<uint32_t> cuts;
+    plan_batches(jt, s->cfg, cuts);
+    BatchClass want = class_for(p, jt, cuts, s->cfg);
+    if (s->have_cls && s->cls.covers(want)) want = s->cls;
+    s->res_off.assign(jt.n, 0);
+    s->res_n.assign(jt.n, 0);
+    s->store_n = 0;
+    if ((rc = grow_store(s, st, 1, nw))) return rc;
+    uint64_t tot_kmers = 0, tot_uniq = 0;
+    for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) {
+        const uint32_t j0 = cuts[bi], j1 = cuts[bi + 1];
+        for (int attempt = 0;; ++attempt) {
+            if (!s->have_cls || !(s->cls == want)) {
+                if ((rc = alloc_class(s, want))) return rc;
+            }
+            const BatchClass& c = s->cls;
+            // per-batch tables
+            tg::BatchDesc* d = reinterpret_cast<tg::BatchDesc*>(s->h_tab);
+            uint32_t* hjc = reinterpret_cast<uint32_t*>(s->h_tab + 256);
+            uint32_t* hjr = hjc + c.cap_jobs + 1;
+            const uint64_t c0 = jt.job_chunk[j0];
+            d->bases = d_bases + jt.job_base[j0];
+            d->off = d_off + c0;
+            d->off_base = jt.job_base[j0];
+            d->n_bases = jt.job_base[j1] - jt.job_base[j0];
+            d->n_chunks = jt.job_chunk[j1] - c0;
+            d->n_jobs = j1 - j0;
+            const double target = (double)s->cfg.load_factor *
+                                  (nw == 1 ? tg::LDS_CAP_NW1 : tg::LDS_CAP_NW2);
+            d->scale = (float)((s->cfg.rho > 0 ? s->cfg.rho : s->rho) / target);
+            uint32_t base = 0;
+            for (uint32_t j = j0; j < j1; ++j) {
+                const uint32_t nb =
+                    job_buckets(jt.job_base[j + 1] - jt.job_base[j], s->cfg.kmers_per_bucket);
+                hjc[j - j0] = (uint32_t)(jt.job_chunk[j] - c0);
+                hjr[j - j0] = base | (ilog2(nb) << 16);
+                base += nb;
+            }
+            hjc[j1 - j0] = (uint32_t)(jt.job_chunk[j1] - c0);
+            const uint64_t tab_bytes = 256 + 4ull * (c.cap_jobs + 1) + 4ull * c.cap_jobs;
+            HIPOK(hipMemcpyAsync(s->b.tab, s->h_tab, tab_bytes, hipMemcpyHostToDevice, st));
+            if (s->cfg.use_graph) {
+                if ((rc = ensure_graph(s, st))) return rc;
+                HIPOK(hipGraphLaunch(s->exec, st));
+                s->last.n_graph_launches += 1;
+            } else {
+                enqueue_batch(s, st);
+                HIPOK(hipGetLastError());
+            }
+            // readback: job offsets, superkmers, k-mers, distinct, flags, counters
+            const uint32_t nj = j1 - j0;
+            uint64_t* h_off = reinterpret_cast<uint64_t*>(s->h_back);
+            uint64_t* h_nsk = h_off + c.cap_jobs + 1;
+            uint64_t* h_nkm = h_nsk + c.cap_jobs;
+            uint64_t* h_unq = h_nkm + c.cap_jobs;
+            uint32_t* h_flags = reinterpret_cast<uint32_t*>(h_unq + c.cap_jobs);
+            HIPOK(hipMemcpyAsync(h_off, s->b.job_off, (nj + 1) * 8ull, hipMemcpyDeviceToHost, st));
+            HIPOK(hipMemcpyAsync(h_nsk, s->b.job_nsk, nj * 8ull, hipMemcpyDeviceToHost, st));
+            HIPOK(hipMemcpyAsync(h_nkm, s->b.job_nkm, nj * 8ull, hipMemcpyDeviceToHost, st));
+            HIPOK(hipMemcpyAsync(h_unq, s->b.job_acc, nj * 8ull, hipMemcpyDeviceToHost, st));
+            HIPOK(hipMemcpyAsync(h_flags, s->b.flags, 4ull * F_WORDS, hipMemcpyDeviceToHost, st));
+            HIPOK(hipStreamSynchronize(st));
+            if (h_flags[F_SKERR] & 4u)
+                return fail(TG_EINVAL, "device chunk offsets disagree with the job table (batch %zu)",
+                            bi);
+            if (h_flags[F_SKERR] || h_flags[F_SPERR]) {
+                if (attempt >= 4)
+                    return fail(TG_ENOMEM, "batch overflow after %d retries (fix-up %u, spill %u)",
+                                attempt, h_flags[F_SKERR], h_flags[F_SPERR]);
+                if (h_flags[F_SKERR])
+                    want.fixb = (uint32_t)std::max<uint64_t>(
+                        want.fixb * 2ull, (h_flags[F_FLAGS] + tg::SK_BLOCK - 1) / tg::SK_BLOCK);
+                if (h_flags[F_SPERR]) want.sp_slots <<= 2;
+                s->last.n_retries += 1;
+                continue;
+            }
+            const uint64_t n_out = h_off[nj];
+            if ((rc = grow_store(s, st, s->store_n + n_out, nw))) return rc;
+            if (n_out) {
+                HIPOK(hipMemcpyAsync(s->store_keys + s->store_n * nw, s->b.fin_keys, n_out * nw * 8ull,
+                                     hipMemcpyDeviceToDevice, st));
+                HIPOK(hipMemcpyAsync(s->store_cnt + s->store_n, s->b.fin_cnt, n_out * 4ull,
+                                     hipMemcpyDeviceToDevice, st));
+            }
+            uint64_t bk = 0, bu = 0;
+            for (uint32_t q = 0; q < nj; ++q) {
+                const uint32_t j = j0 + q;
+                s->res_off[j] = s->store_n + h_off[q];
+                s->res_n[j] = h_off[q + 1] - h_off[q];
+                if (js) {
+                    js[j].n_bases = jt.job_base[j + 1] - jt.job_base[j];
+                    js[j].n_kmers = h_nkm[q];
+                    js[j].n_superkmers = h_nsk[q];
+                    js[j].n_unique = h_unq[q];
+                    js[j].n_out = s->res_n[j];
+                }
+                bk += h_nkm[q];
+                bu += h_unq[q];
+            }
+            s->store_n += n_out;
+            tot_kmers += bk;
+            tot_uniq += bu;
+            // distinct/k-mers of this batch sizes the splits of the next one
+            if (bk) s->rho = std::min(1.0, std::max(0.02, 1.15 * (double)bu / (double)bk));
+            s->last.n_batches += 1;
+            break;
+        }
+    }
+    HIPOK(hipStreamSynchronize(st));
+    s->last.n_jobs = jt.n;
+    s->last.n_kmers = tot_kmers;
+    s->last.n_unique = tot_uniq;
+    s->last.n_out = s->store_n;
+    s->last.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return TG_OK;
+}
+
+}  // namespace
+}  // namespace tgi
+
+using namespace tgi;
+
+extern "C" {
+
+int tg_get_batch_config(tg_ctx* c, tg_batch_config* cfg)
+{
+    if (!c || !cfg) return fail(TG_EINVAL, "null argument");
+    *cfg = state(c)->cfg;
+    return TG_OK;
+}
+
+int tg_set_batch_config(tg_ctx* c, const tg_batch_config* cfg)
+{
+    if (!c || !cfg) return fail(TG_EINVAL, "null argument");
+    if (cfg->kmers_per_bucket < 256 || (cfg->kmers_per_bucket & (cfg->kmers_per_bucket - 1)))
+        return fail(TG_EINVAL, "kmers_per_bucket must be a power of two >= 256");
+    if (cfg->max_tiles < 1 || cfg->max_tiles > 4096)
+        return fail(TG_EINVAL, "max_tiles must be in [1, 4096]");
+    if (cfg->batch_bases < (1ull << 16) || cfg->batch_bases >= (1ull << 31))
+        return fail(TG_EINVAL, "batch_bases must be in [2^16, 2^31)");
+    if (!(cfg->load_factor > 0.05f && cfg->load_factor <= 1.0f))
+        return fail(TG_EINVAL, "load_factor must be in (0.05, 1]");
+    if (!(cfg->rho == 0.0f || (cfg->rho >= 0.001f && cfg->rho <= 1.0f)))
+        return fail(TG_EINVAL, "rho must be 0 (learned) or in [0.001, 1]");
+    BatchState* s = state(c);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    if (cfg->max_tiles != s->cfg.max_tiles) free_class(s);
+    s->cfg = *cfg;
+    return TG_OK;
+}
+
+int tg_count_many_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                         const uint64_t* d_off, uint64_t n_chunks, const uint64_t* job_chunk,
+                         const uint64_t* job_base, uint32_t n_jobs, tg_job_stats* st,
+                         tg_many_stats* ms)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    const JobTab jt{job_chunk, job_base, n_jobs};
+    const int rc = count_many_device(c, p, d_bases, n_bases, d_off, n_chunks, jt, st);
+    if (rc == TG_OK && ms) *ms = c->batch->last;
+    return rc;
+}
+
+int tg_count_many(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                  const uint64_t* off, uint64_t n_chunks, const uint64_t* job_chunk,
+                  uint32_t n_jobs, tg_job_stats* st, tg_many_stats* ms)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    if (!job_chunk || (!off && n_chunks) || (!bases && n_bases)) return fail(TG_EINVAL, "null argument");
+    if (job_chunk[n_jobs] > n_chunks) return fail(TG_EINVAL, "job table exceeds the chunk count");
+    HIPOK(hipSetDevice(c->device));
+    std::vector<uint64_t> job_base(n_jobs + 1);
+    for (uint32_t j = 0; j <= n_jobs; ++j) job_base[j] = n_chunks ? off[job_chunk[j]] : 0;
+    uint8_t* d_b;
+    uint64_t* d_o;
+    int rc;
+    if ((rc = ensure_t(c, "many_bases", n_bases + 64, &d_b)) ||
+        (rc = ensure_t(c, "many_off", n_chunks + 1, &d_o)))
+        return rc;
+    if (n_bases) HIPOK(hipMemcpyAsync(d_b, bases, n_bases, hipMemcpyHostToDevice, c->stream));
+    if (n_chunks) HIPOK(hipMemcpyAsync(d_o, off, (n_chunks + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+    const JobTab jt{job_chunk, job_base.data(), n_jobs};
+    rc = count_many_device(c, p, d_b, n_bases, d_o, n_chunks, jt, st);
+    if (rc == TG_OK && ms) *ms = c->batch->last;
+    return rc;
+}
+
+uint64_t tg_many_result_size(tg_ctx* c, uint32_t job)
+{
+    if (!c || !c->batch || job >= c->batch->res_n.size()) return 0;
+    return c->batch->res_n[job];
+}
+
+int tg_many_result_copy(tg_ctx* c, uint32_t job, uint64_t* keys, uint32_t* counts, uint64_t cap)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    BatchState* s = c->batch;
+    if (!s || job >= s->res_n.size()) return fail(TG_EINVAL, "no result for job %u", job);
+    const uint64_t n = std::min<uint64_t>(cap, s->res_n[job]);
+    if (!n) return TG_OK;
+    HIPOK(hipSetDevice(c->device));
+    const uint32_t nw = s->store_nw;
+    if (keys)
+        HIPOK(hipMemcpyAsync(keys, s->store_keys + s->res_off[job] * nw, n * nw * 8ull,
+                             hipMemcpyDeviceToHost, c->stream));
+    if (counts)
+        HIPOK(hipMemcpyAsync(counts, s->store_cnt + s->res_off[job], n * 4ull, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_many_result_device(tg_ctx* c, uint32_t job, const uint64_t** d_keys, const uint32_t** d_counts,
+                          uint64_t* n)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    BatchState* s = c->batch;
+    if (!s || job >= s->res_n.size()) return fail(TG_EINVAL, "no result for job %u", job);
+    if (d_keys) *d_keys = s->store_keys + s->res_off[job] * s->store_nw;
+    if (d_counts) *d_counts = s->store_cnt + s->res_off[job];
+    if (n) *n = s->res_n[job];
+    return TG_OK;
+}
+
+}  // extern "C"

@@ -4,193 +4,19 @@
 // work items, spill drain).  It replaces the in-memory pipeline of
 // src/pipeline.hpp:111-178 (partition_kmers_mem + count_and_write_mem).
 // Device buffers live in the context and are reused across jobs.
-#include "../../include/tuna_gpu.h"
-#include "tg_kernels.h"
+#include "engine_impl.h"
 
-#include <algorithm>
 #include <chrono>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <string>
-#include <vector>
 
-namespace {
+namespace tgi {
 thread_local std::string g_err;
 }
 
-void set_host_error(const std::string& m) { g_err = m; }
+void set_host_error(const std::string& m) { tgi::g_err = m; }
+
+using namespace tgi;
 
 namespace {
-
-int fail(int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIPOK(expr)                                                                     \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(TG_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
-
-uint32_t round_pow2(uint32_t n)
-{
-    // src/setup.hpp:53-59
-    if (n <= 1) return 1;
-    if ((n & (n - 1)) == 0) return n;
-    while (n & (n - 1)) n &= n - 1;
-    return n << 1;
-}
-
-uint32_t ilog2(uint64_t x)
-{
-    uint32_t r = 0;
-    while ((1ull << r) < x) ++r;
-    return r;
-}
-
-struct Buf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-struct KTime {
-    double ms = 0;
-    uint64_t n = 0;
-};
-
-}  // namespace
-
-struct tg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::map<std::string, Buf> bufs;
-    bool profiling = false;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    std::vector<hipEvent_t> ev_pool;
-    std::map<std::string, KTime> ktimes;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
-    // last job
-    uint64_t n_out = 0;
-    uint32_t nw = 1;
-    uint64_t* d_keys = nullptr;
-    uint32_t* d_cnt = nullptr;
-};
-
-namespace {
-
-int ensure(tg_ctx* c, const char* name, size_t bytes, void** out)
-{
-    Buf& b = c->bufs[name];
-    if (bytes == 0) bytes = 256;
-    if (b.cap < bytes) {
-        if (b.p) HIPOK(hipFree(b.p));
-        b.p = nullptr;
-        size_t want = bytes + bytes / 4;
-        hipError_t e = hipMalloc(&b.p, want);
-        if (e != hipSuccess) {
-            want = bytes;
-            e = hipMalloc(&b.p, want);
-        }
-        if (e != hipSuccess) {
-            b.p = nullptr;
-            b.cap = 0;
-            (void)hipGetLastError();
-            return fail(TG_ENOMEM, "device allocation of %zu bytes for %s failed", bytes, name);
-        }
-        b.cap = want;
-    }
-    *out = b.p;
-    return TG_OK;
-}
-
-template <class T>
-int ensure_t(tg_ctx* c, const char* name, size_t count, T** out)
-{
-    void* p = nullptr;
-    const int rc = ensure(c, name, count * sizeof(T), &p);
-    *out = static_cast<T*>(p);
-    return rc;
-}
-
-hipEvent_t take_event(tg_ctx* c)
-{
-    if (!c->ev_pool.empty()) {
-        hipEvent_t e = c->ev_pool.back();
-        c->ev_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    hipEventCreate(&e);
-    return e;
-}
-
-// Kernel timing on the context stream: events bracket each launch.
-struct Prof {
-    tg_ctx* c;
-    const char* name;
-    hipEvent_t a = nullptr;
-    Prof(tg_ctx* cc, const char* n) : c(cc), name(n)
-    {
-        if (c->profiling) {
-            a = take_event(c);
-            hipEventRecord(a, c->stream);
-        }
-    }
-    ~Prof()
-    {
-        if (c->profiling) {
-            hipEvent_t b = take_event(c);
-            hipEventRecord(b, c->stream);
-            c->pending.push_back({name, {a, b}});
-        }
-    }
-};
-
-void collect_prof(tg_ctx* c)
-{
-    if (c->pending.empty()) return;
-    hipStreamSynchronize(c->stream);
-    for (auto& pe : c->pending) {
-        float ms = 0;
-        hipEventElapsedTime(&ms, pe.second.first, pe.second.second);
-        KTime& kt = c->ktimes[pe.first];
-        kt.ms += ms;
-        kt.n += 1;
-        c->ev_pool.push_back(pe.second.first);
-        c->ev_pool.push_back(pe.second.second);
-    }
-    c->pending.clear();
-}
-
-int check_params(const tg_params* p, uint64_t n_bases, uint64_t n_chunks)
-{
-    if (!p) return fail(TG_EINVAL, "null parameters");
-    if (p->k < 2 || p->k > 64)
-        return fail(TG_EINVAL, "k-mer length (-k %u) must be in [2, 64] on the GPU engine", p->k);
-    if (p->m < 1 || p->m > 32)
-        return fail(TG_EINVAL, "minimizer length (-m %u) must be in [1, 32]", p->m);
-    if (p->m >= p->k)
-        return fail(TG_EINVAL,
-                    "minimizer length (-m %u) must be strictly less than k-mer length (-k %u)",
-                    p->m, p->k);
-    if (n_bases >= (1ull << 32) - 64)
-        return fail(TG_EINVAL, "input of %llu bases exceeds the 2^32 per-job limit",
-                    (unsigned long long)n_bases);
-    if (n_chunks >= (1ull << 32) - 1)
-        return fail(TG_EINVAL, "too many chunks (%llu)", (unsigned long long)n_chunks);
-    return TG_OK;
-}
-
 
 // ---------------------------------------------------------------------------
 struct Phase1 {
@@ -287,6 +113,7 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     a.capk = 256u - o.k;
     a.lookback = a.capk + 1u;
     a.log2n = o.log2n;
+    a.xb = tg::HSEL_BITS - o.log2n;
     a.rec = rec;
     a.run_ptr = run_ptr;
     a.run_cnt = run_cnt;
@@ -637,6 +464,7 @@ void tg_ctx_destroy(tg_ctx* c)
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    destroy_batch(c->batch);
     for (auto& kv : c->bufs)
         if (kv.second.p) hipFree(kv.second.p);
     for (auto& pe : c->pending) {

@@ -13,6 +13,8 @@
 #include "tg_device.h"
 #include "tg_kernels.h"
 
+#include <algorithm>
+
 namespace tg {
 
 // ============================================================== pack ====
@@ -22,23 +24,50 @@ __device__ __forceinline__ uint32_t pack4(uint32_t x)
     return ((t & 3u) << 6) | (((t >> 8) & 3u) << 4) | (((t >> 16) & 3u) << 2) | (t >> 24);
 }
 
+// Word i of the packed stream: bases 16i..16i+15 (zero past n).
+__device__ __forceinline__ uint32_t pack_word(const uint8_t* __restrict__ b, uint64_t n, uint64_t i,
+                                              int aligned)
+{
+    const uint64_t b0 = i * 16u;
+    uint32_t w = 0;
+    if (aligned && b0 + 16u <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(b + b0);
+        w = (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
+    } else {
+        for (uint32_t j = 0; j < 16u; ++j)
+            if (b0 + j < n) w |= kache_code(b[b0 + j]) << (30u - 2u * j);
+    }
+    return w;
+}
+
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ b, uint64_t n,
                                               uint32_t* __restrict__ out, uint64_t n_words,
                                               int aligned)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += stride) {
-        const uint64_t b0 = i * 16u;
-        uint32_t w = 0;
-        if (aligned && b0 + 16u <= n) {
-            const uint4 v = *reinterpret_cast<const uint4*>(b + b0);
-            w = (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
-        } else {
-            for (uint32_t j = 0; j < 16u; ++j)
-                if (b0 + j < n) w |= kache_code(b[b0 + j]) << (30u - 2u * j);
-        }
-        out[i] = w;
-    }
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += stride)
+        out[i] = pack_word(b, n, i, aligned);
+}
+
+// Batch variant: bases and their count come from the batch descriptor.
+__global__ __launch_bounds__(256) void k_pack_desc(const BatchDesc* __restrict__ d,
+                                                   uint32_t* __restrict__ out, uint64_t cap_words)
+{
+    const uint8_t* b = d->bases;
+    const uint64_t n = d->n_bases;
+    const uint64_t nw = min(cap_words, (n + 15) / 16 + 8);
+    const int aligned = ((uintptr_t)b & 15u) == 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw; i += stride)
+        out[i] = pack_word(b, n, i, aligned);
+}
+
+void launch_pack_desc(hipStream_t st, const BatchDesc* d, uint32_t* out, uint64_t cap_words)
+{
+    uint64_t blocks = (cap_words + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    k_pack_desc<<<(unsigned)blocks, 256, 0, st>>>(d, out, cap_words);
 }
 
 void launch_pack(hipStream_t st, const uint8_t* bases, uint64_t n, uint32_t* out, uint64_t n_words)
@@ -56,7 +85,8 @@ __global__ void k_chunk_counts(const uint64_t* __restrict__ off, uint64_t nch, u
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nch; c += stride) {
-        const uint64_t L = off[c + 1] - off[c];
+        // a decreasing offset pair is an empty chunk (keeps every later index in bounds)
+        const uint64_t L = off[c + 1] > off[c] ? off[c + 1] - off[c] : 0;
         const uint64_t K = L >= k ? L - k + 1 : 0;
         nkm[c] = K;
         nstr[c] = (K + S - 1) / S;
@@ -228,6 +258,7 @@ struct SkLane {
     uint32_t t0, P0;
     uint64_t H0, prevH;
     uint32_t hsel;
+    uint32_t rbase, rl;         // bucket base and bucket bits of the strip's job
     bool have, done, flagged, tail;
     uint64_t cb;
 };
@@ -249,15 +280,16 @@ __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uin
     L.rv = rotr1(L.rv) ^ tr[ix];
 }
 
-// Routing field of a record (18 bits of the minimizer hash H): the reference
-// partition H & (n-1) on top (src/partition_hash.hpp:549-553), then the next
-// 18-log2(n) bits of H.  Its top log2(NSP) bits are the device sub-partition,
-// the low SB bits split large sub-partitions in Phase 2.
-__device__ __forceinline__ uint32_t route_hsel(uint64_t H, uint32_t log2n)
+// Routing field of a record (18 bits of the minimizer hash H): the bucket
+// base + (H & (2^L - 1)) on top, then the next XB bits of H.  A single job
+// has base 0 and L = log2(n), so the reference partition H & (n-1)
+// (src/partition_hash.hpp:549-553) is the top of the field and the bits below
+// it pick the device sub-partition and split large ones in Phase 2.  In a batch
+// every job owns its own bucket range, so no table ever mixes two jobs.
+__device__ __forceinline__ uint32_t route_hsel(uint64_t H, uint32_t base, uint32_t L, uint32_t xb)
 {
     const uint32_t h = (uint32_t)H;
-    const uint32_t rest = HSEL_BITS - log2n;
-    return ((h & ((1u << log2n) - 1u)) << rest) | ((h >> log2n) & ((1u << rest) - 1u));
+    return ((base + (h & ((1u << L) - 1u))) << xb) | ((h >> L) & ((1u << xb) - 1u));
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
@@ -320,7 +352,7 @@ __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, const SkStag
     L.t0 = start ? t : L.t0;
     L.H0 = start ? Hc : L.H0;
     L.P0 = start ? Pc : L.P0;
-    L.hsel = start ? route_hsel(Hc, a.log2n) : L.hsel;
+    L.hsel = start ? route_hsel(Hc, L.rbase, L.rl, a.xb) : L.hsel;
     L.have = L.have || start;
     L.prevH = Hc;
     const bool last = !stop && t == L.K - 1u;
@@ -333,6 +365,14 @@ __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, const SkStag
 __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, const uint64_t* s_init)
 {
     const uint32_t c = a.strip_chunk[s];
+    if (a.job_route) {
+        const uint32_t r = a.job_route[a.chunk_job[c]];
+        L.rbase = r & 0xFFFFu;
+        L.rl = r >> 16;
+    } else {
+        L.rbase = 0;
+        L.rl = a.log2n;
+    }
     L.tb = a.strip_tb[s];
     L.cb = a.off[c];
     const uint32_t len = (uint32_t)(a.off[c + 1] - L.cb);
@@ -414,17 +454,27 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
     __syncthreads();
 
     const uint32_t idx = blockIdx.x * SK_BLOCK + threadIdx.x;
+    // strips (or flagged strips) of this launch; with device-side sizes the
+    // launch covers a capacity and the live count is read here
+    uint32_t n = a.n;
+    if (a.n_dev) n = (uint32_t)min((uint64_t)n, *a.n_dev);
+    if (a.fix_dev) {
+        const uint32_t nf = *a.flag_count;
+        if (nf > n && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.err, 1u);
+        n = min(n, nf);
+    }
     SkStage S;
     S.buf = s_buf[wv];
     S.wn = &s_wn[wv];
     S.wg_n = &s_wgn;
     // main pass: the run is the slot region of this block's strips; fix-up:
     // a private region of SK_BLOCK*S records per block
+    const uint32_t first = blockIdx.x * SK_BLOCK;
     S.run = a.list ? a.rec + (uint64_t)blockIdx.x * SK_BLOCK * a.S
-                   : a.rec + a.strip_slot[blockIdx.x * SK_BLOCK];
+                   : (first < n ? a.rec + a.strip_slot[first] : a.rec);
     bool tail = false;
     uint2 tail_rec = make_uint2(0u, 0u);
-    if (idx < a.n) {
+    if (idx < n) {
         const uint32_t s = a.list ? a.list[idx] : idx;
         SkLane L;
         sk_init(L, a, s, s_init);
@@ -515,15 +565,41 @@ constexpr uint32_t ROUTE_T = 1024;
 
 constexpr uint32_t ROUTE_RUNS = 1024;  // run descriptors cached per tile (LDS: 16 KiB)
 
+// Tile geometry: records per tile and live tiles.  With a device-side record
+// total the tile is the smallest power of two >= 16384 that keeps the tile
+// count within max_tiles (the rule the host applies for exact launches).
+struct RouteGeom {
+    uint64_t n_rec;
+    uint32_t tile, ntiles;
+};
+
+__device__ __forceinline__ RouteGeom route_geom(const RouteArgs& a)
+{
+    RouteGeom g;
+    if (a.n_rec_dev) {
+        g.n_rec = *a.n_rec_dev;
+        uint64_t t = 16384;
+        while ((g.n_rec + t - 1) / t > a.max_tiles) t <<= 1;
+        g.tile = (uint32_t)t;
+        g.ntiles = (uint32_t)((g.n_rec + t - 1) / t);
+    } else {
+        g.n_rec = a.n_rec;
+        g.tile = a.tile;
+        g.ntiles = gridDim.x;
+    }
+    return g;
+}
+
 // Calls f(record) for every record of this block's tile.  The tile's run
 // descriptors are cached in LDS; each wave walks 64*U consecutive records at
 // a time and every lane finds its run by a short forward scan.
 template <class F>
-__device__ __forceinline__ void for_tile_records(const RouteArgs& a, uint64_t* s_off, uint2** s_ptr,
+__device__ __forceinline__ void for_tile_records(const RouteArgs& a, const RouteGeom& g,
+                                                 uint64_t* s_off, uint2** s_ptr,
                                                  uint32_t* s_meta, F&& f)
 {
-    const uint64_t lo = (uint64_t)blockIdx.x * a.tile;
-    const uint64_t hi = min(lo + a.tile, a.n_rec);
+    const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
+    const uint64_t hi = min(lo + g.tile, g.n_rec);
     if (threadIdx.x == 0) {
         uint32_t l = 0, h = a.n_runs;
         while (l < h) {  // first run with run_off[r+1] > lo
@@ -580,12 +656,14 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
     __shared__ uint64_t s_off[ROUTE_RUNS + 1];
     __shared__ uint2* s_ptr[ROUTE_RUNS];
     __shared__ uint32_t s_meta[4];
+    const RouteGeom g = route_geom(a);
+    if (blockIdx.x >= g.ntiles) return;
     const uint32_t nsp = a.nsp;
     for (uint32_t i = threadIdx.x; i < 2u * nsp; i += blockDim.x) sh[i] = 0;
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t k1 = a.k - 1u;
-    for_tile_records(a, s_off, s_ptr, s_meta, [&](uint2 v) {
+    for_tile_records(a, g, s_off, s_ptr, s_meta, [&](uint2 v) {
         const uint32_t pid = v.y >> shift;
         atomicAdd(&sh[pid], 1u);
         atomicAdd(&sh[nsp + pid], (v.y & REC_LEN_MASK) - k1);
@@ -599,11 +677,28 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
     }
 }
 
+__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint2* tmp);
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint2* tmp);
+
+// Dynamic-LDS limits of the routing kernels, set once per process at their
+// largest sizes (2^14 sub-partitions), so no attribute call happens while a
+// launch sequence is being captured.
+void set_kernel_attributes()
+{
+    static bool done = false;
+    if (done) return;
+    done = true;
+    const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
+    const int cur = (int)((1u << 14) * sizeof(uint32_t));
+    (void)hipFuncSetAttribute((const void*)k_route_hist, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
+    (void)hipFuncSetAttribute((const void*)k_route_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+    (void)hipFuncSetAttribute((const void*)k_route_fine, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+}
+
 void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles)
 {
     if (ntiles == 0) return;
-    (void)hipFuncSetAttribute((const void*)k_route_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2u * a.nsp * sizeof(uint32_t)));
+    set_kernel_attributes();
     k_route_hist<<<ntiles, ROUTE_T, 2u * a.nsp * sizeof(uint32_t), st>>>(a);
 }
 
@@ -615,6 +710,7 @@ __global__ __launch_bounds__(256) void k_route_colscan(RouteArgs a, uint32_t nti
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nsp) return;
+    if (a.n_rec_dev) ntiles = route_geom(a).ntiles;
     uint32_t* __restrict__ hs = a.hsk;
     const uint32_t* __restrict__ hk = a.hkm;
     uint64_t run = 0, km = 0;
@@ -659,6 +755,8 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     __shared__ uint64_t s_off[ROUTE_RUNS + 1];
     __shared__ uint2* s_ptr[ROUTE_RUNS];
     __shared__ uint32_t s_meta[4];
+    const RouteGeom g = route_geom(a);
+    if (blockIdx.x >= g.ntiles) return;
     const uint32_t nc = a.nsp >> log2f, F = 1u << log2f;
     const uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * a.nsp;
     for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
@@ -668,7 +766,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     }
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb + log2f;
-    for_tile_records(a, s_off, s_ptr, s_meta, [&](uint2 v) {
+    for_tile_records(a, g, s_off, s_ptr, s_meta, [&](uint2 v) {
         const uint32_t pos = atomicAdd(&cur[v.y >> shift], 1u);
         tmp[pos] = v;
     });
@@ -711,11 +809,8 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     uint32_t log2f = 0;
     while ((a.nsp >> log2f) > 256u) ++log2f;
     const uint32_t nc = a.nsp >> log2f;
-    (void)hipFuncSetAttribute((const void*)k_route_coarse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(nc * sizeof(uint32_t)));
+    set_kernel_attributes();
     k_route_coarse<<<ntiles, ROUTE_T, nc * sizeof(uint32_t), st>>>(a, log2f, tmp);
-    (void)hipFuncSetAttribute((const void*)k_route_fine, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)((1u << log2f) * sizeof(uint32_t)));
     k_route_fine<<<nc, ROUTE_T, (1u << log2f) * sizeof(uint32_t), st>>>(a, log2f, tmp);
 }
 
@@ -793,6 +888,50 @@ __device__ __noinline__ bool spill_insert2(const CountArgs& a, uint64_t hi, uint
         const uint64_t kh = __hip_atomic_load(&a.sp_keys[2 * slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t kl = __hip_atomic_load(&a.sp_keys[2 * slot + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (kh == hi && kl == lo) {
+            atomicAdd(cp, 1u);
+            return true;
+        }
+        slot = (slot + 1) & a.sp_mask;
+        ++p;
+    }
+    atomicOr(a.sp_err, 1u);
+    return false;
+}
+
+// Global spill table of a batch: SW words per key (the k-mer words and the
+// job tag), count word as slot state as in spill_insert2.
+struct SpKey {
+    uint64_t w[3];
+};
+
+template <int SW>
+__device__ __noinline__ bool spill_insert_w(const CountArgs& a, SpKey key)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+    for (int i = 0; i < SW; ++i) h = mix64(h ^ key.w[i]);
+    uint64_t slot = h & a.sp_mask;
+    for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16);) {
+        uint32_t* cp = &a.sp_cnt[slot];
+        const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == 0u) {
+            if (atomicCAS(cp, 0u, LOCKED32) == 0u) {
+#pragma unroll
+                for (int i = 0; i < SW; ++i)
+                    __hip_atomic_store(&a.sp_keys[SW * slot + i], key.w[i], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return true;
+            }
+            continue;
+        }
+        if (c == LOCKED32) continue;
+        bool eq = true;
+#pragma unroll
+        for (int i = 0; i < SW; ++i)
+            eq = eq && __hip_atomic_load(&a.sp_keys[SW * slot + i], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) == key.w[i];
+        if (eq) {
             atomicAdd(cp, 1u);
             return true;
         }
@@ -986,7 +1125,11 @@ constexpr uint32_t COUNT_STAGE = 512;  // packed words staged per wave
 // (wave scan) cut into 64 equal pieces, one per lane.  A lane finds its first
 // record once (binary search) and rolls through consecutive k-mers from LDS,
 // re-seeding only when it crosses into the next record.
-template <int NW, uint32_t CAP>
+//
+// BATCH: a persistent launch; workgroups take items from a device queue until
+// the device-side item count is reached, spill keys carry the item's job, and
+// every item records its output segment for the per-job gather.
+template <int NW, uint32_t CAP, bool BATCH>
 __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 {
     constexpr uint32_t NWAVE = COUNT_BLOCK / 64;
@@ -1000,172 +1143,223 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_next;
+    __shared__ uint32_t s_item;
     __shared__ unsigned long long s_base;
-
-    const uint4 it = a.items[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) {
-        if (NW == 1) s_k0[i] = EMPTY64;
-        s_cnt[i] = 0;
-    }
-    if (threadIdx.x == 0) s_next = it.x;
-    __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t k = a.k;
-    const uint32_t r1 = it.y, smask = it.z, sval = it.w;
     const uint32_t pad = NW == 1 ? 3u : 5u;  // words read past a record's last base word
     const uint32_t* stage = s_stage[wv];
-    uint32_t spilled = 0;
-    for (;;) {
-        uint32_t rb = 0;
-        if (lane == 0) rb = atomicAdd(&s_next, 64u);
-        rb = __shfl(rb, 0, 64);
-        if (rb >= r1) break;
-        const uint32_t i = rb + lane;
-        uint32_t nk = 0, st = 0, len = 0;
-        if (i < r1) {
-            const uint2 rr = a.rec[i];
-            if (((rr.y >> REC_HSEL_SHIFT) & smask) == sval) {
-                len = rr.y & REC_LEN_MASK;
-                nk = len - k + 1u;
-                st = rr.x;
-            }
-        }
-        bool pending = nk > 0;
-        while (__ballot(pending)) {
-            const uint32_t w0 = st >> 4;
-            const uint32_t nwd = pending ? ((st + len - 1u) >> 4) - w0 + pad : 0u;
-            const uint32_t wincl = wave_incl_u32(nwd, lane);
-            const bool take = pending && wincl <= COUNT_STAGE;
-            const uint32_t wex = wincl - nwd;
-            if (take) {
-                const uint32_t* g = a.packed + w0;
-                uint32_t j = 0;
-                for (; j + 4 <= nwd; j += 4) {
-                    const uint32_t v0 = g[j], v1 = g[j + 1], v2 = g[j + 2], v3 = g[j + 3];
-                    s_stage[wv][wex + j] = v0;
-                    s_stage[wv][wex + j + 1] = v1;
-                    s_stage[wv][wex + j + 2] = v2;
-                    s_stage[wv][wex + j + 3] = v3;
-                }
-                for (; j < nwd; ++j) s_stage[wv][wex + j] = g[j];
-            }
-            const uint32_t incl = wave_incl_u32(take ? nk : 0u, lane);
-            s_incl[wv][lane] = incl;
-            s_wb[wv][lane] = wex;
-            s_st[wv][lane] = st;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            {
-                // seed of this lane's own record, computed by every lane at once so
-                // that crossing into a record later is two LDS reads
-                KRoll<NW> Q;
-                kroll_init<NW>(Q, stage, take ? wex : 0u, take ? (st & 15u) : 0u, k);
-                s_seed[wv][lane][0] = Q.fl;
-                s_seed[wv][lane][1] = Q.rl;
-                if (NW == 2) {
-                    s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
-                    s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
-                }
-            }
-            const uint32_t total = __shfl(incl, 63, 64);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const uint32_t q = (total + 63u) >> 6;
-            const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
-            if (j0 < j1) {
-                uint32_t s = 0;
-#pragma unroll
-                for (uint32_t step = 32; step; step >>= 1)
-                    if (s_incl[wv][s + step - 1u] <= j0) s += step;
-                const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
-                uint32_t end = s_incl[wv][s];
-                KRoll<NW> R;
-                {
-                    const uint32_t rs = s_st[wv][s], gb = rs + (j0 - ex);
-                    kroll_init<NW>(R, stage, s_wb[wv][s] + (gb >> 4) - (rs >> 4), gb & 15u, k);
-                }
-                for (uint32_t j = j0;;) {
-                    bool ok;
-                    if (NW == 1) {
-                        const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
-                        ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
-                        if (!ok) spill_insert1(a, x);
-                    } else {
-                        const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
-                        const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
-                        ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
-                        if (!ok) spill_insert2(a, xh, xl);
-                    }
-                    spilled += ok ? 0u : 1u;
-                    if (++j == j1) break;
-                    if (j == end) {
-                        // next record with k-mers (records not taken this round have none)
-                        do {
-                            ++s;
-                            end = s_incl[wv][s];
-                        } while (end == j);
-                        const uint32_t rs = s_st[wv][s];
-                        R.fl = s_seed[wv][s][0];
-                        R.rl = s_seed[wv][s][1];
-                        if (NW == 2) {
-                            R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
-                            R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
-                        }
-                        const uint32_t q2 = (rs & 15u) + k;
-                        R.wi = s_wb[wv][s] + (q2 >> 4);
-                        R.wrd = stage[R.wi] << (2u * (q2 & 15u));
-                        R.rem = 16u - (q2 & 15u);
-                    } else {
-                        kroll_step<NW>(R, stage, k);
-                    }
-                }
-            }
-            pending = pending && !take;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+    uint32_t item = blockIdx.x;
+    uint32_t n_items = a.n_items;
+    if (BATCH) {
+        n_items = *a.n_items_dev;
+        if (threadIdx.x == 0) s_item = atomicAdd(a.item_q, 1u);
+        __syncthreads();
+        item = s_item;
     }
-    __syncthreads();
+    while (item < n_items) {
+        const uint4 it = a.items[item];
+        const uint32_t tag = BATCH ? a.item_job[item] : 0u;
+        for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) {
+            if (NW == 1) s_k0[i] = EMPTY64;
+            s_cnt[i] = 0;
+        }
+        if (threadIdx.x == 0) s_next = it.x;
+        __syncthreads();
 
-    // drain: count, reserve once per workgroup, write
-    uint32_t mine = 0, uniq = 0;
-    for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
-        const uint32_t c = s_cnt[s];
-        if (c) {
-            ++uniq;
-            if (c >= a.ci && (uint64_t)c <= a.cx) ++mine;
+        const uint32_t r1 = it.y, smask = it.z, sval = it.w;
+        uint32_t spilled = 0;
+        for (;;) {
+            uint32_t rb = 0;
+            if (lane == 0) rb = atomicAdd(&s_next, 64u);
+            rb = __shfl(rb, 0, 64);
+            if (rb >= r1) break;
+            const uint32_t i = rb + lane;
+            uint32_t nk = 0, st = 0, len = 0;
+            if (i < r1) {
+                const uint2 rr = a.rec[i];
+                if (((rr.y >> REC_HSEL_SHIFT) & smask) == sval) {
+                    len = rr.y & REC_LEN_MASK;
+                    nk = len - k + 1u;
+                    st = rr.x;
+                }
+            }
+            bool pending = nk > 0;
+            while (__ballot(pending)) {
+                const uint32_t w0 = st >> 4;
+                const uint32_t nwd = pending ? ((st + len - 1u) >> 4) - w0 + pad : 0u;
+                const uint32_t wincl = wave_incl_u32(nwd, lane);
+                const bool take = pending && wincl <= COUNT_STAGE;
+                const uint32_t wex = wincl - nwd;
+                if (take) {
+                    const uint32_t* g = a.packed + w0;
+                    uint32_t j = 0;
+                    for (; j + 4 <= nwd; j += 4) {
+                        const uint32_t v0 = g[j], v1 = g[j + 1], v2 = g[j + 2], v3 = g[j + 3];
+                        s_stage[wv][wex + j] = v0;
+                        s_stage[wv][wex + j + 1] = v1;
+                        s_stage[wv][wex + j + 2] = v2;
+                        s_stage[wv][wex + j + 3] = v3;
+                    }
+                    for (; j < nwd; ++j) s_stage[wv][wex + j] = g[j];
+                }
+                const uint32_t incl = wave_incl_u32(take ? nk : 0u, lane);
+                s_incl[wv][lane] = incl;
+                s_wb[wv][lane] = wex;
+                s_st[wv][lane] = st;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                {
+                    // seed of this lane's own record, computed by every lane at once so
+                    // that crossing into a record later is two LDS reads
+                    KRoll<NW> Q;
+                    kroll_init<NW>(Q, stage, take ? wex : 0u, take ? (st & 15u) : 0u, k);
+                    s_seed[wv][lane][0] = Q.fl;
+                    s_seed[wv][lane][1] = Q.rl;
+                    if (NW == 2) {
+                        s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
+                        s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
+                    }
+                }
+                const uint32_t total = __shfl(incl, 63, 64);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t q = (total + 63u) >> 6;
+                const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
+                if (j0 < j1) {
+                    uint32_t s = 0;
+    #pragma unroll
+                    for (uint32_t step = 32; step; step >>= 1)
+                        if (s_incl[wv][s + step - 1u] <= j0) s += step;
+                    const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
+                    uint32_t end = s_incl[wv][s];
+                    KRoll<NW> R;
+                    {
+                        const uint32_t rs = s_st[wv][s], gb = rs + (j0 - ex);
+                        kroll_init<NW>(R, stage, s_wb[wv][s] + (gb >> 4) - (rs >> 4), gb & 15u, k);
+                    }
+                    for (uint32_t j = j0;;) {
+                        bool ok;
+                        if (NW == 1) {
+                            const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
+                            ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
+                            if (!ok) {
+                                if (BATCH) spill_insert_w<2>(a, SpKey{{(uint64_t)tag, x, 0ull}});
+                                else spill_insert1(a, x);
+                            }
+                        } else {
+                            const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
+                            const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
+                            ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
+                            if (!ok) {
+                                if (BATCH) spill_insert_w<3>(a, SpKey{{xh, xl, (uint64_t)tag}});
+                                else spill_insert2(a, xh, xl);
+                            }
+                        }
+                        spilled += ok ? 0u : 1u;
+                        if (++j == j1) break;
+                        if (j == end) {
+                            // next record with k-mers (records not taken this round have none)
+                            do {
+                                ++s;
+                                end = s_incl[wv][s];
+                            } while (end == j);
+                            const uint32_t rs = s_st[wv][s];
+                            R.fl = s_seed[wv][s][0];
+                            R.rl = s_seed[wv][s][1];
+                            if (NW == 2) {
+                                R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
+                                R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
+                            }
+                            const uint32_t q2 = (rs & 15u) + k;
+                            R.wi = s_wb[wv][s] + (q2 >> 4);
+                            R.wrd = stage[R.wi] << (2u * (q2 & 15u));
+                            R.rem = 16u - (q2 & 15u);
+                        } else {
+                            kroll_step<NW>(R, stage, k);
+                        }
+                    }
+                }
+                pending = pending && !take;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
-    }
-    uint32_t tot_out, tot_uniq, tot_sp;
-    const uint32_t my_off = block_excl_u32(mine, s_red, tot_out);
-    block_excl_u32(uniq, s_red, tot_uniq);
-    block_excl_u32(spilled, s_red, tot_sp);
-    if (threadIdx.x == 0) {
-        s_base = tot_out ? atomicAdd(&a.counters[0], (unsigned long long)tot_out) : 0ull;
-        if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
-        if (tot_sp) atomicAdd(&a.counters[2], (unsigned long long)tot_sp);
-        if (a.item_uniq) a.item_uniq[blockIdx.x] = tot_uniq + tot_sp;
-    }
-    __syncthreads();
-    uint64_t pos = s_base + my_off;
-    for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
-        const uint32_t c = s_cnt[s];
-        if (c && c >= a.ci && (uint64_t)c <= a.cx) {
-            emit_out<NW>(a, pos, NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s], c);
-            ++pos;
+        __syncthreads();
+
+        // drain: count, reserve once per workgroup, write
+        uint32_t mine = 0, uniq = 0;
+        for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
+            const uint32_t c = s_cnt[s];
+            if (c) {
+                ++uniq;
+                if (c >= a.ci && (uint64_t)c <= a.cx) ++mine;
+            }
         }
+        uint32_t tot_out, tot_uniq, tot_sp;
+        const uint32_t my_off = block_excl_u32(mine, s_red, tot_out);
+        block_excl_u32(uniq, s_red, tot_uniq);
+        block_excl_u32(spilled, s_red, tot_sp);
+        if (threadIdx.x == 0) {
+            s_base = tot_out ? atomicAdd(&a.counters[0], (unsigned long long)tot_out) : 0ull;
+            if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
+            if (tot_sp) atomicAdd(&a.counters[2], (unsigned long long)tot_sp);
+            if (a.item_uniq) a.item_uniq[item] = tot_uniq + tot_sp;
+            if (BATCH) {
+                a.seg_pos[item] = s_base;
+                a.seg_n[item] = tot_out;
+                if (tot_uniq) atomicAdd(&a.job_uniq[tag], (unsigned long long)tot_uniq);
+            }
+        }
+        __syncthreads();
+        uint64_t pos = s_base + my_off;
+        for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
+            const uint32_t c = s_cnt[s];
+            if (c && c >= a.ci && (uint64_t)c <= a.cx) {
+                emit_out<NW>(a, pos, NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s], c);
+                ++pos;
+            }
+        }
+        if (!BATCH) break;
+        __syncthreads();  // table reads done before the next item clears it
+        if (threadIdx.x == 0) s_item = atomicAdd(a.item_q, 1u);
+        __syncthreads();
+        item = s_item;
     }
 }
 
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw)
 {
     if (a.n_items == 0) return;
-    if (nw == 1) k_count<1, LDS_CAP_NW1><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
-    else k_count<2, LDS_CAP_NW2><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
+    if (nw == 1) k_count<1, LDS_CAP_NW1, false><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, false><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
+}
+
+// Resident workgroups of the persistent count launch: occupancy x CUs.
+uint32_t count_persistent_blocks(uint32_t nw)
+{
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (nw == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, (const void*)k_count<1, LDS_CAP_NW1, true>, COUNT_BLOCK, 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, (const void*)k_count<2, LDS_CAP_NW2, true>, COUNT_BLOCK, 0);
+    if (cus <= 0) cus = 256;
+    if (per <= 0) per = 1;
+    return (uint32_t)(cus * per);
+}
+
+void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks)
+{
+    if (nw == 1) k_count<1, LDS_CAP_NW1, true><<<blocks, COUNT_BLOCK, 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, true><<<blocks, COUNT_BLOCK, 0, st>>>(a);
 }
 
 template <int NW>
@@ -1212,6 +1406,260 @@ void launch_spill_drain(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_
     if (!blocks) return;
     if (nw == 1) k_spill_drain<1><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots);
     else k_spill_drain<2><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots);
+}
+
+// ============================================================ batches ====
+// Largest j in [0, n) with v[j] <= x (v non-decreasing, v[0] <= x).
+__device__ __forceinline__ uint32_t last_le(const uint32_t* v, uint32_t n, uint64_t x, uint32_t mask)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((v[mid] & mask) <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Batch-relative chunk offsets (padded to the capacity with empty chunks), the
+// job of every chunk and the job of every bucket.
+__global__ void k_batch_setup(const BatchDesc* __restrict__ d, BatchJobs j, uint32_t cap_jobs,
+                              uint64_t* __restrict__ boff, uint32_t* __restrict__ chunk_job,
+                              uint64_t cap_ch, uint32_t* __restrict__ bucket_job, uint32_t nsp,
+                              uint32_t* __restrict__ err)
+{
+    const uint64_t nch = min(d->n_chunks, cap_ch), N = d->n_bases, ob = d->off_base;
+    const uint32_t nj = min(d->n_jobs, cap_jobs);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint64_t i = t0; i <= cap_ch; i += stride) {
+        uint64_t v = N;
+        if (i <= nch) {
+            // offsets outside the batch's bases are clamped and reported
+            const uint64_t o = d->off[i];
+            const bool bad = o < ob || o - ob > N || (i == nch && o - ob != N);
+            if (bad) atomicOr(err, 4u);
+            v = o < ob ? 0 : min(o - ob, N);
+        }
+        boff[i] = v;
+        if (i < cap_ch) chunk_job[i] = (i < nch && nj) ? last_le(j.job_chunk, nj, i, ~0u) : 0u;
+    }
+    for (uint64_t b = t0; b < nsp; b += stride) {
+        uint32_t job = 0xFFFFFFFFu;
+        if (nj) {
+            const uint32_t q = last_le(j.job_route, nj, b, 0xFFFFu);
+            const uint32_t r = j.job_route[q];
+            if (b < (r & 0xFFFFu) + (1u << (r >> 16))) job = q;
+        }
+        bucket_job[b] = job;
+    }
+}
+
+void launch_batch_setup(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
+                        uint64_t* boff, uint32_t* chunk_job, uint64_t cap_ch, uint32_t* bucket_job,
+                        uint32_t nsp, uint32_t* err)
+{
+    uint64_t blocks = (std::max<uint64_t>(cap_ch + 1, nsp) + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    k_batch_setup<<<(unsigned)blocks, 256, 0, st>>>(d, j, cap_jobs, boff, chunk_job, cap_ch,
+                                                     bucket_job, nsp, err);
+}
+
+// Work items of a batch, built on the device: every non-empty bucket is one
+// item, split by the low routing bits into r = 2^i items when its estimated
+// distinct k-mers (k-mers x scale, scale = rho / target) exceed one table.
+// Buckets never merge, so an item never spans two jobs.
+__global__ __launch_bounds__(1024) void k_build_items(const uint64_t* __restrict__ part_km,
+                                                      const uint64_t* __restrict__ part_off,
+                                                      uint32_t nsp, uint32_t xb,
+                                                      const uint32_t* __restrict__ bucket_job,
+                                                      const float* __restrict__ scale_p,
+                                                      uint4* __restrict__ items,
+                                                      uint32_t* __restrict__ item_job,
+                                                      uint64_t* __restrict__ bucket_item,
+                                                      uint32_t* __restrict__ n_items)
+{
+    __shared__ uint32_t s_red[16];
+    const float scale = *scale_p;
+    const uint32_t per = (nsp + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(threadIdx.x * per, nsp), b1 = min(b0 + per, nsp);
+    const uint32_t rmax = 1u << xb;
+    auto splits = [&](uint32_t b) -> uint32_t {
+        if (part_off[b + 1] == part_off[b]) return 0u;
+        const float est = (float)part_km[b] * scale;
+        uint32_t r = 1;
+        while (r < rmax && est > (float)r) r <<= 1;
+        return r;
+    };
+    uint32_t mine = 0;
+    for (uint32_t b = b0; b < b1; ++b) mine += splits(b);
+    uint32_t total;
+    uint32_t cur = block_excl_u32(mine, s_red, total);
+    for (uint32_t b = b0; b < b1; ++b) {
+        bucket_item[b] = cur;
+        const uint32_t r = splits(b);
+        const uint32_t lo = (uint32_t)part_off[b], hi = (uint32_t)part_off[b + 1];
+        const uint32_t job = bucket_job[b];
+        for (uint32_t q = 0; q < r; ++q) {
+            items[cur + q] = make_uint4(lo, hi, r - 1u, q);
+            item_job[cur + q] = job;
+        }
+        cur += r;
+    }
+    if (threadIdx.x == 0) {
+        bucket_item[nsp] = total;
+        *n_items = total;
+    }
+}
+
+void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
+                        uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
+                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items)
+{
+    k_build_items<<<1, 1024, 0, st>>>(part_km, part_off, nsp, xb, bucket_job, scale, items,
+                                      item_job, bucket_item, n_items);
+}
+
+// Spill entries per job: distinct keys, and reported keys (ci/cx filter).
+template <int SW>
+__global__ __launch_bounds__(256) void k_spill_jobs(CountArgs a, uint64_t n_slots,
+                                                    unsigned long long* __restrict__ job_sp)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n_slots; s += stride) {
+        const uint32_t c = a.sp_cnt[s];
+        if (!c) continue;
+        const uint32_t tag = (uint32_t)a.sp_keys[SW * s + (SW == 2 ? 0 : 2)];
+        atomicAdd(&a.job_uniq[tag], 1ull);
+        if (c >= a.ci && (uint64_t)c <= a.cx) atomicAdd(&job_sp[tag], 1ull);
+    }
+}
+
+void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+                       unsigned long long* job_sp)
+{
+    uint64_t blocks = (n_slots + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (!blocks) return;
+    if (nw == 1) k_spill_jobs<2><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_sp);
+    else k_spill_jobs<3><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_sp);
+}
+
+// Per job: reported keys from its items (segments) and from the spill table,
+// and its superkmers (records of its buckets).  Jobs past the batch get 0.
+__global__ void k_job_totals(const BatchDesc* __restrict__ d, BatchJobs j, uint32_t cap_jobs,
+                             const uint64_t* __restrict__ bucket_item,
+                             const uint64_t* __restrict__ seg_x,
+                             const uint64_t* __restrict__ part_off,
+                             const unsigned long long* __restrict__ job_sp,
+                             const uint64_t* __restrict__ nkm_x,
+                             uint64_t* __restrict__ job_tot, uint64_t* __restrict__ job_item_tot,
+                             uint64_t* __restrict__ job_nsk, uint64_t* __restrict__ job_nkm)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cap_jobs) return;
+    if (q >= d->n_jobs) {
+        job_tot[q] = 0;
+        job_item_tot[q] = 0;
+        job_nsk[q] = 0;
+        job_nkm[q] = 0;
+        return;
+    }
+    const uint32_t r = j.job_route[q];
+    const uint32_t b = r & 0xFFFFu, be = b + (1u << (r >> 16));
+    const uint64_t it = seg_x[bucket_item[be]] - seg_x[bucket_item[b]];
+    job_item_tot[q] = it;
+    job_tot[q] = it + job_sp[q];
+    job_nsk[q] = part_off[be] - part_off[b];
+    job_nkm[q] = nkm_x[j.job_chunk[q + 1]] - nkm_x[j.job_chunk[q]];
+}
+
+void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
+                       const uint64_t* bucket_item, const uint64_t* seg_x, const uint64_t* part_off,
+                       const unsigned long long* job_sp, const uint64_t* nkm_x, uint64_t* job_tot,
+                       uint64_t* job_item_tot, uint64_t* job_nsk, uint64_t* job_nkm)
+{
+    k_job_totals<<<(cap_jobs + 255) / 256, 256, 0, st>>>(d, j, cap_jobs, bucket_item, seg_x, part_off,
+                                                         job_sp, nkm_x, job_tot, job_item_tot,
+                                                         job_nsk, job_nkm);
+}
+
+// Item segments into job order: job j's items first (item order), its spill
+// entries after them.
+template <int NW>
+__global__ __launch_bounds__(256) void k_gather_items(CountArgs a, BatchJobs j,
+                                                      const uint64_t* __restrict__ bucket_item,
+                                                      const uint64_t* __restrict__ seg_x,
+                                                      const uint64_t* __restrict__ job_off,
+                                                      uint64_t* __restrict__ dst_keys,
+                                                      uint32_t* __restrict__ dst_cnt)
+{
+    const uint32_t n = *a.n_items_dev;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t job = a.item_job[i];
+        const uint32_t b = j.job_route[job] & 0xFFFFu;
+        const uint64_t dst = job_off[job] + seg_x[i] - seg_x[bucket_item[b]];
+        const uint64_t src = a.seg_pos[i], cnt = a.seg_n[i];
+        for (uint64_t e = threadIdx.x; e < cnt; e += blockDim.x) {
+            if (NW == 1) {
+                dst_keys[dst + e] = a.out_keys[src + e];
+            } else {
+                dst_keys[2 * (dst + e)] = a.out_keys[2 * (src + e)];
+                dst_keys[2 * (dst + e) + 1] = a.out_keys[2 * (src + e) + 1];
+            }
+            dst_cnt[dst + e] = a.out_cnt[src + e];
+        }
+    }
+}
+
+void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j,
+                         const uint64_t* bucket_item, const uint64_t* seg_x,
+                         const uint64_t* job_off, uint64_t* dst_keys, uint32_t* dst_cnt,
+                         uint32_t blocks)
+{
+    if (nw == 1)
+        k_gather_items<1><<<blocks, 256, 0, st>>>(a, j, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
+    else
+        k_gather_items<2><<<blocks, 256, 0, st>>>(a, j, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
+}
+
+template <int SW>
+__global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slots,
+                                                     const uint64_t* __restrict__ job_off,
+                                                     const uint64_t* __restrict__ job_item_tot,
+                                                     unsigned long long* __restrict__ job_cur,
+                                                     uint64_t* __restrict__ dst_keys,
+                                                     uint32_t* __restrict__ dst_cnt)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n_slots; s += stride) {
+        const uint32_t c = a.sp_cnt[s];
+        if (!c || c < a.ci || (uint64_t)c > a.cx) continue;
+        const uint32_t tag = (uint32_t)a.sp_keys[SW * s + (SW == 2 ? 0 : 2)];
+        const uint64_t pos = job_off[tag] + job_item_tot[tag] + atomicAdd(&job_cur[tag], 1ull);
+        if (SW == 2) {
+            dst_keys[pos] = a.sp_keys[SW * s + 1];
+        } else {
+            dst_keys[2 * pos] = a.sp_keys[SW * s + 1];
+            dst_keys[2 * pos + 1] = a.sp_keys[SW * s];
+        }
+        dst_cnt[pos] = c;
+    }
+}
+
+void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+                        const uint64_t* job_off, const uint64_t* job_item_tot,
+                        unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt)
+{
+    uint64_t blocks = (n_slots + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (!blocks) return;
+    if (nw == 1)
+        k_spill_place<2><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_off, job_item_tot, job_cur,
+                                                           dst_keys, dst_cnt);
+    else
+        k_spill_place<3><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_off, job_item_tot, job_cur,
+                                                           dst_keys, dst_cnt);
 }
 
 }  // namespace tg

@@ -32,7 +32,16 @@ struct SkArgs {
     uint32_t k, m, W, S;
     uint32_t capk;              // a superkmer spans at most capk k-mers (256-k)
     uint32_t lookback;          // k-mers of look-back before tb, or LOOKBACK_FULL
-    uint32_t log2n, log2s, sb;  // routing: n partitions, s sub-partitions each, SB split bits
+    uint32_t log2n, xb;         // routing field: job bucket bits on top, xb bits below
+    // batched jobs: job of each chunk and its bucket range (base | log2(buckets) << 16);
+    // nullptr for a single job, whose buckets are the reference partitions (log2n bits)
+    const uint32_t* chunk_job;
+    const uint32_t* job_route;
+    // device-side sizes (nullptr: n is exact): main pass reads the strip total,
+    // fix-up pass reads flag_count; n is then the capacity of the launch
+    const uint64_t* n_dev;
+    uint32_t fix_dev;
+    uint32_t* err;              // bit 0: more flagged strips than the fix-up capacity
     uint2* rec;                 // main pass: slot space; fix-up: S*SK_BLOCK records per block
     uint2** run_ptr;            // run table (indexed from run_base + blockIdx.x)
     uint32_t* run_cnt;
@@ -53,6 +62,10 @@ struct RouteArgs {
     uint32_t* hkm;             // [ntiles][nsp] k-mers
     const uint64_t* part_off;  // nsp+1 (scatter only)
     uint2* dst;                // partition-ordered records (scatter only)
+    // device-side record total (nullptr: n_rec/tile/grid are exact); the
+    // kernels then derive tile and tile count themselves, up to max_tiles
+    const uint64_t* n_rec_dev;
+    uint32_t max_tiles;
 };
 
 // Phase-2 work item: records [x, y) of consecutive sub-partitions; records with
@@ -77,6 +90,33 @@ struct CountArgs {
     unsigned long long* counters;  // [0] n_out, [1] n_unique, [2] spilled k-mers
     uint64_t out_cap;
     uint32_t* item_uniq;       // optional: distinct keys per item (calibration)
+    // batched jobs (persistent launch): items are taken from a device queue;
+    // spill keys carry the job as a tag; each item reports its output segment
+    const uint32_t* n_items_dev;
+    uint32_t* item_q;
+    const uint32_t* item_job;
+    uint64_t* seg_pos;
+    uint64_t* seg_n;
+    unsigned long long* job_uniq;
+};
+
+// Batch descriptor (device memory, rewritten before every batch): where the
+// batch's bases and chunk offsets are and how many there are.
+struct BatchDesc {
+    const uint8_t* bases;      // first base of the batch
+    const uint64_t* off;       // offsets of the batch's chunks (n_chunks+1, absolute)
+    uint64_t off_base;         // off[0]: subtracted to get batch-relative positions
+    uint64_t n_bases;
+    uint64_t n_chunks;
+    uint32_t n_jobs;
+    float scale;               // work-item splits: estimated distinct = k-mers x scale / table
+};
+
+// Batch job tables (device): job j owns chunks [job_chunk[j], job_chunk[j+1])
+// (batch-relative) and buckets [base, base + 2^L), job_route[j] = base | L << 16.
+struct BatchJobs {
+    const uint32_t* job_chunk;
+    const uint32_t* job_route;
 };
 
 // launchers (defined in kernels.hip)
@@ -99,10 +139,38 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n);
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw);
 void launch_spill_drain(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots);
+uint32_t count_persistent_blocks(uint32_t nw);
+
+// batched path
+void launch_pack_desc(hipStream_t st, const BatchDesc* d, uint32_t* out, uint64_t cap_words);
+void launch_batch_setup(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
+                        uint64_t* boff, uint32_t* chunk_job, uint64_t cap_ch, uint32_t* bucket_job,
+                        uint32_t nsp, uint32_t* err);
+void set_kernel_attributes();
+void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
+                        uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
+                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items);
+void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks);
+// spill entries of a batch: per-job distinct and reported counts, then placement
+void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+                       unsigned long long* job_sp);
+void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
+                       const uint64_t* bucket_item, const uint64_t* seg_x, const uint64_t* part_off,
+                       const unsigned long long* job_sp, const uint64_t* nkm_x, uint64_t* job_tot,
+                       uint64_t* job_item_tot, uint64_t* job_nsk, uint64_t* job_nkm);
+void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j,
+                         const uint64_t* bucket_item, const uint64_t* seg_x,
+                         const uint64_t* job_off, uint64_t* dst_keys, uint32_t* dst_cnt,
+                         uint32_t blocks);
+void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+                        const uint64_t* job_off, const uint64_t* job_item_tot,
+                        unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt);
 
 // table geometry of the LDS counting kernels
 constexpr uint32_t LDS_CAP_NW1 = 4096;  // 4096 x (8+4) B = 48 KiB
 constexpr uint32_t LDS_CAP_NW2 = 2048;  // 2048 x (16+4) B = 40 KiB
 constexpr uint32_t COUNT_BLOCK = 512;
+constexpr uint32_t ROUTE_MAX_TILES = 512;
+constexpr uint32_t FIX_BLOCKS = 8;      // fix-up capacity of a batch: 8 x SK_BLOCK strips
 
 }  // namespace tg
