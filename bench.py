@@ -99,6 +99,102 @@ def cpu_baseline(text, n_bases_total, threads):
             "sample": f"first {nrec} reads ({len(bases)} bases), oracle brute-force count"}
 
 
+# ------------------------------------------------------- many samples ----
+C4_SPEC = ("kind=samples_fq samples=2000 seed=4 index={i} median=300000 sigma=1.2 lo=10000 "
+           "hi=8000000 cov=10 rl=150 sub=0.005")
+C4_K, C4_M = 21, 15
+
+
+def gen_samples(T, n, threads):
+    """The first n samples of the many-sample workload, generated in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(T.synth_parsed, [C4_SPEC.format(i=i) for i in range(n)]))
+
+
+def bench_batched(T, eng, n_samples, threads):
+    """Run A: the plain per-job sequence (tg_count_device) once per sample.
+    Run B: all samples through tg_count_many_device (batched launch sequences,
+    one captured hipGraph replayed per batch).  Inputs are resident in HBM
+    before either timed region; results stay in HBM (A leaves each job's
+    result in its buffers; B moves every job's result to its result store)."""
+    t0 = time.perf_counter()
+    parts = gen_samples(T, n_samples, threads)
+    sizes = np.array([len(b) for b, _ in parts], dtype=np.uint64)
+    nchs = np.array([len(o) - 1 for _, o in parts], dtype=np.uint64)
+    job_base = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    job_chunk = np.concatenate([[0], np.cumsum(nchs)]).astype(np.uint64)
+    bases = np.concatenate([b for b, _ in parts])
+    off = np.concatenate([[0]] + [o[1:] + job_base[j] for j, (_, o) in enumerate(parts)]).astype(np.uint64)
+    # per-job offsets relative to the job (what tg_count_device takes)
+    roff = np.concatenate([o for _, o in parts]).astype(np.uint64)
+    roff_start = np.concatenate([[0], np.cumsum(nchs + 1)]).astype(np.uint64)
+    del parts
+    N = int(job_base[-1])
+    log(f"many samples: {n_samples} jobs, {N} bases, {int(job_chunk[-1])} chunks "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    d_r = eng.dev_upload(roff)
+    del bases
+    base_ptr, off_ptr = d_b.value, d_r.value
+
+    def run_a():
+        st = []
+        for j in range(n_samples):
+            st.append(eng.count_device(base_ptr + int(job_base[j]), int(sizes[j]),
+                                       off_ptr + 8 * int(roff_start[j]), int(nchs[j]), C4_K, C4_M, 16, 1))
+        eng.sync()
+        return st
+
+    def run_b():
+        st = eng.count_many_device(d_b, N, d_o, int(job_chunk[-1]), job_chunk, job_base, C4_K, C4_M, 1)
+        eng.sync()
+        return st
+
+    # warm both paths (buffer pools, graph capture)
+    big = int(np.argmax(sizes))
+    eng.count_device(base_ptr + int(job_base[big]), int(sizes[big]), off_ptr + 8 * int(roff_start[big]),
+                     int(nchs[big]), C4_K, C4_M, 16, 1)
+    run_b()
+    t0 = time.perf_counter()
+    st_b = run_b()
+    t_b = time.perf_counter() - t0
+    mb = eng.last_many.as_dict()
+    t0 = time.perf_counter()
+    st_a = run_a()
+    t_a = time.perf_counter() - t0
+    same = all(a.n_kmers == b["n_kmers"] and a.n_superkmers == b["n_superkmers"] and
+               a.n_unique == b["n_unique"] and a.n_out == b["n_out"] for a, b in zip(st_a, st_b))
+    # full key/count equality on a spread of jobs
+    checked = 0
+    for j in sorted(set([0, n_samples - 1, big] + list(range(0, n_samples, max(1, n_samples // 8))))):
+        eng.count_device(base_ptr + int(job_base[j]), int(sizes[j]), off_ptr + 8 * int(roff_start[j]),
+                         int(nchs[j]), C4_K, C4_M, 16, 1)
+        ak, ac = eng.result(C4_K)
+        bk, bc = eng.many_result(j, C4_K)
+        oa, ob = np.argsort(ak[:, 0], kind="stable"), np.argsort(bk[:, 0], kind="stable")
+        same = same and np.array_equal(ak[oa], bk[ob]) and np.array_equal(ac[oa], bc[ob])
+        checked += 1
+    eng.dev_free(d_b)
+    eng.dev_free(d_o)
+    eng.dev_free(d_r)
+    return {
+        "workload": f"many samples: first {n_samples} of the 2000-sample family (seed 4, log-normal genome "
+                    f"median 300 kbp sigma 1.2 clipped 10 kbp-8 Mbp, 150 bp reads at 10x, 0.5% subs), "
+                    f"k={C4_K} m={C4_M}",
+        "jobs": n_samples, "bases": N,
+        "run_a_per_job": {"s": round(t_a, 4), "jobs_per_s": round(n_samples / t_a, 2),
+                          "gbases_per_s": round(N / t_a / 1e9, 3)},
+        "run_b_batched": {"s": round(t_b, 4), "jobs_per_s": round(n_samples / t_b, 2),
+                          "gbases_per_s": round(N / t_b / 1e9, 3), "batches": mb["n_batches"],
+                          "graph_launches": mb["n_graph_launches"], "captures": mb["n_captures"]},
+        "speedup_b_over_a": round(t_a / t_b, 3),
+        "a_equals_b": bool(same), "jobs_fully_compared": checked,
+        "batch_config": eng.batch_config().as_dict(),
+    }
+
+
 def load_traffic(kernel):
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_short_reads_c2.json")
     if os.path.exists(p):
@@ -113,6 +209,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--samples", type=int, default=2000,
+                    help="jobs of the many-sample leg (0 skips it)")
+    ap.add_argument("--only", choices=["all", "single", "batched"], default="all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this engine runs on one GPU "
@@ -120,6 +219,12 @@ def main():
         sys.exit(2)
     import tuna_amd as T
 
+    threads = min(16, os.cpu_count() or 1)
+    if args.only == "batched":
+        eng = T.Engine(0)
+        print(json.dumps({"batched": bench_batched(T, eng, args.samples, threads)}))
+        eng.close()
+        return
     t0 = time.perf_counter()
     text = T.synth(SPEC)
     bases, off = T.parse_buffer(text)
@@ -183,8 +288,9 @@ def main():
         "kernels_ms": {k_: round(v[0], 4) for k_, v in kern.items()},
         "roofline": roof,
     }
+    if args.samples > 0 and args.only == "all":
+        out["batched"] = bench_batched(T, eng, args.samples, threads)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         try:
             out["cpu_baseline"] = cpu_baseline(bytes(text), N, threads)
         except Exception as e:  # reported, not fatal

@@ -221,6 +221,8 @@ int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint3
 int64_t tg_synth(const char* spec);
 int tg_synth_copy(uint8_t* dst, uint64_t cap);
 void tg_synth_free(void);
+/* Thread-safe form: the text goes to a malloc'd buffer (release with tg_free). */
+int64_t tg_synth_alloc(const char* spec, uint8_t** data);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ EXPORTS = [
     "tg_auto_partitions", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
     "tg_synth_free", "tg_get_batch_config", "tg_set_batch_config", "tg_count_many",
     "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
+    "tg_synth_alloc",
 ]
 
 
@@ -137,6 +138,8 @@ def lib():
     L.tg_synth.argtypes = [ctypes.c_char_p]
     L.tg_synth_copy.argtypes = [_u8p, ctypes.c_uint64]
     L.tg_synth_free.restype = None
+    L.tg_synth_alloc.restype = ctypes.c_int64
+    L.tg_synth_alloc.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p)]
     L.tg_get_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_set_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_count_many.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
@@ -213,6 +216,21 @@ def synth(spec):
     _check(L.tg_synth_copy(_p(buf, _u8p), n))
     L.tg_synth_free()
     return buf
+
+
+def synth_parsed(spec):
+    """Generate and parse one synthetic input -> (bases, off); thread-safe, so
+    many samples can be generated in parallel threads."""
+    L = lib()
+    ptr = _u8p()
+    n = L.tg_synth_alloc(spec.encode(), ctypes.byref(ptr))
+    if n < 0:
+        raise TunaGpuError(f"bad synth spec {spec!r}")
+    try:
+        text = np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n]
+        return parse_buffer(text)
+    finally:
+        L.tg_free(ptr)
 
 
 # --------------------------------------------------------------- options ----

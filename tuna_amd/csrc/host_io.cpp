@@ -598,18 +598,22 @@ int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases, u
 //   kind=repeat_fq   genome, seed, gc, cov, rl, sub, fam, flen, div, rep, tan
 //   kind=samples_fq  samples, seed, median, sigma, lo, hi, cov, rl, sub, index
 //                    (the index-th sample of a log-normal family)
-int64_t tg_synth(const char* spec_c)
+}  // extern "C"
+
+namespace {
+
+int64_t synth_into(const char* spec_c, std::string& out)
 {
     const std::string spec = spec_c ? spec_c : "";
     const std::string kind = spec_str(spec, "kind", "genome_fa");
     const uint64_t seed = (uint64_t)spec_num(spec, "seed", 1);
     const double gc = spec_num(spec, "gc", 0.5);
-    g_synth.clear();
+    out.clear();
     Rng r(seed);
     std::string g;
     if (kind == "genome_fa") {
         random_genome(r, (uint64_t)spec_num(spec, "len", 4600000), gc, g);
-        fasta("synthetic_seed" + std::to_string(seed), g, g_synth);
+        fasta("synthetic_seed" + std::to_string(seed), g, out);
     } else if (kind == "reads_fq" || kind == "repeat_fq") {
         const uint64_t G = (uint64_t)spec_num(spec, "genome", 5000000);
         if (kind == "reads_fq") random_genome(r, G, gc, g);
@@ -619,9 +623,9 @@ int64_t tg_synth(const char* spec_c)
         const uint32_t rl = (uint32_t)spec_num(spec, "rl", 150);
         uint64_t nreads = (uint64_t)spec_num(spec, "reads", 0);
         if (!nreads) nreads = (uint64_t)std::llround(spec_num(spec, "cov", 50) * (double)G / rl);
-        g_synth.reserve(nreads * (2ull * rl + 16));
+        out.reserve(nreads * (2ull * rl + 16));
         Rng rr(seed * 0x100000001b3ull + 7);
-        sample_reads_fq(rr, g, nreads, rl, spec_num(spec, "sub", 0.005), spec_num(spec, "nrate", 0.0), g_synth, 0);
+        sample_reads_fq(rr, g, nreads, rl, spec_num(spec, "sub", 0.005), spec_num(spec, "nrate", 0.0), out, 0);
     } else if (kind == "samples_fq") {
         const uint64_t idx = (uint64_t)spec_num(spec, "index", 0);
         Rng lr(seed);
@@ -634,11 +638,29 @@ int64_t tg_synth(const char* spec_c)
         random_genome(sr, (uint64_t)len, gc, g);
         const uint32_t rl = (uint32_t)spec_num(spec, "rl", 150);
         const uint64_t nreads = (uint64_t)std::llround(spec_num(spec, "cov", 10) * len / rl);
-        sample_reads_fq(sr, g, nreads, rl, spec_num(spec, "sub", 0.005), spec_num(spec, "nrate", 0.0), g_synth, 0);
+        sample_reads_fq(sr, g, nreads, rl, spec_num(spec, "sub", 0.005), spec_num(spec, "nrate", 0.0), out, 0);
     } else {
         return TG_EINVAL;
     }
-    return (int64_t)g_synth.size();
+    return (int64_t)out.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t tg_synth(const char* spec_c) { return synth_into(spec_c, g_synth); }
+
+int64_t tg_synth_alloc(const char* spec_c, uint8_t** data)
+{
+    if (!data) return TG_EINVAL;
+    std::string t;
+    const int64_t n = synth_into(spec_c, t);
+    if (n < 0) return n;
+    *data = static_cast<uint8_t*>(malloc(t.size() ? t.size() : 1));
+    if (!*data) return TG_ENOMEM;
+    memcpy(*data, t.data(), t.size());
+    return n;
 }
 
 int tg_synth_copy(uint8_t* dst, uint64_t cap)
