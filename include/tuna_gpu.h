@@ -37,6 +37,7 @@ extern "C" {
 #define TG_ENOMEM (-3)   /* device memory admission failed */
 #define TG_ECAP (-4)     /* caller buffer too small */
 #define TG_EINPUT (-5)   /* input file / parse error */
+#define TG_ETIMEOUT (-6) /* request not finished within the wait */
 
 typedef struct tg_ctx tg_ctx;
 
@@ -180,6 +181,61 @@ uint64_t tg_many_result_size(tg_ctx* ctx, uint32_t job);
 int tg_many_result_copy(tg_ctx* ctx, uint32_t job, uint64_t* keys, uint32_t* counts, uint64_t cap);
 int tg_many_result_device(tg_ctx* ctx, uint32_t job, const uint64_t** d_keys,
                           const uint32_t** d_counts, uint64_t* n);
+
+/* ---- serving: a request loop with continuous batching ----------------- */
+/* The reference has no serving mode (one process run per count).  A server
+ * keeps `lanes` engine contexts resident (stream, batch buffers, captured
+ * graph, device input area, worker thread).  A free lane takes every queued
+ * request that fits its share of the HBM budget, in arrival order, counts them
+ * together (one job each: results identical to counting a request alone) and
+ * copies each result to pinned host memory.  Requests arriving while lanes are
+ * busy wait for the next free lane; with two or more lanes, batches overlap on
+ * the device.  All requests of a server share k, m, ci, cx. */
+typedef struct tg_server tg_server;
+
+typedef struct {
+    uint32_t lanes;           /* resident contexts / worker threads (default 2) */
+    uint32_t max_batch_jobs;  /* requests per lane batch (default 64) */
+    uint64_t max_batch_bases; /* bases per lane batch (default 2^27) */
+    uint64_t hbm_budget;      /* device bytes for all lanes (0 = half the free HBM) */
+} tg_server_config;
+
+typedef struct {
+    double submit_to_start_s; /* queueing */
+    double submit_to_done_s;  /* latency: result in host memory */
+    uint32_t lane, batch_jobs;
+} tg_request_times;
+
+typedef struct {
+    uint32_t lanes;
+    uint64_t lane_bases;      /* admission limit of one lane batch */
+    uint64_t hbm_budget, hbm_free;
+    uint64_t queued, completed, failed, rejected, batches;
+    uint64_t pinned_pool_bytes;
+} tg_server_info;
+
+int tg_server_create(int device, const tg_params* p, const tg_server_config* cfg, tg_server** out);
+/* Finishes queued requests, then stops the lanes. */
+void tg_server_destroy(tg_server* s);
+/* Queue a request: bases/off as for tg_count_host (off[0] = 0, off[n_chunks] =
+ * n_bases).  The caller keeps both arrays alive until the request is done (pinned
+ * memory from tg_host_alloc makes the upload faster).  TG_ENOMEM when the
+ * request exceeds a lane's admission limit. */
+int tg_server_submit(tg_server* s, const uint8_t* bases, uint64_t n_bases, const uint64_t* off,
+                     uint64_t n_chunks, uint64_t* id);
+/* 1 done, 0 queued or running, < 0 the request's error. */
+int tg_server_poll(tg_server* s, uint64_t id);
+/* Block until done (timeout_s < 0: no limit); TG_ETIMEOUT otherwise. */
+int tg_server_wait(tg_server* s, uint64_t id, double timeout_s);
+/* Result of a finished request, lent until tg_server_release: n (key, count)
+ * pairs (keys n*ceil(k/32) words), stats and timings (each optional). */
+int tg_server_result(tg_server* s, uint64_t id, const uint64_t** keys, const uint32_t** counts,
+                     uint64_t* n, tg_job_stats* st, tg_request_times* t);
+int tg_server_release(tg_server* s, uint64_t id);
+int tg_server_stats(tg_server* s, tg_server_info* st);
+/* Pinned host memory for request inputs. */
+int tg_host_alloc(uint64_t bytes, void** p);
+int tg_host_free(void* p);
 
 /* ---- host input path (A1) --------------------------------------------- */
 /* Parse FASTA/FASTQ bytes (format by first byte, '@' = FASTQ) into ACGT

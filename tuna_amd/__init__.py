@@ -32,7 +32,9 @@ EXPORTS = [
     "tg_auto_partitions", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
     "tg_synth_free", "tg_get_batch_config", "tg_set_batch_config", "tg_count_many",
     "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
-    "tg_synth_alloc",
+    "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
+    "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
+    "tg_host_alloc", "tg_host_free",
 ]
 
 
@@ -80,6 +82,33 @@ class TgBatchConfig(ctypes.Structure):
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TgServerConfig(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_uint32), ("max_batch_jobs", ctypes.c_uint32),
+                ("max_batch_bases", ctypes.c_uint64), ("hbm_budget", ctypes.c_uint64)]
+
+
+class TgRequestTimes(ctypes.Structure):
+    _fields_ = [("submit_to_start_s", ctypes.c_double), ("submit_to_done_s", ctypes.c_double),
+                ("lane", ctypes.c_uint32), ("batch_jobs", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class TgServerInfo(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_uint32), ("lane_bases", ctypes.c_uint64),
+                ("hbm_budget", ctypes.c_uint64), ("hbm_free", ctypes.c_uint64),
+                ("queued", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("pinned_pool_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+TG_ETIMEOUT = -6
 
 
 class TunaGpuError(RuntimeError):
@@ -140,6 +169,19 @@ def lib():
     L.tg_synth_free.restype = None
     L.tg_synth_alloc.restype = ctypes.c_int64
     L.tg_synth_alloc.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p)]
+    L.tg_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(TgParams),
+                                   ctypes.POINTER(TgServerConfig), ctypes.POINTER(_vp)]
+    L.tg_server_destroy.argtypes = [_vp]
+    L.tg_server_destroy.restype = None
+    L.tg_server_submit.argtypes = [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _u64p]
+    L.tg_server_poll.argtypes = [_vp, ctypes.c_uint64]
+    L.tg_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_double]
+    L.tg_server_result.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(_u64p), ctypes.POINTER(_u32p),
+                                   _u64p, ctypes.POINTER(TgJobStats), ctypes.POINTER(TgRequestTimes)]
+    L.tg_server_release.argtypes = [_vp, ctypes.c_uint64]
+    L.tg_server_stats.argtypes = [_vp, ctypes.POINTER(TgServerInfo)]
+    L.tg_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(_vp)]
+    L.tg_host_free.argtypes = [_vp]
     L.tg_get_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_set_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_count_many.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
@@ -231,6 +273,99 @@ def synth_parsed(spec):
         return parse_buffer(text)
     finally:
         L.tg_free(ptr)
+
+
+class _Pinned:
+    def __init__(self, nbytes):
+        self.p = _vp()
+        _check(lib().tg_host_alloc(max(nbytes, 1), ctypes.byref(self.p)))
+
+    def __del__(self):
+        if self.p:
+            lib().tg_host_free(self.p)
+            self.p = None
+
+
+def pinned_copy(arr):
+    """A copy of arr in pinned host memory (faster uploads); the memory lives as
+    long as the returned array."""
+    arr = np.ascontiguousarray(arr)
+    owner = _Pinned(arr.nbytes)
+    buf = (ctypes.c_uint8 * max(arr.nbytes, 1)).from_address(owner.p.value)
+    buf.owner = owner  # the array keeps buf alive, buf keeps the pinned block
+    out = np.frombuffer(buf, dtype=np.uint8, count=arr.nbytes).view(arr.dtype).reshape(arr.shape)
+    out[...] = arr
+    return out
+
+
+class Server:
+    """Request loop with continuous batching (tg_server_* in include/tuna_gpu.h)."""
+
+    def __init__(self, k, m=21, ci=1, cx=2**64 - 1, lanes=2, max_batch_jobs=64, max_batch_bases=0,
+                 hbm_budget=0, device=0):
+        self.k = k
+        self.nw = 2 if k > 32 else 1
+        p = _params(k, m, 16, ci, cx)
+        cfg = TgServerConfig(lanes, max_batch_jobs, max_batch_bases, hbm_budget)
+        self._h = _vp()
+        _check(lib().tg_server_create(device, ctypes.byref(p), ctypes.byref(cfg), ctypes.byref(self._h)))
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            lib().tg_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, bases, off):
+        bases = bases if isinstance(bases, np.ndarray) and bases.dtype == np.uint8 else _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        rid = ctypes.c_uint64(0)
+        _check(lib().tg_server_submit(self._h, bases.ctypes.data_as(_vp), len(bases),
+                                      off.ctypes.data_as(_vp), len(off) - 1, ctypes.byref(rid)))
+        self._keep[rid.value] = (bases, off)
+        return rid.value
+
+    def poll(self, rid):
+        rc = lib().tg_server_poll(self._h, rid)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def wait(self, rid, timeout=-1.0):
+        rc = lib().tg_server_wait(self._h, rid, timeout)
+        if rc == TG_ETIMEOUT:
+            return False
+        _check(rc)
+        return True
+
+    def result(self, rid, copy=True):
+        kp, cp = _u64p(), _u32p()
+        n = ctypes.c_uint64(0)
+        st, t = TgJobStats(), TgRequestTimes()
+        _check(lib().tg_server_result(self._h, rid, ctypes.byref(kp), ctypes.byref(cp), ctypes.byref(n),
+                                      ctypes.byref(st), ctypes.byref(t)))
+        n = n.value
+        if not copy:
+            return None, None, st.as_dict(), t.as_dict()
+        keys = np.ctypeslib.as_array(kp, shape=(max(n, 1) * self.nw,))[: n * self.nw].copy()
+        cnt = np.ctypeslib.as_array(cp, shape=(max(n, 1),))[:n].copy()
+        return keys.reshape(n, self.nw), cnt, st.as_dict(), t.as_dict()
+
+    def release(self, rid):
+        _check(lib().tg_server_release(self._h, rid))
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        i = TgServerInfo()
+        _check(lib().tg_server_stats(self._h, ctypes.byref(i)))
+        return i.as_dict()
 
 
 # --------------------------------------------------------------- options ----

@@ -14,6 +14,7 @@
 #include "tg_kernels.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace tg {
 
@@ -685,14 +686,14 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
 // launch sequence is being captured.
 void set_kernel_attributes()
 {
-    static bool done = false;
-    if (done) return;
-    done = true;
+    static std::once_flag once;
+    std::call_once(once, [] {
     const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
     const int cur = (int)((1u << 14) * sizeof(uint32_t));
     (void)hipFuncSetAttribute((const void*)k_route_hist, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
     (void)hipFuncSetAttribute((const void*)k_route_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
     (void)hipFuncSetAttribute((const void*)k_route_fine, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+    });
 }
 
 void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles)
