@@ -195,6 +195,77 @@ def bench_batched(T, eng, n_samples, threads):
     }
 
 
+# ------------------------------------------------------------ serving ----
+C5_SPEC = ("kind=samples_fq seed=5 index={i} median=1000000 sigma=1.0 lo=10000 hi=20000000 "
+           "cov=15 rl=150 sub=0")
+C5_K, C5_M, C5_CI = 63, 21, 2
+
+
+def bench_serving(T, n_req, lambdas, threads, lanes=2):
+    """Poisson arrivals (seed 5) of counting requests at each rate in `lambdas`
+    (jobs/s); latency = submit to result in host memory, per request."""
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        reqs = list(ex.map(T.synth_parsed, [C5_SPEC.format(i=i) for i in range(n_req)]))
+    reqs = [(T.pinned_copy(b), T.pinned_copy(o)) for b, o in reqs]
+    sizes = np.array([len(b) for b, _ in reqs], dtype=np.float64)
+    log(f"serving: {n_req} requests, {int(sizes.sum())} bases (mean {sizes.mean() / 1e6:.1f} M, "
+        f"max {sizes.max() / 1e6:.1f} M), generated in {time.perf_counter() - t0:.1f} s")
+    srv = T.Server(C5_K, C5_M, C5_CI, lanes=lanes, max_batch_jobs=64, max_batch_bases=1 << 29)
+
+    def run(arrivals):
+        ids = []
+        t_start = time.perf_counter()
+        for i, a in enumerate(arrivals):
+            dt = t_start + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append((srv.submit(*reqs[i % n_req]), time.perf_counter() - t_start))
+        lat, end, bj = [], 0.0, []
+        for rid, ts in ids:
+            srv.wait(rid, 300.0)
+            _, _, st, tm = srv.result(rid, copy=False)
+            lat.append(tm["submit_to_done_s"])
+            bj.append(tm["batch_jobs"])
+            end = max(end, ts + tm["submit_to_done_s"])
+            srv.release(rid)
+        lat = np.array(lat) * 1e3
+        n = len(arrivals)
+        tot = float(sum(sizes[i % n_req] for i in range(n)))
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "max_ms": round(float(lat.max()), 3), "mean_ms": round(float(lat.mean()), 3),
+                "jobs_per_s": round(n / end, 2), "gbases_per_s": round(tot / end / 1e9, 3),
+                "mean_batch_jobs": round(float(np.mean(bj)), 2), "requests": n}
+
+    run(np.zeros(min(8, n_req)))  # warm lanes (buffers, graph capture, pinned pool)
+    sweep = []
+    for lam in lambdas:
+        rng = np.random.default_rng(5)
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        arr -= arr[0]
+        r = run(arr)
+        r["lambda"] = lam
+        sweep.append(r)
+        log(f"  lambda {lam}/s: p50 {r['p50_ms']} ms p99 {r['p99_ms']} ms, {r['jobs_per_s']} jobs/s, "
+            f"batch {r['mean_batch_jobs']}")
+    sat = run(np.zeros(n_req))
+    info = srv.stats()
+    srv.close()
+    head = next((r for r in sweep if r["lambda"] == 100), sweep[-1] if sweep else sat)
+    return {
+        "workload": f"serving: {n_req} requests of the seed-5 family (own random genome each, log-normal "
+                    f"length median 1 Mbp sigma 1.0 max 20 Mbp, 150 bp reads at 15x, no errors), "
+                    f"k={C5_K} m={C5_M} ci={C5_CI}, Poisson arrivals seed 5, {lanes} lanes",
+        "headline": {"lambda": head.get("lambda"), "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
+                     "jobs_per_s": head["jobs_per_s"]},
+        "sweep": sweep, "saturation": sat,
+        "server": {k_: info[k_] for k_ in ("lanes", "lane_bases", "hbm_budget", "batches",
+                                            "pinned_pool_bytes")},
+    }
+
+
 def load_traffic(kernel):
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_short_reads_c2.json")
     if os.path.exists(p):
@@ -211,7 +282,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--samples", type=int, default=2000,
                     help="jobs of the many-sample leg (0 skips it)")
-    ap.add_argument("--only", choices=["all", "single", "batched"], default="all")
+    ap.add_argument("--requests", type=int, default=128,
+                    help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--lambdas", default="20,50,100,200", help="arrival rates (jobs/s) of the serving leg")
+    ap.add_argument("--only", choices=["all", "single", "batched", "serving"], default="all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this engine runs on one GPU "
@@ -220,10 +294,14 @@ def main():
     import tuna_amd as T
 
     threads = min(16, os.cpu_count() or 1)
+    lambdas = [float(x) for x in args.lambdas.split(",") if x]
     if args.only == "batched":
         eng = T.Engine(0)
         print(json.dumps({"batched": bench_batched(T, eng, args.samples, threads)}))
         eng.close()
+        return
+    if args.only == "serving":
+        print(json.dumps({"serving": bench_serving(T, args.requests, lambdas, threads)}))
         return
     t0 = time.perf_counter()
     text = T.synth(SPEC)
@@ -290,6 +368,8 @@ def main():
     }
     if args.samples > 0 and args.only == "all":
         out["batched"] = bench_batched(T, eng, args.samples, threads)
+    if args.requests > 0 and args.only == "all":
+        out["serving"] = bench_serving(T, args.requests, lambdas, threads)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(bytes(text), N, threads)
