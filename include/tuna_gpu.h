@@ -164,6 +164,15 @@ typedef struct {
 int tg_get_batch_config(tg_ctx* ctx, tg_batch_config* cfg);
 int tg_set_batch_config(tg_ctx* ctx, const tg_batch_config* cfg);
 
+/* Autotuner: times candidate settings (bucket size, batch size, load factor,
+ * routing tiles) on the given device-resident jobs by coordinate search, checks
+ * every candidate reproduces the same per-job totals, installs the fastest in
+ * ctx and returns it (best_ms: wall time of one call with it). */
+int tg_autotune_many(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                     const uint64_t* d_off, uint64_t n_chunks, const uint64_t* job_chunk,
+                     const uint64_t* job_base, uint32_t n_jobs, tg_batch_config* best,
+                     double* best_ms);
+
 /* Host inputs: bases/off as for tg_count_host plus the job table (n_jobs+1
  * entries).  st (n_jobs entries) and ms are optional. */
 int tg_count_many(tg_ctx* ctx, const tg_params* p, const uint8_t* bases, uint64_t n_bases,

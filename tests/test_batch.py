@@ -203,3 +203,22 @@ def test_batch_config_validation(eng):
         eng.set_batch_config(load_factor=0.0)
     c = eng.batch_config()
     assert c.kmers_per_bucket >= 256 and c.use_graph in (0, 1)
+
+
+def test_autotune_installs_a_config_with_identical_results(eng):
+    parts = job_inputs(12)
+    bases, off, jc = concat(parts)
+    jb = off[jc.astype(np.int64)].copy()
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    old = eng.batch_config()
+    try:
+        say("autotune")
+        best, ms = eng.autotune_many(d_b, len(bases), d_o, len(off) - 1, jc, jb, 21, 15)
+        assert ms > 0 and best["kmers_per_bucket"] in (4096, 8192, 16384, 32768, 65536)
+        assert eng.batch_config().as_dict() == best
+        check_jobs(eng, parts, 21, 15, label="tuned")
+    finally:
+        eng.dev_free(d_b)
+        eng.dev_free(d_o)
+        eng.set_batch_config(**{k_: getattr(old, k_) for k_, _ in old._fields_})

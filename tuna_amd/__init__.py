@@ -34,7 +34,7 @@ EXPORTS = [
     "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
-    "tg_host_alloc", "tg_host_free",
+    "tg_host_alloc", "tg_host_free", "tg_autotune_many",
 ]
 
 
@@ -182,6 +182,9 @@ def lib():
     L.tg_server_stats.argtypes = [_vp, ctypes.POINTER(TgServerInfo)]
     L.tg_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(_vp)]
     L.tg_host_free.argtypes = [_vp]
+    L.tg_autotune_many.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
+                                   ctypes.c_uint64, _u64p, _u64p, ctypes.c_uint32,
+                                   ctypes.POINTER(TgBatchConfig), ctypes.POINTER(ctypes.c_double)]
     L.tg_get_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_set_batch_config.argtypes = [_vp, ctypes.POINTER(TgBatchConfig)]
     L.tg_count_many.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
@@ -506,6 +509,19 @@ class Engine:
                                           ctypes.byref(ms)))
         self.last_many = ms
         return [st[j].as_dict() for j in range(nj)] if stats else None
+
+    def autotune_many(self, d_bases, n_bases, d_off, n_chunks, job_chunk, job_base, k, m=21, ci=1,
+                      cx=2**64 - 1):
+        """Pick and install the fastest batch configuration for these jobs."""
+        jc = np.ascontiguousarray(job_chunk, dtype=np.uint64)
+        jb = np.ascontiguousarray(job_base, dtype=np.uint64)
+        p = _params(k, m, 16, ci, cx)
+        best = TgBatchConfig()
+        ms = ctypes.c_double(0)
+        _check(lib().tg_autotune_many(self._h, ctypes.byref(p), d_bases, n_bases, d_off, n_chunks,
+                                      _p(jc, _u64p), _p(jb, _u64p), len(jc) - 1, ctypes.byref(best),
+                                      ctypes.byref(ms)))
+        return best.as_dict(), ms.value
 
     def many_result(self, job, k):
         nw = 2 if k > 32 else 1

@@ -677,6 +677,85 @@ int tg_count_many(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t 
     return rc;
 }
 
+// Coordinate search over the batched path's tunables on the caller's jobs:
+// every candidate runs twice (the first run sizes buffers and captures the
+// graph), the second is timed; every run must reproduce the per-job totals of
+// the starting configuration.  The fastest configuration is installed.
+int tg_autotune_many(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                     const uint64_t* d_off, uint64_t n_chunks, const uint64_t* job_chunk,
+                     const uint64_t* job_base, uint32_t n_jobs, tg_batch_config* best_out,
+                     double* best_ms)
+{
+    if (!c || !job_chunk || !job_base) return fail(TG_EINVAL, "null argument");
+    const JobTab jt{job_chunk, job_base, n_jobs};
+    BatchState* s = state(c);
+    tg_batch_config best = s->cfg;
+    best.rho = 0.0f;
+    std::vector<tg_job_stats> ref(n_jobs), st(n_jobs);
+    auto run = [&](const tg_batch_config& cfg, std::vector<tg_job_stats>& out, double* ms) -> int {
+        int rc = tg_set_batch_config(c, &cfg);
+        if (rc) return rc;
+        for (int rep = 0; rep < 2; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = count_many_device(c, p, d_bases, n_bases, d_off, n_chunks, jt, out.data())))
+                return rc;
+            *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return TG_OK;
+    };
+    double best_t = 0;
+    int rc = run(best, ref, &best_t);
+    if (rc) return rc;
+    const uint64_t total = n_jobs ? job_base[n_jobs] - job_base[0] : 0;
+    const uint32_t kpb[] = {4096, 8192, 16384, 32768, 65536};
+    const float lf[] = {0.5f, 0.6f, 0.75f, 0.9f};
+    const uint32_t tiles[] = {256, 512, 1024};
+    const uint64_t bb[] = {1ull << 26, 1ull << 27, 1ull << 28, 1ull << 29, 1ull << 30};
+    auto same = [&](const std::vector<tg_job_stats>& a) {
+        for (uint32_t j = 0; j < n_jobs; ++j)
+            if (a[j].n_kmers != ref[j].n_kmers || a[j].n_superkmers != ref[j].n_superkmers ||
+                a[j].n_unique != ref[j].n_unique || a[j].n_out != ref[j].n_out)
+                return false;
+        return true;
+    };
+    auto consider = [&](tg_batch_config cand) -> int {
+        double t = 0;
+        int r = run(cand, st, &t);
+        if (r) return r;
+        if (!same(st)) return fail(TG_EDEVICE, "autotune: a candidate configuration changed the results");
+        if (t < best_t) {
+            best_t = t;
+            best = cand;
+        }
+        return TG_OK;
+    };
+    for (uint32_t v : kpb) {
+        tg_batch_config cand = best;
+        cand.kmers_per_bucket = v;
+        if (v != best.kmers_per_bucket && (rc = consider(cand))) return rc;
+    }
+    for (uint64_t v : bb) {
+        tg_batch_config cand = best;
+        cand.batch_bases = v;
+        if (v > 2 * std::max<uint64_t>(total, 1u << 16)) continue;  // beyond one batch: no change
+        if (v != best.batch_bases && (rc = consider(cand))) return rc;
+    }
+    for (float v : lf) {
+        tg_batch_config cand = best;
+        cand.load_factor = v;
+        if (v != best.load_factor && (rc = consider(cand))) return rc;
+    }
+    for (uint32_t v : tiles) {
+        tg_batch_config cand = best;
+        cand.max_tiles = v;
+        if (v != best.max_tiles && (rc = consider(cand))) return rc;
+    }
+    if ((rc = tg_set_batch_config(c, &best))) return rc;
+    if (best_out) *best_out = best;
+    if (best_ms) *best_ms = best_t;
+    return TG_OK;
+}
+
 uint64_t tg_many_result_size(tg_ctx* c, uint32_t job)
 {
     if (!c || !c->batch || job >= c->batch->res_n.size()) return 0;
