@@ -112,7 +112,7 @@ def gen_samples(T, n, threads):
         return list(ex.map(T.synth_parsed, [C4_SPEC.format(i=i) for i in range(n)]))
 
 
-def bench_batched(T, eng, n_samples, threads):
+def bench_batched(T, eng, n_samples, threads, use_graph=True):
     """Run A: the plain per-job sequence (tg_count_device) once per sample.
     Run B: all samples through tg_count_many_device (batched launch sequences,
     one captured hipGraph replayed per batch).  Inputs are resident in HBM
@@ -138,6 +138,7 @@ def bench_batched(T, eng, n_samples, threads):
     d_r = eng.dev_upload(roff)
     del bases
     base_ptr, off_ptr = d_b.value, d_r.value
+    eng.set_batch_config(use_graph=1 if use_graph else 0)
 
     def run_a():
         st = []
@@ -290,10 +291,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--samples", type=int, default=2000,
                     help="jobs of the many-sample leg (0 skips it)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched leg: plain launches instead of graph replay (same kernels)")
     ap.add_argument("--requests", type=int, default=128,
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--lambdas", default="20,50,100,200", help="arrival rates (jobs/s) of the serving leg")
-    ap.add_argument("--only", choices=["all", "single", "batched", "serving"], default="all")
+    ap.add_argument("--only", choices=["all", "single", "batched", "serving", "copy"], default="all",
+                    help="run one leg; 'copy' runs a plain 256 MiB device copy kernel 5 times "
+                         "(the reference size for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this engine runs on one GPU "
@@ -302,10 +307,20 @@ def main():
     import tuna_amd as T
 
     threads = min(16, os.cpu_count() or 1)
+    if args.only == "copy":
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": src.numel() * 4,
+                          "launches": 5}))
+        return
     lambdas = [float(x) for x in args.lambdas.split(",") if x]
     if args.only == "batched":
         eng = T.Engine(0)
-        print(json.dumps({"batched": bench_batched(T, eng, args.samples, threads)}))
+        print(json.dumps({"batched": bench_batched(T, eng, args.samples, threads, not args.no_graph)}))
         eng.close()
         return
     if args.only == "serving":
@@ -375,7 +390,7 @@ def main():
         "roofline": roof,
     }
     if args.samples > 0 and args.only == "all":
-        out["batched"] = bench_batched(T, eng, args.samples, threads)
+        out["batched"] = bench_batched(T, eng, args.samples, threads, not args.no_graph)
     if args.requests > 0 and args.only == "all":
         out["serving"] = bench_serving(T, args.requests, lambdas, threads)
     if not args.no_cpu:

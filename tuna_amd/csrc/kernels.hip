@@ -844,58 +844,73 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n)
 // ========================================================== counting ====
 constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 
-// Global spill table, one 64-bit word per key: claim by CAS on the key.
-__device__ __noinline__ bool spill_insert1(const CountArgs& a, uint64_t x)
+// Spill-table view passed by value to the out-of-line insert paths (a
+// reference to the kernel's argument struct would force a per-thread copy of
+// it into scratch memory).
+struct SpillRef {
+    uint64_t* keys;
+    uint32_t* cnt;
+    uint64_t mask;
+    uint32_t* err;
+};
+
+__device__ __forceinline__ SpillRef spill_ref(const CountArgs& a)
 {
-    uint64_t slot = mix64(x ^ 0x9e3779b97f4a7c15ull) & a.sp_mask;
-    for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16); ++p) {
-        unsigned long long* kp = (unsigned long long*)&a.sp_keys[slot];
+    return SpillRef{a.sp_keys, a.sp_cnt, a.sp_mask, a.sp_err};
+}
+
+// Global spill table, one 64-bit word per key: claim by CAS on the key.
+__device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x)
+{
+    uint64_t slot = mix64(x ^ 0x9e3779b97f4a7c15ull) & t.mask;
+    for (uint32_t p = 0; p <= t.mask && p < (1u << 16); ++p) {
+        unsigned long long* kp = (unsigned long long*)&t.keys[slot];
         const uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == x) {
-            atomicAdd(&a.sp_cnt[slot], 1u);
+            atomicAdd(&t.cnt[slot], 1u);
             return true;
         }
         if (cur == EMPTY64) {
             const uint64_t old = atomicCAS(kp, (unsigned long long)EMPTY64, (unsigned long long)x);
             if (old == EMPTY64 || old == x) {
-                atomicAdd(&a.sp_cnt[slot], 1u);
+                atomicAdd(&t.cnt[slot], 1u);
                 return true;
             }
         }
-        slot = (slot + 1) & a.sp_mask;
+        slot = (slot + 1) & t.mask;
     }
-    atomicOr(a.sp_err, 1u);
+    atomicOr(t.err, 1u);
     return false;
 }
 
 // Global spill table, two words per key: the count word is the slot state
 // (0 empty, LOCKED32 being written, otherwise published count).
-__device__ __noinline__ bool spill_insert2(const CountArgs& a, uint64_t hi, uint64_t lo)
+__device__ __noinline__ bool spill_insert2(SpillRef t, uint64_t hi, uint64_t lo)
 {
-    uint64_t slot = mix64(lo ^ mix64(hi ^ 0x9e3779b97f4a7c15ull)) & a.sp_mask;
-    for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16);) {
-        uint32_t* cp = &a.sp_cnt[slot];
+    uint64_t slot = mix64(lo ^ mix64(hi ^ 0x9e3779b97f4a7c15ull)) & t.mask;
+    for (uint32_t p = 0; p <= t.mask && p < (1u << 16);) {
+        uint32_t* cp = &t.cnt[slot];
         const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (c == 0u) {
             if (atomicCAS(cp, 0u, LOCKED32) == 0u) {
-                __hip_atomic_store(&a.sp_keys[2 * slot], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&a.sp_keys[2 * slot + 1], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&t.keys[2 * slot], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&t.keys[2 * slot + 1], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 return true;
             }
             continue;
         }
         if (c == LOCKED32) continue;
-        const uint64_t kh = __hip_atomic_load(&a.sp_keys[2 * slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t kl = __hip_atomic_load(&a.sp_keys[2 * slot + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t kh = __hip_atomic_load(&t.keys[2 * slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t kl = __hip_atomic_load(&t.keys[2 * slot + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (kh == hi && kl == lo) {
             atomicAdd(cp, 1u);
             return true;
         }
-        slot = (slot + 1) & a.sp_mask;
+        slot = (slot + 1) & t.mask;
         ++p;
     }
-    atomicOr(a.sp_err, 1u);
+    atomicOr(t.err, 1u);
     return false;
 }
 
@@ -906,20 +921,20 @@ struct SpKey {
 };
 
 template <int SW>
-__device__ __noinline__ bool spill_insert_w(const CountArgs& a, SpKey key)
+__device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
 {
     uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
     for (int i = 0; i < SW; ++i) h = mix64(h ^ key.w[i]);
-    uint64_t slot = h & a.sp_mask;
-    for (uint32_t p = 0; p <= a.sp_mask && p < (1u << 16);) {
-        uint32_t* cp = &a.sp_cnt[slot];
+    uint64_t slot = h & t.mask;
+    for (uint32_t p = 0; p <= t.mask && p < (1u << 16);) {
+        uint32_t* cp = &t.cnt[slot];
         const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (c == 0u) {
             if (atomicCAS(cp, 0u, LOCKED32) == 0u) {
 #pragma unroll
                 for (int i = 0; i < SW; ++i)
-                    __hip_atomic_store(&a.sp_keys[SW * slot + i], key.w[i], __ATOMIC_RELAXED,
+                    __hip_atomic_store(&t.keys[SW * slot + i], key.w[i], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 return true;
@@ -930,16 +945,16 @@ __device__ __noinline__ bool spill_insert_w(const CountArgs& a, SpKey key)
         bool eq = true;
 #pragma unroll
         for (int i = 0; i < SW; ++i)
-            eq = eq && __hip_atomic_load(&a.sp_keys[SW * slot + i], __ATOMIC_RELAXED,
+            eq = eq && __hip_atomic_load(&t.keys[SW * slot + i], __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT) == key.w[i];
         if (eq) {
             atomicAdd(cp, 1u);
             return true;
         }
-        slot = (slot + 1) & a.sp_mask;
+        slot = (slot + 1) & t.mask;
         ++p;
     }
-    atomicOr(a.sp_err, 1u);
+    atomicOr(t.err, 1u);
     return false;
 }
 
@@ -1248,16 +1263,16 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                             const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
                             ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
                             if (!ok) {
-                                if (BATCH) spill_insert_w<2>(a, SpKey{{(uint64_t)tag, x, 0ull}});
-                                else spill_insert1(a, x);
+                                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
+                                else spill_insert1(spill_ref(a), x);
                             }
                         } else {
                             const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
                             const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
                             ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
                             if (!ok) {
-                                if (BATCH) spill_insert_w<3>(a, SpKey{{xh, xl, (uint64_t)tag}});
-                                else spill_insert2(a, xh, xl);
+                                if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
+                                else spill_insert2(spill_ref(a), xh, xl);
                             }
                         }
                         spilled += ok ? 0u : 1u;
