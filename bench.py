@@ -276,11 +276,18 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
 
 
 def load_traffic(kernel):
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_short_reads_c2.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f)
-    return None
+    """HBM bytes per launch of `kernel` on this workload, from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel)."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_short_reads.md")
+    if not os.path.exists(p):
+        return None
+    out = {}
+    with open(p) as f:
+        for line in f:
+            for key in ("hbm_bytes_per_launch", "fetch_bytes_per_launch", "write_bytes_per_launch"):
+                if line.startswith(key + ":"):
+                    out[key] = int(line.split(":", 1)[1].strip())
+    return out or None
 
 
 def main():
