@@ -169,11 +169,22 @@ def bench_batched(T, eng, n_samples, threads, use_graph=True):
     st_b = run_b()
     t_b = time.perf_counter() - t0
     mb = eng.last_many.as_dict()
+    # run A: the plain per-job launch sequence (host-driven between its steps)
+    eng.set_plain_sequence(True)
+    eng.count_device(base_ptr + int(job_base[big]), int(sizes[big]), off_ptr + 8 * int(roff_start[big]),
+                     int(nchs[big]), C4_K, C4_M, 16, 1)
     t0 = time.perf_counter()
     st_a = run_a()
     t_a = time.perf_counter() - t0
+    eng.set_plain_sequence(False)
+    # for reference: the batched engine called once per job (graph per job)
+    run_a()
+    t0 = time.perf_counter()
+    st_g = run_a()
+    t_g = time.perf_counter() - t0
     same = all(a.n_kmers == b["n_kmers"] and a.n_superkmers == b["n_superkmers"] and
                a.n_unique == b["n_unique"] and a.n_out == b["n_out"] for a, b in zip(st_a, st_b))
+    same = same and all(a.n_out == g.n_out and a.n_unique == g.n_unique for a, g in zip(st_a, st_g))
     # full key/count equality on a spread of jobs
     checked = 0
     for j in sorted(set([0, n_samples - 1, big] + list(range(0, n_samples, max(1, n_samples // 8))))):
@@ -193,7 +204,11 @@ def bench_batched(T, eng, n_samples, threads, use_graph=True):
                     f"k={C4_K} m={C4_M}",
         "jobs": n_samples, "bases": N,
         "run_a_per_job": {"s": round(t_a, 4), "jobs_per_s": round(n_samples / t_a, 2),
-                          "gbases_per_s": round(N / t_a / 1e9, 3)},
+                          "gbases_per_s": round(N / t_a / 1e9, 3),
+                          "sequence": "plain launch sequence per job (tg_set_plain_sequence)"},
+        "graph_per_job": {"s": round(t_g, 4), "jobs_per_s": round(n_samples / t_g, 2),
+                          "gbases_per_s": round(N / t_g / 1e9, 3),
+                          "sequence": "batched engine called once per job (one graph launch each)"},
         "run_b_batched": {"s": round(t_b, 4), "jobs_per_s": round(n_samples / t_b, 2),
                           "gbases_per_s": round(N / t_b / 1e9, 3), "batches": mb["n_batches"],
                           "graph_launches": mb["n_graph_launches"], "captures": mb["n_captures"]},
@@ -346,18 +361,23 @@ def main():
     for _ in range(args.warmup):
         eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
     eng.sync()
-    eng.reset_kernel_times()
-    eng.set_profiling(True)
-    eng.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
     eng.sync()
     el = time.perf_counter() - t0
-    eng.set_profiling(False)
-    kt = eng.kernel_times()
     ms_step = el * 1e3 / args.steps
     gbases = N * args.steps / el / 1e9
+    # per-kernel times: the same job with HIP events around every kernel on
+    # the context stream (plain launches of the same kernels)
+    eng.reset_kernel_times()
+    eng.set_profiling(True)
+    prof_steps = 3
+    for _ in range(prof_steps):
+        eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
+    eng.sync()
+    eng.set_profiling(False)
+    kt = eng.kernel_times()
 
     # exact counters of the job for the algorithmic-bytes figures
     sk = eng.superkmers(bases, off, K, M, NPART)
@@ -394,6 +414,7 @@ def main():
                   "unique_out": st.n_out, "subparts": st.n_subparts, "work_items": st.n_lds_groups,
                   "spilled_kmers": st.n_hbm_parts},
         "kernels_ms": {k_: round(v[0], 4) for k_, v in kern.items()},
+        "kernel_timing": f"HIP events around each kernel, {prof_steps} profiled jobs after the timed region",
         "roofline": roof,
     }
     if args.samples > 0 and args.only == "all":

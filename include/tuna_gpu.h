@@ -98,6 +98,12 @@ int tg_count_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
 int tg_count_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases,
                     uint64_t n_bases, const uint64_t* d_off, uint64_t n_chunks,
                     tg_stats* st);
+/* How a context runs single jobs.  Default (0): through the batched engine as
+ * a batch of one job (one captured launch sequence, no host round trip before
+ * the result).  1: the plain launch sequence, driven from the host between
+ * its steps (calibration and sizing read back in between), as
+ * partition_kmers_mem then count_and_write_mem run in the reference. */
+int tg_set_plain_sequence(tg_ctx* ctx, int on);
 /* Result of the last job: n_out (key, count) pairs in unspecified order. */
 uint64_t tg_result_size(tg_ctx* ctx);
 /* keys: n*ceil(k/32) words; counts: n.  Copies min(n, cap) pairs. */

@@ -28,6 +28,16 @@ def eng():
     e.close()
 
 
+@pytest.fixture(scope="module", params=["batched", "plain"])
+def ceng(request):
+    """Counting engine in both single-job modes: the batched engine (one
+    captured launch sequence) and the plain host-driven sequence."""
+    e = T.Engine(0)
+    e.set_plain_sequence(request.param == "plain")
+    yield e
+    e.close()
+
+
 def rs(rng, n, alph="ACGT"):
     return "".join(rng.choice(alph) for _ in range(n))
 
@@ -120,23 +130,23 @@ def test_superkmers_match_oracle(eng, k, m):
 
 @pytest.mark.parametrize("k,m", [(31, 21), (21, 15), (63, 21), (2, 1), (5, 3), (15, 7), (32, 16),
                                  (33, 15), (45, 9), (64, 31), (25, 11)])
-def test_counts_match_oracle_random_chunks(eng, k, m):
+def test_counts_match_oracle_random_chunks(ceng, k, m):
     rng = random.Random(1000 + k)
     strings = [rs(rng, rng.randint(1, 300)) for _ in range(500)] + [rs(rng, 20000)]
     bases, off = O.chunks_of(strings)
-    check_counts(eng, bases, off, k, m, 16, label="random")
+    check_counts(ceng, bases, off, k, m, 16, label="random")
 
 
-def test_counts_low_complexity_and_caps(eng):
+def test_counts_low_complexity_and_caps(ceng):
     rng = random.Random(7)
     strings = ["A" * 7000, "AC" * 3000, rs(rng, 3000) + "T" * 2600 + rs(rng, 3000),
                "CAGT" * 2000, ("ACGGT" * 40 + rs(rng, 5)) * 30]
     bases, off = O.chunks_of(strings)
     for (k, m) in [(31, 21), (63, 21), (21, 15)]:
-        check_counts(eng, bases, off, k, m, 4, label="lowcomplexity")
+        check_counts(ceng, bases, off, k, m, 4, label="lowcomplexity")
 
 
-def test_counts_high_multiplicity_and_filters(eng):
+def test_counts_high_multiplicity_and_filters(ceng):
     rng = random.Random(8)
     g = rs(rng, 5000)
     reads = []
@@ -144,45 +154,45 @@ def test_counts_high_multiplicity_and_filters(eng):
         p = rng.randint(0, len(g) - 150)
         reads.append(g[p:p + 150])
     bases, off = O.chunks_of(reads)
-    check_counts(eng, bases, off, 31, 21, 16, label="coverage")
-    check_counts(eng, bases, off, 31, 21, 16, ci=2, label="ci2")
-    check_counts(eng, bases, off, 31, 21, 16, ci=50, cx=100, label="ci50cx100")
-    check_counts(eng, bases, off, 63, 21, 16, ci=3, label="k63ci3")
+    check_counts(ceng, bases, off, 31, 21, 16, label="coverage")
+    check_counts(ceng, bases, off, 31, 21, 16, ci=2, label="ci2")
+    check_counts(ceng, bases, off, 31, 21, 16, ci=50, cx=100, label="ci50cx100")
+    check_counts(ceng, bases, off, 63, 21, 16, ci=3, label="k63ci3")
 
 
-def test_counts_edge_inputs(eng):
+def test_counts_edge_inputs(ceng):
     # all chunks shorter than k, single chunk of exactly k, empty input
     for strings, k in [(["ACGT", "ACG", "A"], 5), (["ACGTACGTAC"], 10), ([], 31), (["ACGTACGTACGTA"], 13)]:
         bases, off = O.chunks_of(strings) if strings else (np.zeros(0, np.uint8), np.zeros(1, np.uint64))
-        check_counts(eng, bases, off, k, min(3, k - 1), 16, label="edge")
+        check_counts(ceng, bases, off, k, min(3, k - 1), 16, label="edge")
 
 
-def test_counts_fastq_synthetic_matches_oracle(eng):
+def test_counts_fastq_synthetic_matches_oracle(ceng):
     text = T.synth("kind=reads_fq genome=200000 seed=2 cov=20 rl=150 sub=0.005 nrate=0.0001")
     bases, off = T.parse_buffer(text)
     ob, oo = O.parse(text)
     assert np.array_equal(bases, ob) and np.array_equal(off, oo)
-    check_counts(eng, bases, off, 31, 21, 256, ci=2, label="reads_fq ci2")
-    check_counts(eng, bases, off, 31, 21, 256, ci=1, label="reads_fq")
+    check_counts(ceng, bases, off, 31, 21, 256, ci=2, label="reads_fq ci2")
+    check_counts(ceng, bases, off, 31, 21, 256, ci=1, label="reads_fq")
 
 
-def test_counts_genome_c1_shape(eng):
+def test_counts_genome_c1_shape(ceng):
     text = T.synth("kind=genome_fa len=4600000 seed=1")
     bases, off = T.parse_buffer(text)
-    st = check_counts(eng, bases, off, 31, 21, 16, label="C1 genome")
+    st = check_counts(ceng, bases, off, 31, 21, 16, label="C1 genome")
     _, _, _, _, km = O.superkmers(bases, off, 31, 21)
     assert st.n_superkmers == len(O.superkmers(bases, off, 31, 21)[0])
 
 
-def test_repeat_genome_reads(eng):
+def test_repeat_genome_reads(ceng):
     text = T.synth("kind=repeat_fq genome=300000 seed=3 gc=0.45 cov=5 rl=150 sub=0.003")
     bases, off = T.parse_buffer(text)
-    check_counts(eng, bases, off, 31, 23, 64, label="repeat reads")
+    check_counts(ceng, bases, off, 31, 23, 64, label="repeat reads")
 
 
-def test_repeated_calls_reuse_buffers(eng):
+def test_repeated_calls_reuse_buffers(ceng):
     rng = random.Random(9)
     for trial in range(3):
         strings = [rs(rng, rng.randint(50, 2000)) for _ in range(50 - 20 * trial)]
         bases, off = O.chunks_of(strings)
-        check_counts(eng, bases, off, 31, 21, 16, label=f"reuse{trial}")
+        check_counts(ceng, bases, off, 31, 21, 16, label=f"reuse{trial}")

@@ -34,7 +34,7 @@ EXPORTS = [
     "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
-    "tg_host_alloc", "tg_host_free", "tg_autotune_many",
+    "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
 ]
 
 
@@ -182,6 +182,7 @@ def lib():
     L.tg_server_stats.argtypes = [_vp, ctypes.POINTER(TgServerInfo)]
     L.tg_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(_vp)]
     L.tg_host_free.argtypes = [_vp]
+    L.tg_set_plain_sequence.argtypes = [_vp, ctypes.c_int]
     L.tg_autotune_many.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
                                    ctypes.c_uint64, _u64p, _u64p, ctypes.c_uint32,
                                    ctypes.POINTER(TgBatchConfig), ctypes.POINTER(ctypes.c_double)]
@@ -420,6 +421,11 @@ class Engine:
         n = lib().tg_kernel_times(self._h, names, 8192, ms, ln, 64)
         keys = names.raw.split(b"\0")[:n]
         return {keys[i].decode(): (ms[i], ln[i]) for i in range(min(n, 64))}
+
+    def set_plain_sequence(self, on=True):
+        """Run single jobs with the plain host-driven launch sequence (True)
+        or through the batched engine as a batch of one (False, default)."""
+        _check(lib().tg_set_plain_sequence(self._h, 1 if on else 0))
 
     def reset_kernel_times(self):
         lib().tg_reset_kernel_times(self._h)

@@ -35,14 +35,14 @@ struct BatchClass {
     uint32_t k = 0, m = 0, ci = 0, nw = 0;
     uint64_t cx = 0;
     uint64_t cap_bases = 0, cap_ch = 0, cap_strips = 0;
-    uint32_t cap_jobs = 0, nsp = 0, xb = 0, max_tiles = 0, fixb = 0;
-    uint64_t sp_slots = 0;
+    uint32_t cap_jobs = 0, nsp = 0, xb = 0, max_tiles = 0, fixb = 0, rw = 4;
+    uint64_t sp_slots = 0, rec_cap = 0;
     bool operator==(const BatchClass& o) const
     {
         return k == o.k && m == o.m && ci == o.ci && nw == o.nw && cx == o.cx &&
                cap_bases == o.cap_bases && cap_ch == o.cap_ch && cap_strips == o.cap_strips &&
                cap_jobs == o.cap_jobs && nsp == o.nsp && xb == o.xb && max_tiles == o.max_tiles &&
-               fixb == o.fixb && sp_slots == o.sp_slots;
+               fixb == o.fixb && sp_slots == o.sp_slots && rw == o.rw && rec_cap == o.rec_cap;
     }
     // every capacity of o fits in this class (same parameters)
     bool covers(const BatchClass& o) const
@@ -50,7 +50,7 @@ struct BatchClass {
         return k == o.k && m == o.m && ci == o.ci && nw == o.nw && cx == o.cx &&
                cap_bases >= o.cap_bases && cap_ch >= o.cap_ch && cap_strips >= o.cap_strips &&
                cap_jobs >= o.cap_jobs && nsp >= o.nsp && max_tiles == o.max_tiles &&
-               fixb >= o.fixb && sp_slots >= o.sp_slots;
+               fixb >= o.fixb && sp_slots >= o.sp_slots && rw == o.rw && rec_cap >= o.rec_cap;
     }
 };
 
@@ -63,10 +63,9 @@ struct BatchBufs {
     uint32_t* packed = nullptr;
     uint64_t cap_words = 0;
     uint64_t *boff, *nstr, *nkm, *nstr_x, *nkm_x, *scan_tmp;
-    uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_cnt, *flags;
+    uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_start, *run_cnt, *flags;
     uint64_t *strip_slot, *run_cnt64, *run_off;
-    uint2 **run_ptr;
-    uint2 *rec, *rec_fix, *sorted, *route_tmp;
+    uint32_t *recs, *sorted, *route_tmp;  // rw words per record
     uint32_t *hsk, *hkm, *bucket_job, *item_job;
     uint64_t *part_sk, *part_km, *part_off, *bucket_item;
     uint4* items;
@@ -76,16 +75,28 @@ struct BatchBufs {
     unsigned long long* counters;
     uint64_t *out_keys, *fin_keys;
     uint32_t *out_cnt, *fin_cnt;
-    unsigned long long* job_acc;  // [0,J): distinct, [J,2J): spill reported, [2J,3J): spill cursor
+    unsigned long long* job_acc;  // [0,J): distinct, [J,2J): spill reported, [2J,3J): spill cursor,
+                                  // [3J,4J): superkmers
     uint64_t *job_tot, *job_item_tot, *job_nsk, *job_nkm, *job_off;
     uint64_t n_blocks = 0, n_runs = 0;
 };
 
-// flags block (u32): [0] flagged strips, [1] fix-up overflow, [2] spill
+// flags block (u32): [0] flagged strips, [1] phase-1 errors (bit 0 fix-up
+// capacity, bit 1 partition-buffer capacity, bit 2 offsets), [2] spill
 // overflow, [3] work items, [4] item queue
 enum { F_FLAGS = 0, F_SKERR = 1, F_SPERR = 2, F_NITEMS = 3, F_ITEMQ = 4, F_WORDS = 16 };
 
 }  // namespace
+
+// Device-resident results of a call: keys (nw words each) and counts, job j
+// at [off[j], off[j] + len[j]).
+struct ResultStore {
+    uint64_t* keys = nullptr;
+    uint32_t* cnt = nullptr;
+    uint64_t cap = 0, n = 0;
+    uint32_t nw = 1;
+    std::vector<uint64_t> off, len;
+};
 
 struct BatchState {
     BatchClass cls;
@@ -100,13 +111,15 @@ struct BatchState {
     uint32_t count_blocks = 0;
     tg_batch_config cfg{};
     double rho = 0.3;          // distinct / k-mers, updated from every batch
-    // results
-    uint64_t* store_keys = nullptr;
-    uint32_t* store_cnt = nullptr;
-    uint64_t store_cap = 0, store_n = 0;
-    uint32_t store_nw = 1;
-    std::vector<uint64_t> res_off, res_n;
+    // results: of the last tg_count_many* call, and of the last single job
+    ResultStore many, one;
     tg_many_stats last{};
+    // device-time split of the last call (phase 1: pack..routing; phase 2:
+    // counting and gathering), from events around and inside the sequence
+    hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_end = nullptr;
+    bool mid_in_graph = true;   // the captured graph records ev_mid
+    double ms_p1 = 0, ms_p2 = 0;
+    uint64_t last_items = 0, last_spilled = 0;
 };
 
 void destroy_batch(BatchState* s);
@@ -169,21 +182,23 @@ int alloc_class(BatchState* s, const BatchClass& c)
     const uint64_t nch1 = c.cap_ch + 1, ns1 = c.cap_strips + 1;
     b.n_blocks = (c.cap_strips + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
     b.n_runs = b.n_blocks + c.fixb;
+    const uint64_t slots = c.cap_bases + (uint64_t)c.fixb * tg::SK_BLOCK * STRIP;
     const uint64_t scan_words =
         tg::scan_tmp_words(std::max<uint64_t>({nch1, b.n_runs + 1, c.nsp + 1ull, ITEM_CAP + 1ull,
                                                c.cap_jobs + 1ull})) + 64;
-    const uint64_t recs = c.cap_bases + 1;
+    const uint64_t outs = c.cap_bases + 1;
     if ((rc = balloc(s, &b.packed, b.cap_words)) || (rc = balloc(s, &b.boff, nch1)) ||
         (rc = balloc(s, &b.nstr, nch1)) || (rc = balloc(s, &b.nkm, nch1)) ||
         (rc = balloc(s, &b.nstr_x, nch1)) || (rc = balloc(s, &b.nkm_x, nch1)) ||
         (rc = balloc(s, &b.scan_tmp, scan_words)) || (rc = balloc(s, &b.chunk_job, nch1)) ||
         (rc = balloc(s, &b.strip_chunk, ns1)) || (rc = balloc(s, &b.strip_tb, ns1)) ||
         (rc = balloc(s, &b.strip_slot, ns1)) || (rc = balloc(s, &b.flag_list, ns1)) ||
-        (rc = balloc(s, &b.flags, F_WORDS)) || (rc = balloc(s, &b.run_ptr, b.n_runs + 1)) ||
+        (rc = balloc(s, &b.flags, F_WORDS)) || (rc = balloc(s, &b.run_start, b.n_runs + 1)) ||
         (rc = balloc(s, &b.run_cnt, b.n_runs + 1)) || (rc = balloc(s, &b.run_cnt64, b.n_runs + 1)) ||
-        (rc = balloc(s, &b.run_off, b.n_runs + 2)) || (rc = balloc(s, &b.rec, recs)) ||
-        (rc = balloc(s, &b.rec_fix, (uint64_t)c.fixb * tg::SK_BLOCK * STRIP)) ||
-        (rc = balloc(s, &b.sorted, recs)) || (rc = balloc(s, &b.route_tmp, recs)) ||
+        (rc = balloc(s, &b.run_off, b.n_runs + 2)) ||
+        (rc = balloc(s, &b.recs, (slots + 1) * c.rw)) ||
+        (rc = balloc(s, &b.sorted, (c.rec_cap + 1) * c.rw)) ||
+        (rc = balloc(s, &b.route_tmp, (c.rec_cap + 1) * c.rw)) ||
         (rc = balloc(s, &b.hsk, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.hkm, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.part_sk, c.nsp + 1)) || (rc = balloc(s, &b.part_km, c.nsp + 1)) ||
@@ -193,9 +208,9 @@ int alloc_class(BatchState* s, const BatchClass& c)
         (rc = balloc(s, &b.seg_n, ITEM_CAP + 1)) || (rc = balloc(s, &b.seg_x, ITEM_CAP + 2)) ||
         (rc = balloc(s, &b.sp_keys, c.sp_slots * (c.nw + 1))) ||
         (rc = balloc(s, &b.sp_cnt, c.sp_slots)) || (rc = balloc(s, &b.counters, 8)) ||
-        (rc = balloc(s, &b.out_keys, recs * c.nw)) || (rc = balloc(s, &b.out_cnt, recs)) ||
-        (rc = balloc(s, &b.fin_keys, recs * c.nw)) || (rc = balloc(s, &b.fin_cnt, recs)) ||
-        (rc = balloc(s, &b.job_acc, 3ull * c.cap_jobs)) ||
+        (rc = balloc(s, &b.out_keys, outs * c.nw)) || (rc = balloc(s, &b.out_cnt, outs)) ||
+        (rc = balloc(s, &b.fin_keys, outs * c.nw)) || (rc = balloc(s, &b.fin_cnt, outs)) ||
+        (rc = balloc(s, &b.job_acc, 4ull * c.cap_jobs)) ||
         (rc = balloc(s, &b.job_tot, c.cap_jobs + 1)) ||
         (rc = balloc(s, &b.job_item_tot, c.cap_jobs)) || (rc = balloc(s, &b.job_nsk, c.cap_jobs)) ||
         (rc = balloc(s, &b.job_nkm, c.cap_jobs)) || (rc = balloc(s, &b.job_off, c.cap_jobs + 2)))
@@ -206,27 +221,37 @@ int alloc_class(BatchState* s, const BatchClass& c)
 }
 
 // The launch sequence of one batch (no host synchronisation inside).
-void enqueue_batch(BatchState* s, hipStream_t st)
+// prof: the context whose per-kernel timing is on (nullptr while capturing);
+// mid: record ev_mid between the phases.
+void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
 {
     const BatchClass& c = s->cls;
     BatchBufs& b = s->b;
     const tg::BatchJobs jobs{b.job_chunk, b.job_route};
     const uint32_t J = c.cap_jobs;
-    // clear state (kernels only: the sequence may be captured)
-    tg::launch_fill_u32(st, b.flags, 0u, F_WORDS);
-    tg::launch_fill_u64(st, (uint64_t*)b.counters, 0ull, 8);
-    tg::launch_fill_u32(st, b.sp_cnt, 0u, c.sp_slots);
-    tg::launch_fill_u64(st, b.seg_n, 0ull, ITEM_CAP + 1);
-    tg::launch_fill_u64(st, (uint64_t*)b.job_acc, 0ull, 3ull * J);
-    // inputs
-    tg::launch_batch_setup(st, b.desc, jobs, J, b.boff, b.chunk_job, c.cap_ch, b.bucket_job, c.nsp,
-                           b.flags + F_SKERR);
-    tg::launch_pack_desc(st, b.desc, b.packed, b.cap_words);
-    tg::launch_chunk_counts(st, b.boff, c.cap_ch, c.k, STRIP, b.nstr, b.nkm);
-    tg::launch_scan_u64(st, b.nstr, c.cap_ch, b.nstr_x, b.scan_tmp);
-    tg::launch_scan_u64(st, b.nkm, c.cap_ch, b.nkm_x, b.scan_tmp);
-    tg::launch_fill_strips(st, c.cap_ch, b.nstr_x, b.nkm_x, b.nkm, STRIP, b.strip_chunk, b.strip_tb,
-                           b.strip_slot);
+    {
+        // clear state (kernels only: the sequence may be captured) and inputs
+        Prof pr(prof, "batch_init");
+        tg::launch_fill_u32(st, b.flags, 0u, F_WORDS);
+        tg::launch_fill_u64(st, (uint64_t*)b.counters, 0ull, 8);
+        tg::launch_fill_u32(st, b.sp_cnt, 0u, c.sp_slots);
+        tg::launch_fill_u64(st, b.seg_n, 0ull, ITEM_CAP + 1);
+        tg::launch_fill_u64(st, (uint64_t*)b.job_acc, 0ull, 4ull * J);
+        tg::launch_batch_setup(st, b.desc, jobs, J, b.boff, b.chunk_job, c.cap_ch, b.bucket_job, c.nsp,
+                               b.flags + F_SKERR);
+    }
+    {
+        Prof pr(prof, "pack");
+        tg::launch_pack_desc(st, b.desc, b.packed, b.cap_words);
+    }
+    {
+        Prof pr(prof, "strips");
+        tg::launch_chunk_counts(st, b.boff, c.cap_ch, c.k, STRIP, b.nstr, b.nkm);
+        tg::launch_scan_u64(st, b.nstr, c.cap_ch, b.nstr_x, b.scan_tmp);
+        tg::launch_scan_u64(st, b.nkm, c.cap_ch, b.nkm_x, b.scan_tmp);
+        tg::launch_fill_strips(st, c.cap_ch, b.nstr_x, b.nkm_x, b.nkm, STRIP, b.strip_chunk, b.strip_tb,
+                               b.strip_slot);
+    }
     // superkmers: main pass over the strip capacity, fix-up over FIXB blocks
     tg::SkArgs a{};
     a.packed = b.packed;
@@ -241,6 +266,7 @@ void enqueue_batch(BatchState* s, hipStream_t st)
     a.W = c.k - c.m + 1;
     a.S = STRIP;
     a.capk = 256u - c.k;
+    a.piece = 16u * (c.rw - 1u) - c.k + 1u;
     a.lookback = a.capk + 1u;
     a.log2n = 0;
     a.xb = c.xb;
@@ -249,26 +275,32 @@ void enqueue_batch(BatchState* s, hipStream_t st)
     a.n_dev = b.nstr_x + c.cap_ch;
     a.fix_dev = 0;
     a.err = b.flags + F_SKERR;
-    a.rec = b.rec;
-    a.run_ptr = b.run_ptr;
+    a.recs = b.recs;
+    a.fix_base = (uint32_t)c.cap_bases;
+    a.run_start = b.run_start;
     a.run_cnt = b.run_cnt;
     a.run_base = 0;
+    a.sk_count = b.job_acc + 3ull * J;
     a.flag_list = b.flag_list;
     a.flag_count = b.flags + F_FLAGS;
-    tg::launch_superkmers(st, a);
+    {
+        Prof pr(prof, "superkmers");
+        tg::launch_superkmers(st, a, c.rw);
+    }
     a.list = b.flag_list;
     a.n = c.fixb * tg::SK_BLOCK;
     a.n_dev = nullptr;
     a.fix_dev = 1;
     a.lookback = tg::LOOKBACK_FULL;
-    a.rec = b.rec_fix;
     a.run_base = (uint32_t)b.n_blocks;
-    tg::launch_superkmers(st, a);
+    {
+        Prof pr(prof, "superkmers_fixup");
+        tg::launch_superkmers(st, a, c.rw);
+    }
     // routing into job buckets
-    tg::launch_u32_to_u64(st, b.run_cnt, b.n_runs, b.run_cnt64);
-    tg::launch_scan_u64(st, b.run_cnt64, b.n_runs, b.run_off, b.scan_tmp);
     tg::RouteArgs r{};
-    r.run_ptr = b.run_ptr;
+    r.recs = b.recs;
+    r.run_start = b.run_start;
     r.run_cnt = b.run_cnt;
     r.run_off = b.run_off;
     r.n_runs = (uint32_t)b.n_runs;
@@ -281,15 +313,34 @@ void enqueue_batch(BatchState* s, hipStream_t st)
     r.dst = b.sorted;
     r.n_rec_dev = b.run_off + b.n_runs;
     r.max_tiles = c.max_tiles;
-    tg::launch_route_hist(st, r, c.max_tiles);
-    tg::launch_route_colscan(st, r, c.max_tiles, b.part_sk, b.part_km);
-    tg::launch_scan_u64(st, b.part_sk, c.nsp, b.part_off, b.scan_tmp);
-    tg::launch_route_scatter(st, r, c.max_tiles, b.route_tmp);
+    r.rec_cap = c.rec_cap;
+    r.err = b.flags + F_SKERR;
+    {
+        Prof pr(prof, "route_scan");
+        tg::launch_u32_to_u64(st, b.run_cnt, b.n_runs, b.run_cnt64);
+        tg::launch_scan_u64(st, b.run_cnt64, b.n_runs, b.run_off, b.scan_tmp);
+    }
+    {
+        Prof pr(prof, "route_hist");
+        tg::launch_route_hist(st, r, c.max_tiles, c.rw);
+    }
+    {
+        Prof pr(prof, "route_scan");
+        tg::launch_route_colscan(st, r, c.max_tiles, b.part_sk, b.part_km);
+        tg::launch_scan_u64(st, b.part_sk, c.nsp, b.part_off, b.scan_tmp);
+    }
+    {
+        Prof pr(prof, "route_scatter");
+        tg::launch_route_scatter(st, r, c.max_tiles, b.route_tmp, c.rw);
+    }
+    if (mid) (void)hipEventRecord(s->ev_mid, st);
     // work items and counting
-    tg::launch_build_items(st, b.part_km, b.part_off, c.nsp, c.xb, b.bucket_job, &b.desc->scale,
-                           b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS);
+    {
+        Prof pr(prof, "count_items");
+        tg::launch_build_items(st, b.part_km, b.part_off, c.nsp, c.xb, b.bucket_job, &b.desc->scale,
+                               b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS);
+    }
     tg::CountArgs ca{};
-    ca.packed = b.packed;
     ca.rec = b.sorted;
     ca.items = b.items;
     ca.k = c.k;
@@ -311,38 +362,54 @@ void enqueue_batch(BatchState* s, hipStream_t st)
     ca.seg_pos = b.seg_pos;
     ca.seg_n = b.seg_n;
     ca.job_uniq = b.job_acc;
-    tg::launch_count_batch(st, ca, c.nw, s->count_blocks);
-    // per-job results in job order
-    tg::launch_spill_jobs(st, ca, c.nw, c.sp_slots, b.job_acc + J);
-    tg::launch_scan_u64(st, b.seg_n, ITEM_CAP, b.seg_x, b.scan_tmp);
-    tg::launch_job_totals(st, b.desc, jobs, J, b.bucket_item, b.seg_x, b.part_off, b.job_acc + J,
-                          b.nkm_x, b.job_tot, b.job_item_tot, b.job_nsk, b.job_nkm);
-    tg::launch_scan_u64(st, b.job_tot, J, b.job_off, b.scan_tmp);
-    tg::launch_gather_items(st, ca, c.nw, jobs, b.bucket_item, b.seg_x, b.job_off, b.fin_keys,
-                            b.fin_cnt, GATHER_BLOCKS);
-    tg::launch_spill_place(st, ca, c.nw, c.sp_slots, b.job_off, b.job_item_tot, b.job_acc + 2ull * J,
-                           b.fin_keys, b.fin_cnt);
+    {
+        Prof pr(prof, "count");
+        tg::launch_count_batch(st, ca, c.nw, s->count_blocks);
+    }
+    {
+        // per-job results in job order
+        Prof pr(prof, "job_gather");
+        tg::launch_spill_jobs(st, ca, c.nw, c.sp_slots, b.job_acc + J);
+        tg::launch_scan_u64(st, b.seg_n, ITEM_CAP, b.seg_x, b.scan_tmp);
+        tg::launch_job_totals(st, b.desc, jobs, J, b.bucket_item, b.seg_x, b.job_acc + 3ull * J,
+                              b.job_acc + J, b.nkm_x, b.job_tot, b.job_item_tot, b.job_nsk, b.job_nkm);
+        tg::launch_scan_u64(st, b.job_tot, J, b.job_off, b.scan_tmp);
+        tg::launch_gather_items(st, ca, c.nw, jobs, b.bucket_item, b.seg_x, b.job_off, b.fin_keys,
+                                b.fin_cnt, GATHER_BLOCKS);
+        tg::launch_spill_place(st, ca, c.nw, c.sp_slots, b.job_off, b.job_item_tot,
+                               b.job_acc + 2ull * J, b.fin_keys, b.fin_cnt);
+    }
 }
 
 int ensure_graph(BatchState* s, hipStream_t st)
 {
     if (s->exec) return TG_OK;
-    // kernel attributes are set outside the capture
-    HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    enqueue_batch(s, st);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(st, &g);
-    if (e != hipSuccess) return fail(TG_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    s->graph = g;
-    HIPOK(hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0));
-    s->last.n_captures += 1;
-    return TG_OK;
+    // kernel attributes are set outside the capture; the phase-split event is
+    // recorded inside the graph when the runtime supports capturing it
+    for (int attempt = 0;; ++attempt) {
+        HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_batch(s, st, nullptr, s->mid_in_graph);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) {
+            s->graph = g;
+            s->last.n_captures += 1;
+            return TG_OK;
+        }
+        (void)hipGetLastError();
+        if (g) hipGraphDestroy(g);
+        s->exec = nullptr;
+        if (!s->mid_in_graph || attempt)
+            return fail(TG_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        s->mid_in_graph = false;
+    }
 }
 
-int grow_store(BatchState* s, hipStream_t st, uint64_t need, uint32_t nw)
+int grow_store(ResultStore& r, hipStream_t st, uint64_t need, uint32_t nw)
 {
-    if (s->store_cap >= need && s->store_nw == nw) return TG_OK;
-    uint64_t cap = std::max<uint64_t>(need, s->store_cap + s->store_cap / 2);
+    if (r.cap >= need && r.nw == nw) return TG_OK;
+    uint64_t cap = std::max<uint64_t>(need, r.cap + r.cap / 2);
     cap = std::max<uint64_t>(cap, 1u << 20);
     uint64_t* k2 = nullptr;
     uint32_t* c2 = nullptr;
@@ -352,18 +419,25 @@ int grow_store(BatchState* s, hipStream_t st, uint64_t need, uint32_t nw)
         return fail(TG_ENOMEM, "result store of %llu k-mers does not fit in device memory",
                     (unsigned long long)cap);
     }
-    if (s->store_n) {
-        HIPOK(hipMemcpyAsync(k2, s->store_keys, s->store_n * nw * 8ull, hipMemcpyDeviceToDevice, st));
-        HIPOK(hipMemcpyAsync(c2, s->store_cnt, s->store_n * 4ull, hipMemcpyDeviceToDevice, st));
+    if (r.n && r.nw == nw) {
+        HIPOK(hipMemcpyAsync(k2, r.keys, r.n * nw * 8ull, hipMemcpyDeviceToDevice, st));
+        HIPOK(hipMemcpyAsync(c2, r.cnt, r.n * 4ull, hipMemcpyDeviceToDevice, st));
     }
     HIPOK(hipStreamSynchronize(st));
-    if (s->store_keys) hipFree(s->store_keys);
-    if (s->store_cnt) hipFree(s->store_cnt);
-    s->store_keys = k2;
-    s->store_cnt = c2;
-    s->store_cap = cap;
-    s->store_nw = nw;
+    if (r.keys) hipFree(r.keys);
+    if (r.cnt) hipFree(r.cnt);
+    r.keys = k2;
+    r.cnt = c2;
+    r.cap = cap;
+    r.nw = nw;
     return TG_OK;
+}
+
+void free_store(ResultStore& r)
+{
+    if (r.keys) hipFree(r.keys);
+    if (r.cnt) hipFree(r.cnt);
+    r = ResultStore{};
 }
 
 struct JobTab {
@@ -431,6 +505,10 @@ BatchClass class_for(const tg_params* p, const JobTab& jt, const std::vector<uin
     c.nsp = (uint32_t)pow2_ceil64(std::max<uint32_t>(buckets, 2));
     c.xb = tg::HSEL_BITS - ilog2(c.nsp);
     c.sp_slots = pow2_ceil64(std::max<uint64_t>(c.cap_bases / 64, 1u << 16));
+    c.rw = tg::fat_rw(c.k);
+    // partition buffers: a typical bound (a superkmer every (W+1)/4 k-mers); a
+    // batch that overflows it re-runs at the worst case (one per k-mer)
+    c.rec_cap = std::min<uint64_t>(c.cap_bases, c.cap_bases * 4 / (c.k - c.m + 2) + 2 * c.cap_strips + 65536);
     return c;
 }
 
@@ -440,10 +518,13 @@ void destroy_batch(BatchState* s)
 {
     if (!s) return;
     free_class(s);
+    if (s->ev_start) hipEventDestroy(s->ev_start);
+    if (s->ev_mid) hipEventDestroy(s->ev_mid);
+    if (s->ev_end) hipEventDestroy(s->ev_end);
     if (s->h_tab) hipHostFree(s->h_tab);
     if (s->h_back) hipHostFree(s->h_back);
-    if (s->store_keys) hipFree(s->store_keys);
-    if (s->store_cnt) hipFree(s->store_cnt);
+    free_store(s->many);
+    free_store(s->one);
     delete s;
 }
 
@@ -460,13 +541,17 @@ BatchState* state(tg_ctx* c)
         g.load_factor = 0.75f;
         g.use_graph = 1;
         g.rho = 0.0f;
+        BatchState* s = c->batch;
+        if (hipEventCreate(&s->ev_start) != hipSuccess || hipEventCreate(&s->ev_mid) != hipSuccess ||
+            hipEventCreate(&s->ev_end) != hipSuccess)
+            (void)hipGetLastError();
     }
     return c->batch;
 }
 
 int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
                       const uint64_t* d_off, uint64_t n_chunks, const JobTab& jt,
-                      tg_job_stats* js)
+                      tg_job_stats* js, bool single = false)
 {
     int rc = check_params(p, 0, 0);
     if (rc) return rc;
@@ -486,6 +571,8 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     hipStream_t st = ctx->stream;
     const auto t_start = std::chrono::steady_clock::now();
     s->last = tg_many_stats{};
+    s->ms_p1 = s->ms_p2 = 0;
+    s->last_items = s->last_spilled = 0;
     const uint32_t nw = p->k > 32 ? 2u : 1u;
     s->count_blocks = tg::count_persistent_blocks(nw);
     tg::set_kernel_attributes();
@@ -493,10 +580,11 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     plan_batches(jt, s->cfg, cuts);
     BatchClass want = class_for(p, jt, cuts, s->cfg);
     if (s->have_cls && s->cls.covers(want)) want = s->cls;
-    s->res_off.assign(jt.n, 0);
-    s->res_n.assign(jt.n, 0);
-    s->store_n = 0;
-    if ((rc = grow_store(s, st, 1, nw))) return rc;
+    ResultStore& out = single ? s->one : s->many;
+    out.off.assign(jt.n, 0);
+    out.len.assign(jt.n, 0);
+    out.n = 0;
+    if ((rc = grow_store(out, st, 1, nw))) return rc;
     uint64_t tot_kmers = 0, tot_uniq = 0;
     for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) {
         const uint32_t j0 = cuts[bi], j1 = cuts[bi + 1];
@@ -530,14 +618,19 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
             hjc[j1 - j0] = (uint32_t)(jt.job_chunk[j1] - c0);
             const uint64_t tab_bytes = 256 + 4ull * (c.cap_jobs + 1) + 4ull * c.cap_jobs;
             HIPOK(hipMemcpyAsync(s->b.tab, s->h_tab, tab_bytes, hipMemcpyHostToDevice, st));
-            if (s->cfg.use_graph) {
+            HIPOK(hipEventRecord(s->ev_start, st));
+            bool have_mid = true;
+            if (s->cfg.use_graph && !ctx->profiling) {
                 if ((rc = ensure_graph(s, st))) return rc;
                 HIPOK(hipGraphLaunch(s->exec, st));
                 s->last.n_graph_launches += 1;
+                have_mid = s->mid_in_graph;
             } else {
-                enqueue_batch(s, st);
+                // plain launches (also whenever per-kernel timing is on)
+                enqueue_batch(s, st, ctx, true);
                 HIPOK(hipGetLastError());
             }
+            HIPOK(hipEventRecord(s->ev_end, st));
             // readback: job offsets, superkmers, k-mers, distinct, flags, counters
             const uint32_t nj = j1 - j0;
             uint64_t* h_off = reinterpret_cast<uint64_t*>(s->h_back);
@@ -550,7 +643,20 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
             HIPOK(hipMemcpyAsync(h_nkm, s->b.job_nkm, nj * 8ull, hipMemcpyDeviceToHost, st));
             HIPOK(hipMemcpyAsync(h_unq, s->b.job_acc, nj * 8ull, hipMemcpyDeviceToHost, st));
             HIPOK(hipMemcpyAsync(h_flags, s->b.flags, 4ull * F_WORDS, hipMemcpyDeviceToHost, st));
+            unsigned long long* h_ctr = reinterpret_cast<unsigned long long*>(h_flags + F_WORDS);
+            HIPOK(hipMemcpyAsync(h_ctr, s->b.counters, 8ull * 8, hipMemcpyDeviceToHost, st));
             HIPOK(hipStreamSynchronize(st));
+            {
+                float a1 = 0, a2 = 0;
+                if (have_mid && hipEventElapsedTime(&a1, s->ev_start, s->ev_mid) == hipSuccess &&
+                    hipEventElapsedTime(&a2, s->ev_mid, s->ev_end) == hipSuccess) {
+                    s->ms_p1 += a1;
+                    s->ms_p2 += a2;
+                } else if (hipEventElapsedTime(&a1, s->ev_start, s->ev_end) == hipSuccess) {
+                    s->ms_p1 += a1;  // no split available: all of it as phase 1
+                }
+                (void)hipGetLastError();
+            }
             if (h_flags[F_SKERR] & 4u)
                 return fail(TG_EINVAL, "device chunk offsets disagree with the job table (batch %zu)",
                             bi);
@@ -558,42 +664,45 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
                 if (attempt >= 4)
                     return fail(TG_ENOMEM, "batch overflow after %d retries (fix-up %u, spill %u)",
                                 attempt, h_flags[F_SKERR], h_flags[F_SPERR]);
-                if (h_flags[F_SKERR])
+                if (h_flags[F_SKERR] & 1u)
                     want.fixb = (uint32_t)std::max<uint64_t>(
                         want.fixb * 2ull, (h_flags[F_FLAGS] + tg::SK_BLOCK - 1) / tg::SK_BLOCK);
+                if (h_flags[F_SKERR] & 2u) want.rec_cap = want.cap_bases;
                 if (h_flags[F_SPERR]) want.sp_slots <<= 2;
                 s->last.n_retries += 1;
                 continue;
             }
             const uint64_t n_out = h_off[nj];
-            if ((rc = grow_store(s, st, s->store_n + n_out, nw))) return rc;
+            if ((rc = grow_store(out, st, out.n + n_out, nw))) return rc;
             if (n_out) {
-                HIPOK(hipMemcpyAsync(s->store_keys + s->store_n * nw, s->b.fin_keys, n_out * nw * 8ull,
+                HIPOK(hipMemcpyAsync(out.keys + out.n * nw, s->b.fin_keys, n_out * nw * 8ull,
                                      hipMemcpyDeviceToDevice, st));
-                HIPOK(hipMemcpyAsync(s->store_cnt + s->store_n, s->b.fin_cnt, n_out * 4ull,
+                HIPOK(hipMemcpyAsync(out.cnt + out.n, s->b.fin_cnt, n_out * 4ull,
                                      hipMemcpyDeviceToDevice, st));
             }
             uint64_t bk = 0, bu = 0;
             for (uint32_t q = 0; q < nj; ++q) {
                 const uint32_t j = j0 + q;
-                s->res_off[j] = s->store_n + h_off[q];
-                s->res_n[j] = h_off[q + 1] - h_off[q];
+                out.off[j] = out.n + h_off[q];
+                out.len[j] = h_off[q + 1] - h_off[q];
                 if (js) {
                     js[j].n_bases = jt.job_base[j + 1] - jt.job_base[j];
                     js[j].n_kmers = h_nkm[q];
                     js[j].n_superkmers = h_nsk[q];
                     js[j].n_unique = h_unq[q];
-                    js[j].n_out = s->res_n[j];
+                    js[j].n_out = out.len[j];
                 }
                 bk += h_nkm[q];
                 bu += h_unq[q];
             }
-            s->store_n += n_out;
+            out.n += n_out;
             tot_kmers += bk;
             tot_uniq += bu;
             // distinct/k-mers of this batch sizes the splits of the next one
             if (bk) s->rho = std::min(1.0, std::max(0.02, 1.15 * (double)bu / (double)bk));
             s->last.n_batches += 1;
+            s->last_items += h_flags[F_NITEMS];
+            s->last_spilled += h_ctr[2];
             break;
         }
     }
@@ -601,12 +710,48 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     s->last.n_jobs = jt.n;
     s->last.n_kmers = tot_kmers;
     s->last.n_unique = tot_uniq;
-    s->last.n_out = s->store_n;
+    s->last.n_out = out.n;
     s->last.ms_wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return TG_OK;
 }
 
 }  // namespace
+
+int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                      const uint64_t* d_off, uint64_t n_chunks, tg_stats* st)
+{
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    const uint64_t jc[2] = {0, n_chunks};
+    const uint64_t jb[2] = {0, n_bases};
+    const JobTab jt{jc, jb, 1};
+    tg_job_stats js{};
+    if ((rc = count_many_device(c, p, d_bases, n_bases, d_off, n_chunks, jt, &js, true))) return rc;
+    BatchState* s = c->batch;
+    c->n_out = s->one.len[0];
+    c->nw = s->one.nw;
+    c->d_keys = s->one.keys + s->one.off[0] * s->one.nw;
+    c->d_cnt = s->one.cnt + s->one.off[0];
+    collect_prof(c);
+    if (st) {
+        tg_stats o{};
+        o.n_bases = n_bases;
+        o.n_chunks = n_chunks;
+        o.n_kmers = js.n_kmers;
+        o.n_superkmers = js.n_superkmers;
+        o.n_unique = js.n_unique;
+        o.n_out = js.n_out;
+        o.n_parts = round_pow2(p->n_partitions ? p->n_partitions : 16u);
+        o.n_subparts = job_buckets(n_bases, s->cfg.kmers_per_bucket);
+        o.n_lds_groups = (uint32_t)std::min<uint64_t>(s->last_items, 0xFFFFFFFFull);
+        o.n_hbm_parts = (uint32_t)std::min<uint64_t>(s->last_spilled, 0xFFFFFFFFull);
+        o.ms_phase1 = s->ms_p1;
+        o.ms_phase2 = s->ms_p2;
+        *st = o;
+    }
+    return TG_OK;
+}
+
 }  // namespace tgi
 
 using namespace tgi;
@@ -758,24 +903,24 @@ int tg_autotune_many(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint
 
 uint64_t tg_many_result_size(tg_ctx* c, uint32_t job)
 {
-    if (!c || !c->batch || job >= c->batch->res_n.size()) return 0;
-    return c->batch->res_n[job];
+    if (!c || !c->batch || job >= c->batch->many.len.size()) return 0;
+    return c->batch->many.len[job];
 }
 
 int tg_many_result_copy(tg_ctx* c, uint32_t job, uint64_t* keys, uint32_t* counts, uint64_t cap)
 {
     if (!c) return fail(TG_EINVAL, "null context");
     BatchState* s = c->batch;
-    if (!s || job >= s->res_n.size()) return fail(TG_EINVAL, "no result for job %u", job);
-    const uint64_t n = std::min<uint64_t>(cap, s->res_n[job]);
+    if (!s || job >= s->many.len.size()) return fail(TG_EINVAL, "no result for job %u", job);
+    const uint64_t n = std::min<uint64_t>(cap, s->many.len[job]);
     if (!n) return TG_OK;
     HIPOK(hipSetDevice(c->device));
-    const uint32_t nw = s->store_nw;
+    const uint32_t nw = s->many.nw;
     if (keys)
-        HIPOK(hipMemcpyAsync(keys, s->store_keys + s->res_off[job] * nw, n * nw * 8ull,
+        HIPOK(hipMemcpyAsync(keys, s->many.keys + s->many.off[job] * nw, n * nw * 8ull,
                              hipMemcpyDeviceToHost, c->stream));
     if (counts)
-        HIPOK(hipMemcpyAsync(counts, s->store_cnt + s->res_off[job], n * 4ull, hipMemcpyDeviceToHost,
+        HIPOK(hipMemcpyAsync(counts, s->many.cnt + s->many.off[job], n * 4ull, hipMemcpyDeviceToHost,
                              c->stream));
     HIPOK(hipStreamSynchronize(c->stream));
     return TG_OK;
@@ -786,10 +931,10 @@ int tg_many_result_device(tg_ctx* c, uint32_t job, const uint64_t** d_keys, cons
 {
     if (!c) return fail(TG_EINVAL, "null context");
     BatchState* s = c->batch;
-    if (!s || job >= s->res_n.size()) return fail(TG_EINVAL, "no result for job %u", job);
-    if (d_keys) *d_keys = s->store_keys + s->res_off[job] * s->store_nw;
-    if (d_counts) *d_counts = s->store_cnt + s->res_off[job];
-    if (n) *n = s->res_n[job];
+    if (!s || job >= s->many.len.size()) return fail(TG_EINVAL, "no result for job %u", job);
+    if (d_keys) *d_keys = s->many.keys + s->many.off[job] * s->many.nw;
+    if (d_counts) *d_counts = s->many.cnt + s->many.off[job];
+    if (n) *n = s->many.len[job];
     return TG_OK;
 }
 

@@ -20,29 +20,31 @@ namespace {
 
 // ---------------------------------------------------------------------------
 struct Phase1 {
-    uint32_t k = 0, m = 0, W = 0, S = 0;
+    uint32_t k = 0, m = 0, W = 0, S = 0, rw = 4;
     uint32_t n = 0, log2n = 0, nsp = 0, log2s = 0, sb = 0;
-    uint64_t n_strips = 0, n_kmers = 0, n_sk = 0;
+    uint64_t n_strips = 0, n_kmers = 0, n_sk = 0, n_rec = 0;
     uint32_t* packed = nullptr;
-    uint2* sorted = nullptr;    // partition order
+    uint32_t* sorted = nullptr;  // partition-ordered records (rw words each)
     uint64_t* part_off = nullptr;
     uint64_t* part_km = nullptr;
     uint32_t ntiles = 0;
     uint64_t n_flagged = 0;
 };
 
+// Phase 1: pack, strips, superkmers into bump-allocated runs, routing.
+// slim = true keeps (start, meta) records for the export entry points; the
+// counting path uses fat records that carry their bases.
 int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N,
-               const uint64_t* d_off, uint64_t nch, Phase1& o)
+               const uint64_t* d_off, uint64_t nch, Phase1& o, bool slim = false)
 {
     hipStream_t st = c->stream;
     o.k = p->k;
     o.m = p->m;
     o.W = p->k - p->m + 1;
+    o.rw = slim ? 2u : tg::fat_rw(p->k);
     o.n = round_pow2(p->n_partitions ? p->n_partitions : 16u);
     if (o.n > (1u << 14)) return fail(TG_EINVAL, "n_partitions %u exceeds 16384 on the GPU engine", o.n);
     o.log2n = ilog2(o.n);
-    // device sub-partitions are chosen after Phase 1 (records carry 18 hash
-    // bits; the sub-partition id is their top log2(nsp) bits)
     // strip length: short chunks are one strip; long chunks are cut so that
     // the look-back (256-k k-mers) stays a small fraction
     o.S = 2048;
@@ -76,153 +78,164 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     o.n_kmers = tot[1];
     if (o.n_strips >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
     const uint64_t n_blocks = (o.n_strips + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
-    uint32_t *strip_chunk, *strip_tb, *flag_list, *flag_count, *run_cnt;
-    uint64_t *strip_slot, *run_off;
-    uint2* rec;
-    uint2** run_ptr;
+    uint32_t *strip_chunk, *strip_tb, *flag_list, *flags, *run_start, *run_cnt;
+    uint64_t *run_off, *strip_slot;
+    unsigned long long* sk_count;
     const uint64_t ns1 = o.n_strips + 1;
-    // run table: one run per main-pass block, then the fix-up blocks
-    const uint64_t run_cap = n_blocks + n_blocks + 2;
     if ((rc = ensure_t(c, "strip_chunk", ns1, &strip_chunk)) ||
         (rc = ensure_t(c, "strip_tb", ns1, &strip_tb)) ||
         (rc = ensure_t(c, "strip_slot", ns1, &strip_slot)) ||
-        (rc = ensure_t(c, "flag_list", ns1, &flag_list)) ||
-        (rc = ensure_t(c, "flag_count", 64, &flag_count)) ||
-        (rc = ensure_t(c, "run_ptr", run_cap, &run_ptr)) ||
-        (rc = ensure_t(c, "run_cnt", run_cap, &run_cnt)) ||
-        (rc = ensure_t(c, "run_off", run_cap + 1, &run_off)) ||
-        (rc = ensure_t(c, "rec", o.n_kmers + 1, &rec)))
+        (rc = ensure_t(c, "flag_list", ns1, &flag_list)) || (rc = ensure_t(c, "sk_flags", 64, &flags)) ||
+        (rc = ensure_t(c, "sk_count", 8, &sk_count)))
         return rc;
     {
         Prof pr(c, "strips");
         tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, strip_slot);
-        tg::launch_fill_u32(st, flag_count, 0u, 1);
     }
-    tg::SkArgs a{};
-    a.packed = o.packed;
-    a.off = d_off;
-    a.strip_chunk = strip_chunk;
-    a.strip_tb = strip_tb;
-    a.strip_slot = strip_slot;
-    a.list = nullptr;
-    a.n = (uint32_t)o.n_strips;
-    a.k = o.k;
-    a.m = o.m;
-    a.W = o.W;
-    a.S = o.S;
-    a.capk = 256u - o.k;
-    a.lookback = a.capk + 1u;
-    a.log2n = o.log2n;
-    a.xb = tg::HSEL_BITS - o.log2n;
-    a.rec = rec;
-    a.run_ptr = run_ptr;
-    a.run_cnt = run_cnt;
-    a.run_base = 0;
-    a.flag_list = flag_list;
-    a.flag_count = flag_count;
-    {
-        Prof pr(c, "superkmers");
-        tg::launch_superkmers(st, a);
-    }
-    HIPOK(hipGetLastError());
-    uint32_t nflag = 0;
-    HIPOK(hipMemcpyAsync(&nflag, flag_count, 4, hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
-    o.n_flagged = nflag;
-    uint64_t n_runs = n_blocks;
-    if (nflag) {
-        // strips whose first run began before the look-back: redo them from
-        // the chunk start (only long low-complexity runs in long chunks)
-        const uint64_t fb = (nflag + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
-        uint32_t* list2;
-        uint2* rec_fix;
-        if ((rc = ensure_t(c, "flag_list2", nflag + 1, &list2)) ||
-            (rc = ensure_t(c, "rec_fix", fb * tg::SK_BLOCK * o.S, &rec_fix)))
+    // record space: the slot space of the strips (a strip of K k-mers has at
+    // most K records), then fix-up regions of SK_BLOCK*S records per block
+    uint64_t fixb = tg::FIX_BLOCKS;
+    for (;;) {
+        const uint64_t fix_recs = fixb * tg::SK_BLOCK * o.S;
+        const uint64_t rec_cap = o.n_kmers + fix_recs;
+        const uint64_t run_cap = n_blocks + fixb + 1;
+        if (rec_cap >= (1ull << 32)) return fail(TG_EINVAL, "input too large for one job's record space");
+        uint32_t* recs;
+        if ((rc = ensure_t(c, "recs", rec_cap * o.rw, &recs)) ||
+            (rc = ensure_t(c, "run_start", run_cap, &run_start)) ||
+            (rc = ensure_t(c, "run_cnt", run_cap, &run_cnt)) ||
+            (rc = ensure_t(c, "run_off", run_cap + 1, &run_off)))
             return rc;
-        HIPOK(hipMemcpyAsync(list2, flag_list, nflag * 4ull, hipMemcpyDeviceToDevice, st));
-        a.list = list2;
-        a.n = nflag;
-        a.lookback = tg::LOOKBACK_FULL;
-        a.rec = rec_fix;
-        a.run_base = (uint32_t)n_blocks;
         {
-            Prof pr(c, "superkmers_fixup");
-            tg::launch_superkmers(st, a);
+            Prof pr(c, "strips");
+            tg::launch_fill_u32(st, flags, 0u, 64);
+            tg::launch_fill_u64(st, (uint64_t*)sk_count, 0ull, 1);
+        }
+        tg::SkArgs a{};
+        a.packed = o.packed;
+        a.off = d_off;
+        a.strip_chunk = strip_chunk;
+        a.strip_tb = strip_tb;
+        a.strip_slot = strip_slot;
+        a.list = nullptr;
+        a.n = (uint32_t)o.n_strips;
+        a.k = o.k;
+        a.m = o.m;
+        a.W = o.W;
+        a.S = o.S;
+        a.capk = 256u - o.k;
+        a.piece = slim ? 0u : 16u * (o.rw - 1u) - o.k + 1u;
+        a.lookback = a.capk + 1u;
+        a.log2n = o.log2n;
+        a.xb = tg::HSEL_BITS - o.log2n;
+        a.err = flags + 1;
+        a.recs = recs;
+        a.fix_base = (uint32_t)o.n_kmers;
+        a.run_start = run_start;
+        a.run_cnt = run_cnt;
+        a.run_base = 0;
+        a.sk_count = sk_count;
+        a.flag_list = flag_list;
+        a.flag_count = flags;
+        {
+            Prof pr(c, "superkmers");
+            tg::launch_superkmers(st, a, o.rw);
         }
         HIPOK(hipGetLastError());
-        n_runs += fb;
-    }
-
-    // -- routing: tiles over the concatenated run space
-    uint64_t* run_cnt64;
-    if ((rc = ensure_t(c, "run_cnt64", n_runs + 1, &run_cnt64)) ||
-        (rc = ensure_t(c, "scan_tmp3", tg::scan_tmp_words(n_runs + 1) + 64, &tmp)))
-        return rc;
-    {
-        Prof pr(c, "route_scan");
-        tg::launch_u32_to_u64(st, run_cnt, n_runs, run_cnt64);
-        tg::launch_scan_u64(st, run_cnt64, n_runs, run_off, tmp);
-    }
-    HIPOK(hipMemcpyAsync(&o.n_sk, run_off + n_runs, 8, hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
-    uint64_t tile = 16384;
-    while ((o.n_sk + tile - 1) / tile > 512) tile <<= 1;
-    o.ntiles = (uint32_t)((o.n_sk + tile - 1) / tile);
-    // sub-partitions: ~16K k-mers each (N/16384), at least the reference n,
-    // at least 2, at most 2^14 (LDS histograms)
-    {
-        uint64_t want = 2;
-        while (want < o.n_kmers / 16384 && want < (1u << 14)) want <<= 1;
-        if (want < o.n) want = o.n;
-        o.nsp = (uint32_t)want;
-        o.log2s = ilog2(o.nsp) - o.log2n;
-        o.sb = tg::HSEL_BITS - ilog2(o.nsp);
-    }
-    uint32_t *hsk, *hkm;
-    uint64_t* part_sk;
-    uint2* route_tmp;
-    const uint64_t msz = (uint64_t)std::max<uint32_t>(o.ntiles, 1) * o.nsp;
-    if ((rc = ensure_t(c, "hsk", msz, &hsk)) || (rc = ensure_t(c, "hkm", msz, &hkm)) ||
-        (rc = ensure_t(c, "part_sk", o.nsp + 1, &part_sk)) ||
-        (rc = ensure_t(c, "part_km", o.nsp + 1, &o.part_km)) ||
-        (rc = ensure_t(c, "part_off", o.nsp + 1, &o.part_off)) ||
-        (rc = ensure_t(c, "sorted", o.n_sk + 1, &o.sorted)) ||
-        (rc = ensure_t(c, "route_tmp", o.n_sk + 1, &route_tmp)) ||
-        (rc = ensure_t(c, "scan_tmp2", tg::scan_tmp_words(o.nsp + 1) + 64, &tmp)))
-        return rc;
-    tg::RouteArgs r{};
-    r.run_ptr = run_ptr;
-    r.run_cnt = run_cnt;
-    r.run_off = run_off;
-    r.n_runs = (uint32_t)n_runs;
-    r.n_rec = o.n_sk;
-    r.tile = (uint32_t)tile;
-    r.nsp = o.nsp;
-    r.sb = o.sb;
-    r.k = o.k;
-    r.hsk = hsk;
-    r.hkm = hkm;
-    r.part_off = o.part_off;
-    r.dst = o.sorted;
-    {
-        Prof pr(c, "route_hist");
-        tg::launch_route_hist(st, r, o.ntiles);
-    }
-    {
-        Prof pr(c, "route_scan");
-        if (o.ntiles) tg::launch_route_colscan(st, r, o.ntiles, part_sk, o.part_km);
-        else {
-            tg::launch_fill_u64(st, part_sk, 0, o.nsp);
-            tg::launch_fill_u64(st, o.part_km, 0, o.nsp);
+        uint32_t nflag = 0;
+        HIPOK(hipMemcpyAsync(&nflag, flags, 4, hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        o.n_flagged = nflag;
+        const uint64_t fb = (nflag + tg::SK_BLOCK - 1) / tg::SK_BLOCK;
+        if (fb > fixb) {
+            // more flagged strips than fix-up regions: size them and redo
+            fixb = fb;
+            continue;
         }
-        tg::launch_scan_u64(st, part_sk, o.nsp, o.part_off, tmp);
+        if (nflag) {
+            // strips whose first superkmer began before the look-back: redo
+            // them from the chunk start (long low-complexity runs)
+            a.list = flag_list;
+            a.n = nflag;
+            a.lookback = tg::LOOKBACK_FULL;
+            a.run_base = (uint32_t)n_blocks;
+            Prof pr(c, "superkmers_fixup");
+            tg::launch_superkmers(st, a, o.rw);
+        }
+        HIPOK(hipGetLastError());
+        const uint32_t n_runs = (uint32_t)(n_blocks + fb);
+        uint64_t* run_cnt64;
+        if ((rc = ensure_t(c, "run_cnt64", n_runs + 1, &run_cnt64)) ||
+            (rc = ensure_t(c, "scan_tmp3", tg::scan_tmp_words(n_runs + 1) + 64, &tmp)))
+            return rc;
+        {
+            Prof pr(c, "route_scan");
+            tg::launch_u32_to_u64(st, run_cnt, n_runs, run_cnt64);
+            tg::launch_scan_u64(st, run_cnt64, n_runs, run_off, tmp);
+        }
+        HIPOK(hipMemcpyAsync(&o.n_rec, run_off + n_runs, 8, hipMemcpyDeviceToHost, st));
+        HIPOK(hipMemcpyAsync(&o.n_sk, sk_count, 8, hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        // -- routing: tiles over the concatenated run space
+        uint64_t tile = 16384;
+        while ((o.n_rec + tile - 1) / tile > tg::ROUTE_MAX_TILES) tile <<= 1;
+        o.ntiles = (uint32_t)((o.n_rec + tile - 1) / tile);
+        // sub-partitions: ~16K k-mers each (N/16384), at least the reference n,
+        // at least 2, at most 2^14 (LDS histograms)
+        {
+            uint64_t want = 2;
+            while (want < o.n_kmers / 16384 && want < (1u << 14)) want <<= 1;
+            if (want < o.n) want = o.n;
+            o.nsp = (uint32_t)want;
+            o.log2s = ilog2(o.nsp) - o.log2n;
+            o.sb = tg::HSEL_BITS - ilog2(o.nsp);
+        }
+        uint32_t *hsk, *hkm, *route_tmp;
+        uint64_t* part_sk;
+        const uint64_t msz = (uint64_t)std::max<uint32_t>(o.ntiles, 1) * o.nsp;
+        if ((rc = ensure_t(c, "hsk", msz, &hsk)) || (rc = ensure_t(c, "hkm", msz, &hkm)) ||
+            (rc = ensure_t(c, "part_sk", o.nsp + 1, &part_sk)) ||
+            (rc = ensure_t(c, "part_km", o.nsp + 1, &o.part_km)) ||
+            (rc = ensure_t(c, "part_off", o.nsp + 1, &o.part_off)) ||
+            (rc = ensure_t(c, "sorted", (o.n_rec + 1) * o.rw, &o.sorted)) ||
+            (rc = ensure_t(c, "route_tmp", (o.n_rec + 1) * o.rw, &route_tmp)) ||
+            (rc = ensure_t(c, "scan_tmp2", tg::scan_tmp_words(o.nsp + 1) + 64, &tmp)))
+            return rc;
+        tg::RouteArgs r{};
+        r.recs = recs;
+        r.run_start = run_start;
+        r.run_cnt = run_cnt;
+        r.run_off = run_off;
+        r.n_runs = n_runs;
+        r.n_rec = o.n_rec;
+        r.tile = (uint32_t)tile;
+        r.nsp = o.nsp;
+        r.sb = o.sb;
+        r.k = o.k;
+        r.hsk = hsk;
+        r.hkm = hkm;
+        r.part_off = o.part_off;
+        r.dst = o.sorted;
+        {
+            Prof pr(c, "route_hist");
+            tg::launch_route_hist(st, r, o.ntiles, o.rw);
+        }
+        {
+            Prof pr(c, "route_scan");
+            if (o.ntiles) tg::launch_route_colscan(st, r, o.ntiles, part_sk, o.part_km);
+            else {
+                tg::launch_fill_u64(st, part_sk, 0, o.nsp);
+                tg::launch_fill_u64(st, o.part_km, 0, o.nsp);
+            }
+            tg::launch_scan_u64(st, part_sk, o.nsp, o.part_off, tmp);
+        }
+        {
+            Prof pr(c, "route_scatter");
+            tg::launch_route_scatter(st, r, o.ntiles, route_tmp, o.rw);
+        }
+        HIPOK(hipGetLastError());
+        return TG_OK;
     }
-    {
-        Prof pr(c, "route_scatter");
-        tg::launch_route_scatter(st, r, o.ntiles, route_tmp);
-    }
-    HIPOK(hipGetLastError());
-    return TG_OK;
 }
 
 // Work items over partitions [p_begin, p_end): consecutive sub-partitions
@@ -313,7 +326,6 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
             (rc = ensure_t(c, "sp_cnt", sp_slots, &sp_cnt)))
             return rc;
         tg::CountArgs a{};
-        a.packed = o.packed;
         a.rec = o.sorted;
         a.k = p->k;
         a.ci = p->ci;
@@ -559,7 +571,15 @@ int tg_count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint6
                     const uint64_t* d_off, uint64_t n_chunks, tg_stats* st)
 {
     if (!c) return fail(TG_EINVAL, "null context");
-    return count_device(c, p, d_bases, n_bases, d_off, n_chunks, st);
+    if (c->plain) return count_device(c, p, d_bases, n_bases, d_off, n_chunks, st);
+    return count_one_batched(c, p, d_bases, n_bases, d_off, n_chunks, st);
+}
+
+int tg_set_plain_sequence(tg_ctx* c, int on)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    c->plain = on != 0;
+    return TG_OK;
 }
 
 int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
@@ -572,7 +592,8 @@ int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t 
     uint8_t* d_b;
     uint64_t* d_o;
     if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
-    return count_device(c, p, d_b, n_bases, d_o, n_chunks, st);
+    if (c->plain) return count_device(c, p, d_b, n_bases, d_o, n_chunks, st);
+    return count_one_batched(c, p, d_b, n_bases, d_o, n_chunks, st);
 }
 
 uint64_t tg_result_size(tg_ctx* c) { return c ? c->n_out : 0; }
@@ -618,7 +639,7 @@ int tg_superkmers_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     uint64_t* d_o;
     if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
     Phase1 o;
-    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o))) return rc;
+    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
     collect_prof(c);
     if (n_sk) *n_sk = o.n_sk;
     if (n_subparts) *n_subparts = o.nsp;
@@ -650,7 +671,7 @@ int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     uint64_t* d_o;
     if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
     Phase1 o;
-    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o))) return rc;
+    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
     collect_prof(c);
     std::vector<uint2> recs(o.n_sk);
     std::vector<uint64_t> poff(o.nsp + 1);

@@ -63,6 +63,9 @@ struct KTime {
 
 struct BatchState;
 void destroy_batch(BatchState* b);
+// One job through the batched engine; fills st and the context's result.
+int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                      const uint64_t* d_off, uint64_t n_chunks, tg_stats* st);
 
 }  // namespace tgi
 
@@ -82,6 +85,9 @@ struct tg_ctx {
     uint32_t* d_cnt = nullptr;
     // batched jobs (count_many): buffers, captured graph and results
     tgi::BatchState* batch = nullptr;
+    // single jobs: false = through the batched engine (one captured launch
+    // sequence, no host round trip); true = the plain host-driven sequence
+    bool plain = false;
 };
 
 namespace tgi {
@@ -139,14 +145,14 @@ struct Prof {
     hipEvent_t a = nullptr;
     Prof(tg_ctx* cc, const char* n) : c(cc), name(n)
     {
-        if (c->profiling) {
+        if (c && c->profiling) {
             a = take_event(c);
             hipEventRecord(a, c->stream);
         }
     }
     ~Prof()
     {
-        if (c->profiling) {
+        if (c && c->profiling) {
             hipEvent_t b = take_event(c);
             hipEventRecord(b, c->stream);
             c->pending.push_back({name, {a, b}});

@@ -251,15 +251,16 @@ struct SkLane {
     // packed-stream reader + rolling hash (include/nt_hash.hpp:114-118)
     const uint32_t* packed;
     uint64_t widx;
-    uint32_t wrd, rem;
+    uint32_t wrd, rem, nextw;   // nextw: the word after wrd, loaded one word ahead
     uint64_t f, rv, lmer;
     uint32_t osh;
     // cut state
     uint32_t ts, tb, te, K;
-    uint32_t t0, P0;
-    uint64_t H0, prevH;
+    uint32_t t0, t0p, P0;       // superkmer start, current piece start, minimizer position
+    uint64_t prevH;
     uint32_t hsel;
     uint32_t rbase, rl;         // bucket base and bucket bits of the strip's job
+    uint32_t job, nsk;          // job of the strip, superkmers emitted
     bool have, done, flagged, tail;
     uint64_t cb;
 };
@@ -267,7 +268,8 @@ struct SkLane {
 __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uint64_t* tr)
 {
     if (L.rem == 0) {
-        L.wrd = L.packed[++L.widx];
+        L.wrd = L.nextw;
+        L.nextw = L.packed[++L.widx + 1];  // the stream keeps 8 spare words
         L.rem = 16;
     }
     const uint32_t kb = L.wrd >> 30;
@@ -298,30 +300,117 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Per-wave staging of records: the active lanes append with a ballot, and
-// when the buffer nears full they copy it to the workgroup's run (reserved by
-// one LDS atomic), so global writes are contiguous.
+// A record of RW words, moved with vector loads and stores.
+template <int RW>
+struct Rec {
+    uint32_t w[RW];
+};
+
+template <int RW>
+__device__ __forceinline__ Rec<RW> load_rec(const uint32_t* base, uint64_t i)
+{
+    Rec<RW> r;
+    if (RW == 2) {
+        const uint2 v = reinterpret_cast<const uint2*>(base)[i];
+        r.w[0] = v.x;
+        r.w[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < RW / 4; ++q) {
+            const uint4 v = reinterpret_cast<const uint4*>(base)[i * (RW / 4) + q];
+            r.w[4 * q] = v.x;
+            r.w[4 * q + 1] = v.y;
+            r.w[4 * q + 2] = v.z;
+            r.w[4 * q + 3] = v.w;
+        }
+    }
+    return r;
+}
+
+template <int RW>
+__device__ __forceinline__ void store_rec(uint32_t* base, uint64_t i, const Rec<RW>& r)
+{
+    if (RW == 2) {
+        reinterpret_cast<uint2*>(base)[i] = make_uint2(r.w[0], r.w[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < RW / 4; ++q)
+            reinterpret_cast<uint4*>(base)[i * (RW / 4) + q] =
+                make_uint4(r.w[4 * q], r.w[4 * q + 1], r.w[4 * q + 2], r.w[4 * q + 3]);
+    }
+}
+
+template <int RW>
+__device__ __forceinline__ uint32_t rec_meta(const Rec<RW>& r) { return RW == 2 ? r.w[1] : r.w[0]; }
+
+template <int RW>
+__device__ __forceinline__ uint32_t rec_len(uint32_t meta)
+{
+    return RW == 2 ? (meta & REC_LEN_MASK) : (meta & REC_FAT_LEN_MASK);
+}
+
+// Fat record of the piece starting at global base `start` with meta `meta`:
+// its bases realigned so the first is in bits 31:30 of word 1, bases past the
+// piece cleared (the packed stream keeps 8 spare words, so the reads stay in
+// bounds).
+template <int RW>
+__device__ __forceinline__ Rec<RW> fat_record(const uint32_t* packed, uint32_t start, uint32_t meta)
+{
+    Rec<RW> r;
+    const uint32_t len = meta & REC_FAT_LEN_MASK;
+    const uint32_t* src = packed + (start >> 4);
+    const uint32_t sh = 2u * (start & 15u);
+    uint32_t in[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) in[q] = src[q];
+    r.w[0] = meta;
+#pragma unroll
+    for (int q = 0; q < RW - 1; ++q) {
+        uint32_t v = sh ? (in[q] << sh) | (in[q + 1] >> (32u - sh)) : in[q];
+        const int keep = (int)len - 16 * q;
+        v = keep <= 0 ? 0u : (keep >= 16 ? v : v & ~(0xFFFFFFFFu >> (2 * keep)));
+        r.w[q + 1] = v;
+    }
+    return r;
+}
+
+// Per-wave staging of records: the active lanes append (start, meta) with a
+// ballot, and when the buffer nears full they write it to the workgroup's run
+// (space taken with one LDS atomic) with every lane active: that is where a
+// fat record gathers its bases, outside the divergent per-k-mer loop.
 struct SkStage {
     uint2* buf;        // this wave's buffer (LDS)
     uint32_t* wn;      // this wave's fill count (LDS)
     uint32_t* wg_n;    // workgroup run fill (LDS)
-    uint2* run;        // workgroup run (global)
+    uint64_t run;      // workgroup run: first record index
 };
 
-__device__ __forceinline__ void sk_flush(const SkStage& S, uint32_t cnt)
+template <int RW>
+__device__ __forceinline__ void sk_flush(const SkStage& S, const SkArgs& a, uint32_t cnt)
 {
     const uint64_t act = __ballot(1);
     const uint32_t nact = (uint32_t)__popcll(act), rank = lane_rank(act);
     uint32_t base = 0;
     if (rank == 0) base = atomicAdd(S.wg_n, cnt);
     base = __builtin_amdgcn_readfirstlane(base);
-    for (uint32_t i = rank; i < cnt; i += nact) S.run[base + i] = S.buf[i];
+    for (uint32_t i = rank; i < cnt; i += nact) {
+        const uint2 e = S.buf[i];
+        if (RW == 2) {
+            Rec<RW> r;
+            r.w[0] = e.x;
+            r.w[1] = e.y;
+            store_rec<RW>(a.recs, S.run + base + i, r);
+        } else {
+            store_rec<RW>(a.recs, S.run + base + i, fat_record<RW>(a.packed, e.x, e.y));
+        }
+    }
     __builtin_amdgcn_wave_barrier();
     *S.wn = 0;
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void sk_push(const SkStage& S, bool e, uint2 rec)
+template <int RW>
+__device__ __forceinline__ void sk_push(const SkStage& S, const SkArgs& a, bool e, uint2 rec)
 {
     const uint64_t mask = __ballot(e);
     if (!mask) return;
@@ -333,44 +422,63 @@ __device__ __forceinline__ void sk_push(const SkStage& S, bool e, uint2 rec)
     __builtin_amdgcn_wave_barrier();
     *S.wn = cnt;
     __builtin_amdgcn_wave_barrier();
-    if (cnt > SK_WAVE_BUF - 64u) sk_flush(S, cnt);
+    if (cnt > SK_WAVE_BUF - 64u) sk_flush<RW>(S, a, cnt);
+}
+
+// Staged entry of the piece [p0, t_end) of the current superkmer: global start
+// and meta.  Slim records keep the superkmer's minpos; a fat piece after the
+// first sets REC_CONT instead.
+template <int RW>
+__device__ __forceinline__ uint2 sk_entry(const SkLane& L, const SkArgs& a, uint32_t p0, uint32_t t_end)
+{
+    const uint32_t len = t_end - p0 + a.k - 1u;
+    const bool first = p0 == L.t0;
+    const uint32_t meta = (RW != 2 && !first) ? (len | REC_CONT | (L.hsel << REC_HSEL_SHIFT))
+                                              : (len | ((L.P0 - L.t0) << REC_MINPOS_SHIFT) |
+                                                 (L.hsel << REC_HSEL_SHIFT));
+    return make_uint2((uint32_t)L.cb + p0, meta);
 }
 
 // Decision for k-mer t with window minimum (Hc, Pc), written without branches
-// so the active lanes stay converged for the one ballot in sk_push.
+// so the active lanes stay converged for the one ballot in sk_push.  With fat
+// records a piece ends after a.piece k-mers (a cut that starts no new
+// superkmer).
+template <int RW>
 __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, const SkStage& S, uint32_t t,
                                         uint64_t Hc, uint32_t Pc)
 {
     const bool first = t == L.ts;
     const bool cut = !first && ((Hc != L.prevH) || (L.have && (t - L.t0 >= a.capk)));
-    const bool emit = cut && L.have && L.t0 >= L.tb;
+    const bool pcut = RW != 2 && !first && !cut && L.have && (t - L.t0p >= a.piece);
+    const bool emit = (cut || pcut) && L.have && L.t0 >= L.tb;
     const bool flag = !first && !cut && t == L.tb && !L.have;
-    const uint2 rec = make_uint2((uint32_t)L.cb + L.t0,
-                                 (t - L.t0 + a.k - 1u) | ((L.P0 - L.t0) << REC_MINPOS_SHIFT) |
-                                     (L.hsel << REC_HSEL_SHIFT));
+    const uint2 rec = sk_entry<RW>(L, a, L.t0p, t);
+    L.nsk += (emit && L.t0p == L.t0) ? 1u : 0u;
     const bool stop = (cut && t >= L.te) || flag;
     const bool start = (first && L.ts == 0) || (cut && t < L.te);
     L.t0 = start ? t : L.t0;
-    L.H0 = start ? Hc : L.H0;
+    L.t0p = (start || pcut) ? t : L.t0p;
     L.P0 = start ? Pc : L.P0;
     L.hsel = start ? route_hsel(Hc, L.rbase, L.rl, a.xb) : L.hsel;
     L.have = L.have || start;
     L.prevH = Hc;
     const bool last = !stop && t == L.K - 1u;
-    L.tail = last && L.have && L.t0 >= L.tb;   // [t0, K) is emitted after the loop
+    L.tail = last && L.have && L.t0 >= L.tb;   // [t0p, K) is emitted after the loop
     L.flagged = L.flagged || flag;
     L.done = stop || last;
-    sk_push(S, emit, rec);
+    sk_push<RW>(S, a, emit, rec);
 }
 
 __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, const uint64_t* s_init)
 {
     const uint32_t c = a.strip_chunk[s];
     if (a.job_route) {
-        const uint32_t r = a.job_route[a.chunk_job[c]];
+        L.job = a.chunk_job[c];
+        const uint32_t r = a.job_route[L.job];
         L.rbase = r & 0xFFFFu;
         L.rl = r >> 16;
     } else {
+        L.job = 0;
         L.rbase = 0;
         L.rl = a.log2n;
     }
@@ -385,13 +493,15 @@ __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, 
     const uint64_t gq = L.cb + L.ts;
     L.widx = gq >> 4;
     L.wrd = a.packed[L.widx] << (2u * (uint32_t)(gq & 15u));
+    L.nextw = a.packed[L.widx + 1];
     L.rem = 16u - (uint32_t)(gq & 15u);
     L.f = s_init[0];
     L.rv = s_init[1];
     L.lmer = 0;
     L.osh = 2u * (a.m - 1u);
-    L.t0 = 0; L.P0 = 0; L.H0 = 0; L.prevH = 0;
+    L.t0 = 0; L.t0p = 0; L.P0 = 0; L.prevH = 0;
     L.hsel = 0;
+    L.nsk = 0;
     L.have = false; L.done = false; L.flagged = false; L.tail = false;
 }
 
@@ -441,7 +551,7 @@ __device__ __forceinline__ void sk_backward(uint64_t (&sv)[WMAX], uint8_t (&sp)[
     }
 }
 
-template <int WMAX>
+template <int WMAX, int RW>
 __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
 {
     __shared__ uint64_t tf[16], tr[16], s_init[2];
@@ -471,10 +581,11 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
     // main pass: the run is the slot region of this block's strips; fix-up:
     // a private region of SK_BLOCK*S records per block
     const uint32_t first = blockIdx.x * SK_BLOCK;
-    S.run = a.list ? a.rec + (uint64_t)blockIdx.x * SK_BLOCK * a.S
-                   : (first < n ? a.rec + a.strip_slot[first] : a.rec);
+    S.run = a.list ? (uint64_t)a.fix_base + (uint64_t)blockIdx.x * SK_BLOCK * a.S
+                   : (first < n ? a.strip_slot[first] : 0ull);
     bool tail = false;
     uint2 tail_rec = make_uint2(0u, 0u);
+    uint32_t nsk = 0, job = 0;
     if (idx < n) {
         const uint32_t s = a.list ? a.list[idx] : idx;
         SkLane L;
@@ -511,7 +622,7 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
                         Hc = pv;
                         Pc = bs + W + pp;
                     }
-                    sk_kmer(L, a, S, t, Hc, Pc);
+                    sk_kmer<RW>(L, a, S, t, Hc, Pc);
                     if (!L.done) {
                         sk_roll(L, tf, tr);
                         const uint64_t h = L.f ^ L.rv;
@@ -533,29 +644,44 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
             a.flag_list[slot] = s;
         }
         tail = L.tail;
-        tail_rec = make_uint2((uint32_t)L.cb + L.t0, (L.K - L.t0 + a.k - 1u) |
-                                                         ((L.P0 - L.t0) << REC_MINPOS_SHIFT) |
-                                                         (L.hsel << REC_HSEL_SHIFT));
+        tail_rec = sk_entry<RW>(L, a, L.t0p, L.K);
+        nsk = L.nsk + ((tail && L.t0p == L.t0) ? 1u : 0u);
+        job = L.job;
     }
     // every lane of the wave is here again: the chunk-end records, then flush
-    sk_push(S, tail, tail_rec);
+    sk_push<RW>(S, a, tail, tail_rec);
     const uint32_t left = s_wn[wv];
-    if (left) sk_flush(S, left);
+    if (left) sk_flush<RW>(S, a, left);
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.run_ptr[a.run_base + blockIdx.x] = S.run;
+        a.run_start[a.run_base + blockIdx.x] = (uint32_t)S.run;
         a.run_cnt[a.run_base + blockIdx.x] = s_wgn;
+    }
+    // superkmers per job: one atomic per wave when the wave's strips share a job
+    const uint32_t j0 = __builtin_amdgcn_readfirstlane(job);
+    if (__ballot(job != j0 && nsk) == 0) {
+        uint32_t sum = nsk;
+#pragma unroll
+        for (uint32_t d = 32; d; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if ((threadIdx.x & 63u) == 0 && sum) atomicAdd(&a.sk_count[j0], (unsigned long long)sum);
+    } else if (nsk) {
+        atomicAdd(&a.sk_count[job], (unsigned long long)nsk);
     }
 }
 
-void launch_superkmers(hipStream_t st, const SkArgs& a)
+void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
 {
     if (a.n == 0) return;
     const unsigned blocks = (a.n + SK_BLOCK - 1) / SK_BLOCK;
-    if (a.W <= 8) k_superkmers<8><<<blocks, SK_BLOCK, 0, st>>>(a);
-    else if (a.W <= 16) k_superkmers<16><<<blocks, SK_BLOCK, 0, st>>>(a);
-    else if (a.W <= 32) k_superkmers<32><<<blocks, SK_BLOCK, 0, st>>>(a);
-    else k_superkmers<64><<<blocks, SK_BLOCK, 0, st>>>(a);
+#define TG_SK(WM, RWV) k_superkmers<WM, RWV><<<blocks, SK_BLOCK, 0, st>>>(a)
+#define TG_SK_RW(WM) \
+    do { if (rw == 2) TG_SK(WM, 2); else if (rw == 4) TG_SK(WM, 4); else TG_SK(WM, 8); } while (0)
+    if (a.W <= 8) TG_SK_RW(8);
+    else if (a.W <= 16) TG_SK_RW(16);
+    else if (a.W <= 32) TG_SK_RW(32);
+    else TG_SK_RW(64);
+#undef TG_SK_RW
+#undef TG_SK
 }
 
 // =========================================================== routing ====
@@ -579,6 +705,11 @@ __device__ __forceinline__ RouteGeom route_geom(const RouteArgs& a)
     RouteGeom g;
     if (a.n_rec_dev) {
         g.n_rec = *a.n_rec_dev;
+        if (g.n_rec > a.rec_cap) {
+            // more records than the partition buffers hold: route nothing
+            if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.err, 2u);
+            g.n_rec = 0;
+        }
         uint64_t t = 16384;
         while ((g.n_rec + t - 1) / t > a.max_tiles) t <<= 1;
         g.tile = (uint32_t)t;
@@ -592,11 +723,12 @@ __device__ __forceinline__ RouteGeom route_geom(const RouteArgs& a)
 }
 
 // Calls f(record) for every record of this block's tile.  The tile's run
-// descriptors are cached in LDS; each wave walks 64*U consecutive records at
-// a time and every lane finds its run by a short forward scan.
-template <class F>
+// descriptors (record index of the run's first record minus its offset in the
+// run space) are cached in LDS; each wave walks 64*U consecutive records at a
+// time and every lane finds its run by a short forward scan.
+template <int RW, class F>
 __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const RouteGeom& g,
-                                                 uint64_t* s_off, uint2** s_ptr,
+                                                 uint64_t* s_off, int64_t* s_base,
                                                  uint32_t* s_meta, F&& f)
 {
     const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
@@ -618,23 +750,24 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
     const uint32_t r0 = s_meta[0], nr = s_meta[1];
     for (uint32_t i = threadIdx.x; i <= nr; i += blockDim.x) {
         s_off[i] = a.run_off[r0 + i];
-        if (i < nr) s_ptr[i] = a.run_ptr[r0 + i] - a.run_off[r0 + i];
+        if (i < nr) s_base[i] = (int64_t)a.run_start[r0 + i] - (int64_t)a.run_off[r0 + i];
     }
     __syncthreads();
     if (s_meta[2]) {
         // many tiny runs: walk them one by one
         for (uint32_t r = r0; r < a.n_runs && a.run_off[r] < hi; ++r) {
             const uint64_t b = max(a.run_off[r], lo), e = min(a.run_off[r + 1], hi);
-            const uint2* p = a.run_ptr[r] - a.run_off[r];
-            for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) f(p[i]);
+            const int64_t base = (int64_t)a.run_start[r] - (int64_t)a.run_off[r];
+            for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x)
+                f(load_rec<RW>(a.recs, (uint64_t)(base + (int64_t)i)));
         }
         return;
     }
-    constexpr uint32_t U = 8;
+    constexpr uint32_t U = RW == 8 ? 4 : 8;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     uint32_t rr = 0;  // run of this lane's last record (non-decreasing)
     for (uint64_t base = lo + (uint64_t)wv * 64u * U; base < hi; base += (uint64_t)nwv * 64u * U) {
-        uint2 v[U];
+        Rec<RW> v[U];
         bool ok[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
@@ -642,7 +775,7 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
             ok[u] = i < hi;
             if (ok[u]) {
                 while (s_off[rr + 1] <= i) ++rr;
-                v[u] = s_ptr[rr][i];
+                v[u] = load_rec<RW>(a.recs, (uint64_t)(s_base[rr] + (int64_t)i));
             }
         }
 #pragma unroll
@@ -651,11 +784,12 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
     }
 }
 
+template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
 {
     extern __shared__ uint32_t sh[];  // [0, nsp): superkmers, [nsp, 2nsp): k-mers
     __shared__ uint64_t s_off[ROUTE_RUNS + 1];
-    __shared__ uint2* s_ptr[ROUTE_RUNS];
+    __shared__ int64_t s_base[ROUTE_RUNS];
     __shared__ uint32_t s_meta[4];
     const RouteGeom g = route_geom(a);
     if (blockIdx.x >= g.ntiles) return;
@@ -664,10 +798,11 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t k1 = a.k - 1u;
-    for_tile_records(a, g, s_off, s_ptr, s_meta, [&](uint2 v) {
-        const uint32_t pid = v.y >> shift;
+    for_tile_records<RW>(a, g, s_off, s_base, s_meta, [&](const Rec<RW>& v) {
+        const uint32_t meta = rec_meta<RW>(v);
+        const uint32_t pid = meta >> shift;
         atomicAdd(&sh[pid], 1u);
-        atomicAdd(&sh[nsp + pid], (v.y & REC_LEN_MASK) - k1);
+        atomicAdd(&sh[nsp + pid], rec_len<RW>(meta) - k1);
     });
     __syncthreads();
     uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * nsp;
@@ -678,29 +813,42 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
     }
 }
 
-__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint2* tmp);
-__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint2* tmp);
+template <int RW>
+__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint32_t* tmp);
+template <int RW>
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp);
 
 // Dynamic-LDS limits of the routing kernels, set once per process at their
 // largest sizes (2^14 sub-partitions), so no attribute call happens while a
 // launch sequence is being captured.
+template <int RW>
+void set_route_attributes()
+{
+    const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
+    const int cur = (int)((1u << 14) * sizeof(uint32_t));
+    (void)hipFuncSetAttribute((const void*)k_route_hist<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
+    (void)hipFuncSetAttribute((const void*)k_route_coarse<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+    (void)hipFuncSetAttribute((const void*)k_route_fine<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+}
+
 void set_kernel_attributes()
 {
     static std::once_flag once;
     std::call_once(once, [] {
-    const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
-    const int cur = (int)((1u << 14) * sizeof(uint32_t));
-    (void)hipFuncSetAttribute((const void*)k_route_hist, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
-    (void)hipFuncSetAttribute((const void*)k_route_coarse, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
-    (void)hipFuncSetAttribute((const void*)k_route_fine, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
+        set_route_attributes<2>();
+        set_route_attributes<4>();
+        set_route_attributes<8>();
     });
 }
 
-void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles)
+void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t rw)
 {
     if (ntiles == 0) return;
     set_kernel_attributes();
-    k_route_hist<<<ntiles, ROUTE_T, 2u * a.nsp * sizeof(uint32_t), st>>>(a);
+    const size_t lds = 2u * a.nsp * sizeof(uint32_t);
+    if (rw == 2) k_route_hist<2><<<ntiles, ROUTE_T, lds, st>>>(a);
+    else if (rw == 4) k_route_hist<4><<<ntiles, ROUTE_T, lds, st>>>(a);
+    else k_route_hist<8><<<ntiles, ROUTE_T, lds, st>>>(a);
 }
 
 // Per sub-partition: tile-exclusive offsets (in place) and totals.  Eight
@@ -750,11 +898,12 @@ void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
 // Scatter, level 1: every tile writes its records into coarse buckets of F
 // consecutive sub-partitions.  A tile's runs are long (tile/nc records) and
 // the lines being filled at once fit in L2, so partial lines merge there.
-__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint2* tmp)
+template <int RW>
+__global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint32_t* tmp)
 {
     extern __shared__ uint32_t cur[];  // nc cursors
     __shared__ uint64_t s_off[ROUTE_RUNS + 1];
-    __shared__ uint2* s_ptr[ROUTE_RUNS];
+    __shared__ int64_t s_base[ROUTE_RUNS];
     __shared__ uint32_t s_meta[4];
     const RouteGeom g = route_geom(a);
     if (blockIdx.x >= g.ntiles) return;
@@ -767,15 +916,16 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     }
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb + log2f;
-    for_tile_records(a, g, s_off, s_ptr, s_meta, [&](uint2 v) {
-        const uint32_t pos = atomicAdd(&cur[v.y >> shift], 1u);
-        tmp[pos] = v;
+    for_tile_records<RW>(a, g, s_off, s_base, s_meta, [&](const Rec<RW>& v) {
+        const uint32_t pos = atomicAdd(&cur[rec_meta<RW>(v) >> shift], 1u);
+        store_rec<RW>(tmp, pos, v);
     });
 }
 
 // Scatter, level 2: one workgroup per coarse bucket reads its region
 // (contiguous) and places records by sub-partition inside the same region.
-__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint2* tmp)
+template <int RW>
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp)
 {
     extern __shared__ uint32_t cur[];  // F cursors
     const uint32_t F = 1u << log2f, c = blockIdx.x;
@@ -784,26 +934,26 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
     const uint64_t b = a.part_off[c * F], e = a.part_off[(c + 1) * F];
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t fm = F - 1u;
-    constexpr uint32_t U = 8;
+    constexpr uint32_t U = RW == 8 ? 4 : 8;
     uint64_t i = b + threadIdx.x;
     for (; i + (U - 1) * (uint64_t)blockDim.x < e; i += U * (uint64_t)blockDim.x) {
-        uint2 v[U];
+        Rec<RW> v[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) v[u] = tmp[i + u * blockDim.x];
+        for (uint32_t u = 0; u < U; ++u) v[u] = load_rec<RW>(tmp, i + u * blockDim.x);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t pos = atomicAdd(&cur[(v[u].y >> shift) & fm], 1u);
-            a.dst[pos] = v[u];
+            const uint32_t pos = atomicAdd(&cur[(rec_meta<RW>(v[u]) >> shift) & fm], 1u);
+            store_rec<RW>(a.dst, pos, v[u]);
         }
     }
     for (; i < e; i += blockDim.x) {
-        const uint2 v = tmp[i];
-        const uint32_t pos = atomicAdd(&cur[(v.y >> shift) & fm], 1u);
-        a.dst[pos] = v;
+        const Rec<RW> v = load_rec<RW>(tmp, i);
+        const uint32_t pos = atomicAdd(&cur[(rec_meta<RW>(v) >> shift) & fm], 1u);
+        store_rec<RW>(a.dst, pos, v);
     }
 }
 
-void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint2* tmp)
+void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t* tmp, uint32_t rw)
 {
     if (ntiles == 0) return;
     // coarse buckets: up to 256, F = nsp / nc sub-partitions each
@@ -811,8 +961,17 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     while ((a.nsp >> log2f) > 256u) ++log2f;
     const uint32_t nc = a.nsp >> log2f;
     set_kernel_attributes();
-    k_route_coarse<<<ntiles, ROUTE_T, nc * sizeof(uint32_t), st>>>(a, log2f, tmp);
-    k_route_fine<<<nc, ROUTE_T, (1u << log2f) * sizeof(uint32_t), st>>>(a, log2f, tmp);
+    const size_t lc = nc * sizeof(uint32_t), lf = (1u << log2f) * sizeof(uint32_t);
+    if (rw == 2) {
+        k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+        k_route_fine<2><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
+    } else if (rw == 4) {
+        k_route_coarse<4><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+        k_route_fine<4><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
+    } else {
+        k_route_coarse<8><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+        k_route_fine<8><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
+    }
 }
 
 // ============================================================== fill ====
@@ -1056,8 +1215,8 @@ __device__ __forceinline__ void emit_out(const CountArgs& a, uint64_t pos, uint6
 template <int NW>
 struct KRoll {
     uint64_t fh, fl, rh, rl;   // forward / reverse complement (hi used when NW == 2)
-    uint32_t wi;               // next word (stage index)
-    uint32_t wrd, rem;
+    uint32_t wi;               // stage index of the word being read
+    uint32_t wrd, rem, nw;     // bits left of that word, their count, the next word
 };
 
 // 2k-bit k-mer whose first base is at bit offset 2*sh of stage word wi.
@@ -1092,18 +1251,18 @@ __device__ __forceinline__ void kroll_init(KRoll<NW>& R, const uint32_t* src, ui
     const uint32_t q = sh + k;  // next base, relative to word wi
     R.wi = wi + (q >> 4);
     R.wrd = src[R.wi] << (2u * (q & 15u));
+    R.nw = src[R.wi + 1];
     R.rem = 16u - (q & 15u);
 }
 
 template <int NW>
 __device__ __forceinline__ void kroll_step(KRoll<NW>& R, const uint32_t* src, uint32_t k)
 {
-    {
-        const uint32_t nxt = src[R.wi + 1u];
-        const bool ld = R.rem == 0;
-        R.wrd = ld ? nxt : R.wrd;
-        R.wi += ld ? 1u : 0u;
-        R.rem = ld ? 16u : R.rem;
+    if (R.rem == 0) {
+        R.wrd = R.nw;
+        ++R.wi;
+        R.nw = src[R.wi + 1u];
+        R.rem = 16u;
     }
     const uint64_t b = R.wrd >> 30;
     R.wrd <<= 2;
@@ -1133,14 +1292,14 @@ __device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
     return v;
 }
 
-constexpr uint32_t COUNT_STAGE = 512;  // packed words staged per wave
+constexpr uint32_t COUNT_STAGE = 512;  // packed words staged per wave (64 records x <= 7, + 1)
 
-// One workgroup per work item.  A wave takes 64 records at a time from a
-// workgroup dispenser, copies the packed words they cover into its LDS stage
-// (one round of global loads), and then their k-mers form a flat index space
-// (wave scan) cut into 64 equal pieces, one per lane.  A lane finds its first
-// record once (binary search) and rolls through consecutive k-mers from LDS,
-// re-seeding only when it crosses into the next record.
+// One workgroup per work item.  A wave takes 64 fat records at a time from a
+// workgroup dispenser (one coalesced vector load per lane), puts their bases
+// into its LDS stage at a fixed stride, and then their k-mers form a flat index
+// space (wave scan) cut into 64 equal pieces, one per lane.  A lane finds its
+// first record once (binary search) and rolls through consecutive k-mers from
+// LDS, re-seeding only when it crosses into the next record.
 //
 // BATCH: a persistent launch; workgroups take items from a device queue until
 // the device-side item count is reached, spill keys carry the item's job, and
@@ -1149,13 +1308,11 @@ template <int NW, uint32_t CAP, bool BATCH>
 __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 {
     constexpr uint32_t NWAVE = COUNT_BLOCK / 64;
-    __shared__ uint64_t s_k0[CAP];
+    __shared__ __align__(16) uint64_t s_k0[CAP];
     __shared__ uint64_t s_k1[NW == 2 ? CAP : 1];
     __shared__ uint32_t s_cnt[CAP];
     __shared__ uint32_t s_stage[NWAVE][COUNT_STAGE];
     __shared__ uint32_t s_incl[NWAVE][64];
-    __shared__ uint32_t s_wb[NWAVE][64];   // stage word of the record's first base word
-    __shared__ uint32_t s_st[NWAVE][64];   // first base of the record (global index)
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_next;
@@ -1164,7 +1321,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t k = a.k;
-    const uint32_t pad = NW == 1 ? 3u : 5u;  // words read past a record's last base word
+    constexpr uint32_t RW = NW == 1 ? 4u : 8u, PW = RW - 1u;  // record words, base words
     const uint32_t* stage = s_stage[wv];
     uint32_t item = blockIdx.x;
     uint32_t n_items = a.n_items;
@@ -1186,124 +1343,110 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 
         const uint32_t r1 = it.y, smask = it.z, sval = it.w;
         uint32_t spilled = 0;
+        // records are loaded one batch ahead: the next 64 are in flight while
+        // the current ones are counted
+        uint32_t rb = 0;
+        if (lane == 0) rb = atomicAdd(&s_next, 64u);
+        rb = __shfl(rb, 0, 64);
+        Rec<RW> nxt;
+        if (rb + lane < r1) nxt = load_rec<RW>(a.rec, rb + lane);
         for (;;) {
-            uint32_t rb = 0;
-            if (lane == 0) rb = atomicAdd(&s_next, 64u);
-            rb = __shfl(rb, 0, 64);
             if (rb >= r1) break;
             const uint32_t i = rb + lane;
-            uint32_t nk = 0, st = 0, len = 0;
+            const Rec<RW> rr = nxt;
+            {
+                uint32_t rn = 0;
+                if (lane == 0) rn = atomicAdd(&s_next, 64u);
+                rb = __shfl(rn, 0, 64);
+                if (rb + lane < r1) nxt = load_rec<RW>(a.rec, rb + lane);
+            }
+            uint32_t nk = 0;
             if (i < r1) {
-                const uint2 rr = a.rec[i];
-                if (((rr.y >> REC_HSEL_SHIFT) & smask) == sval) {
-                    len = rr.y & REC_LEN_MASK;
-                    nk = len - k + 1u;
-                    st = rr.x;
+                if (((rr.w[0] >> REC_HSEL_SHIFT) & smask) == sval) {
+                    nk = (rr.w[0] & REC_FAT_LEN_MASK) - k + 1u;
+#pragma unroll
+                    for (uint32_t q = 0; q < PW; ++q) s_stage[wv][lane * PW + q] = rr.w[1 + q];
                 }
             }
-            bool pending = nk > 0;
-            while (__ballot(pending)) {
-                const uint32_t w0 = st >> 4;
-                const uint32_t nwd = pending ? ((st + len - 1u) >> 4) - w0 + pad : 0u;
-                const uint32_t wincl = wave_incl_u32(nwd, lane);
-                const bool take = pending && wincl <= COUNT_STAGE;
-                const uint32_t wex = wincl - nwd;
-                if (take) {
-                    const uint32_t* g = a.packed + w0;
-                    uint32_t j = 0;
-                    for (; j + 4 <= nwd; j += 4) {
-                        const uint32_t v0 = g[j], v1 = g[j + 1], v2 = g[j + 2], v3 = g[j + 3];
-                        s_stage[wv][wex + j] = v0;
-                        s_stage[wv][wex + j + 1] = v1;
-                        s_stage[wv][wex + j + 2] = v2;
-                        s_stage[wv][wex + j + 3] = v3;
-                    }
-                    for (; j < nwd; ++j) s_stage[wv][wex + j] = g[j];
+            const uint32_t incl = wave_incl_u32(nk, lane);
+            s_incl[wv][lane] = incl;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            {
+                // seed of this lane's own record, computed by every lane at once so
+                // that crossing into a record later is two LDS reads
+                KRoll<NW> Q;
+                kroll_init<NW>(Q, stage, lane * PW, 0u, k);
+                s_seed[wv][lane][0] = Q.fl;
+                s_seed[wv][lane][1] = Q.rl;
+                if (NW == 2) {
+                    s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
+                    s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
                 }
-                const uint32_t incl = wave_incl_u32(take ? nk : 0u, lane);
-                s_incl[wv][lane] = incl;
-                s_wb[wv][lane] = wex;
-                s_st[wv][lane] = st;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            const uint32_t total = __shfl(incl, 63, 64);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint32_t q = (total + 63u) >> 6;
+            const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
+            if (j0 < j1) {
+                uint32_t s = 0;
+#pragma unroll
+                for (uint32_t step = 32; step; step >>= 1)
+                    if (s_incl[wv][s + step - 1u] <= j0) s += step;
+                const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
+                uint32_t end = s_incl[wv][s];
+                KRoll<NW> R;
                 {
-                    // seed of this lane's own record, computed by every lane at once so
-                    // that crossing into a record later is two LDS reads
-                    KRoll<NW> Q;
-                    kroll_init<NW>(Q, stage, take ? wex : 0u, take ? (st & 15u) : 0u, k);
-                    s_seed[wv][lane][0] = Q.fl;
-                    s_seed[wv][lane][1] = Q.rl;
-                    if (NW == 2) {
-                        s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
-                        s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
-                    }
+                    const uint32_t gb = j0 - ex;  // k-mer offset inside record s
+                    kroll_init<NW>(R, stage, s * PW + (gb >> 4), gb & 15u, k);
                 }
-                const uint32_t total = __shfl(incl, 63, 64);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const uint32_t q = (total + 63u) >> 6;
-                const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
-                if (j0 < j1) {
-                    uint32_t s = 0;
-    #pragma unroll
-                    for (uint32_t step = 32; step; step >>= 1)
-                        if (s_incl[wv][s + step - 1u] <= j0) s += step;
-                    const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
-                    uint32_t end = s_incl[wv][s];
-                    KRoll<NW> R;
-                    {
-                        const uint32_t rs = s_st[wv][s], gb = rs + (j0 - ex);
-                        kroll_init<NW>(R, stage, s_wb[wv][s] + (gb >> 4) - (rs >> 4), gb & 15u, k);
-                    }
-                    for (uint32_t j = j0;;) {
-                        bool ok;
-                        if (NW == 1) {
-                            const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
-                            ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
-                            if (!ok) {
-                                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
-                                else spill_insert1(spill_ref(a), x);
-                            }
-                        } else {
-                            const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
-                            const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
-                            ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
-                            if (!ok) {
-                                if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
-                                else spill_insert2(spill_ref(a), xh, xl);
-                            }
+                for (uint32_t j = j0;;) {
+                    bool ok;
+                    if (NW == 1) {
+                        const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
+                        ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
+                        if (!ok) {
+                            if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
+                            else spill_insert1(spill_ref(a), x);
                         }
-                        spilled += ok ? 0u : 1u;
-                        if (++j == j1) break;
-                        if (j == end) {
-                            // next record with k-mers (records not taken this round have none)
-                            do {
-                                ++s;
-                                end = s_incl[wv][s];
-                            } while (end == j);
-                            const uint32_t rs = s_st[wv][s];
-                            R.fl = s_seed[wv][s][0];
-                            R.rl = s_seed[wv][s][1];
-                            if (NW == 2) {
-                                R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
-                                R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
-                            }
-                            const uint32_t q2 = (rs & 15u) + k;
-                            R.wi = s_wb[wv][s] + (q2 >> 4);
-                            R.wrd = stage[R.wi] << (2u * (q2 & 15u));
-                            R.rem = 16u - (q2 & 15u);
-                        } else {
-                            kroll_step<NW>(R, stage, k);
+                    } else {
+                        const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
+                        const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
+                        ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
+                        if (!ok) {
+                            if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
+                            else spill_insert2(spill_ref(a), xh, xl);
                         }
                     }
+                    spilled += ok ? 0u : 1u;
+                    if (++j == j1) break;
+                    if (j == end) {
+                        // next record with k-mers (filtered records have none)
+                        do {
+                            ++s;
+                            end = s_incl[wv][s];
+                        } while (end == j);
+                        R.fl = s_seed[wv][s][0];
+                        R.rl = s_seed[wv][s][1];
+                        if (NW == 2) {
+                            R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
+                            R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
+                        }
+                        R.wi = s * PW + (k >> 4);
+                        R.wrd = stage[R.wi] << (2u * (k & 15u));
+                        R.nw = stage[R.wi + 1];
+                        R.rem = 16u - (k & 15u);
+                    } else {
+                        kroll_step<NW>(R, stage, k);
+                    }
                 }
-                pending = pending && !take;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
 
@@ -1562,11 +1705,11 @@ void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t
 }
 
 // Per job: reported keys from its items (segments) and from the spill table,
-// and its superkmers (records of its buckets).  Jobs past the batch get 0.
+// its superkmers and its k-mers.  Jobs past the batch get 0.
 __global__ void k_job_totals(const BatchDesc* __restrict__ d, BatchJobs j, uint32_t cap_jobs,
                              const uint64_t* __restrict__ bucket_item,
                              const uint64_t* __restrict__ seg_x,
-                             const uint64_t* __restrict__ part_off,
+                             const unsigned long long* __restrict__ job_sk,
                              const unsigned long long* __restrict__ job_sp,
                              const uint64_t* __restrict__ nkm_x,
                              uint64_t* __restrict__ job_tot, uint64_t* __restrict__ job_item_tot,
@@ -1586,16 +1729,17 @@ __global__ void k_job_totals(const BatchDesc* __restrict__ d, BatchJobs j, uint3
     const uint64_t it = seg_x[bucket_item[be]] - seg_x[bucket_item[b]];
     job_item_tot[q] = it;
     job_tot[q] = it + job_sp[q];
-    job_nsk[q] = part_off[be] - part_off[b];
+    job_nsk[q] = job_sk[q];
     job_nkm[q] = nkm_x[j.job_chunk[q + 1]] - nkm_x[j.job_chunk[q]];
 }
 
 void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
-                       const uint64_t* bucket_item, const uint64_t* seg_x, const uint64_t* part_off,
-                       const unsigned long long* job_sp, const uint64_t* nkm_x, uint64_t* job_tot,
-                       uint64_t* job_item_tot, uint64_t* job_nsk, uint64_t* job_nkm)
+                       const uint64_t* bucket_item, const uint64_t* seg_x,
+                       const unsigned long long* job_sk, const unsigned long long* job_sp,
+                       const uint64_t* nkm_x, uint64_t* job_tot, uint64_t* job_item_tot,
+                       uint64_t* job_nsk, uint64_t* job_nkm)
 {
-    k_job_totals<<<(cap_jobs + 255) / 256, 256, 0, st>>>(d, j, cap_jobs, bucket_item, seg_x, part_off,
+    k_job_totals<<<(cap_jobs + 255) / 256, 256, 0, st>>>(d, j, cap_jobs, bucket_item, seg_x, job_sk,
                                                          job_sp, nkm_x, job_tot, job_item_tot,
                                                          job_nsk, job_nkm);
 }

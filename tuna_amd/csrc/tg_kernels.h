@@ -6,11 +6,21 @@
 
 namespace tg {
 
-// Superkmer record, 8 bytes (device partition buffers):
-//   x = global base index of the first base (packed stream)
-//   y = len (bits 0-7) | minpos (bits 8-13) | hsel (bits 14-31)
-//   hsel = (sub-partition id << SB) | split bits, SB = 18 - log2(NSP)
+// Superkmer records (device partition buffers), RW 32-bit words each:
+//   slim (RW = 2, for exports):  w0 = global index of the first base,
+//                                w1 = meta
+//   fat  (RW = 4 for k <= 32, 8 for k <= 64, for counting):
+//                                w0 = meta, w1.. = the record's bases packed
+//                                16 per word, first base in bits 31:30
+//   meta = len (bits 0-7) | minpos (bits 8-13) | hsel (bits 14-31)
+//   hsel = (bucket << SB) | split bits
+// A fat record holds at most 16*(RW-1) bases, so a longer superkmer is cut into
+// pieces of the same minimizer (same bucket); the first piece keeps the
+// superkmer's minpos, the others set REC_CONT in the length byte (fat lengths
+// are < 128).  Pieces change nothing counted: every k-mer is in one piece.
 constexpr uint32_t REC_LEN_MASK = 0xFFu;
+constexpr uint32_t REC_FAT_LEN_MASK = 0x7Fu;
+constexpr uint32_t REC_CONT = 0x80u;
 constexpr uint32_t REC_MINPOS_SHIFT = 8;
 constexpr uint32_t REC_HSEL_SHIFT = 14;
 constexpr uint32_t HSEL_BITS = 18;
@@ -18,19 +28,26 @@ constexpr uint32_t LOOKBACK_FULL = 0xFFFFFFFFu;
 constexpr uint32_t SK_BLOCK = 256;      // lanes (strips) per superkmer workgroup
 constexpr uint32_t SK_WAVE_BUF = 256;   // records staged per wave before a flush
 
-// Phase-1 output is a list of "runs": run i holds run_cnt[i] records at
-// run_ptr[i].  The main pass writes one compacted run per workgroup; the
-// fix-up pass (strips redone from their chunk start) writes further runs.
+// words per record of the counting path
+__host__ __device__ constexpr uint32_t fat_rw(uint32_t k) { return k <= 32 ? 4u : 8u; }
+
+// Phase-1 output is a list of "runs": run r holds run_cnt[r] records starting
+// at record index run_start[r] of `recs`.  The main pass writes one compacted
+// run per workgroup into the slot space of its strips (a strip of K k-mers has
+// at most K records, so slot = first k-mer index: no global atomics); the
+// fix-up pass writes one run per workgroup into a region of SK_BLOCK*S records
+// per block starting at fix_base.
 struct SkArgs {
     const uint32_t* packed;
     const uint64_t* off;        // chunk offsets (global base index), n_chunks+1
     const uint32_t* strip_chunk;
     const uint32_t* strip_tb;   // first owned k-mer of the strip (chunk-relative)
-    const uint64_t* strip_slot; // record slot base of the strip (main pass)
+    const uint64_t* strip_slot; // main pass: record slot of the strip's first k-mer
     const uint32_t* list;       // fix-up pass: strip indices; nullptr in the main pass
     uint32_t n;                 // strips (or list entries) to process
     uint32_t k, m, W, S;
     uint32_t capk;              // a superkmer spans at most capk k-mers (256-k)
+    uint32_t piece;             // fat records: k-mers per piece (16*(RW-1)-k+1); 0 for slim
     uint32_t lookback;          // k-mers of look-back before tb, or LOOKBACK_FULL
     uint32_t log2n, xb;         // routing field: job bucket bits on top, xb bits below
     // batched jobs: job of each chunk and its bucket range (base | log2(buckets) << 16);
@@ -41,17 +58,20 @@ struct SkArgs {
     // fix-up pass reads flag_count; n is then the capacity of the launch
     const uint64_t* n_dev;
     uint32_t fix_dev;
-    uint32_t* err;              // bit 0: more flagged strips than the fix-up capacity
-    uint2* rec;                 // main pass: slot space; fix-up: S*SK_BLOCK records per block
-    uint2** run_ptr;            // run table (indexed from run_base + blockIdx.x)
+    uint32_t* err;              // bit 0: fix-up capacity exceeded
+    uint32_t* recs;             // record storage (RW words per record)
+    uint32_t fix_base;          // fix-up pass: first record index of its regions
+    uint32_t* run_start;        // run table, indexed from run_base + blockIdx.x
     uint32_t* run_cnt;
     uint32_t run_base;
+    unsigned long long* sk_count;  // superkmers (first pieces) per job (one entry for a single job)
     uint32_t* flag_list;
     uint32_t* flag_count;
 };
 
 struct RouteArgs {
-    uint2* const* run_ptr;
+    const uint32_t* recs;      // record storage of the runs (RW words per record)
+    const uint32_t* run_start;
     const uint32_t* run_cnt;
     const uint64_t* run_off;   // exclusive scan of run_cnt (n_runs+1)
     uint32_t n_runs;
@@ -61,18 +81,21 @@ struct RouteArgs {
     uint32_t* hsk;             // [ntiles][nsp] superkmers, becomes tile-exclusive offsets
     uint32_t* hkm;             // [ntiles][nsp] k-mers
     const uint64_t* part_off;  // nsp+1 (scatter only)
-    uint2* dst;                // partition-ordered records (scatter only)
+    uint32_t* dst;             // partition-ordered records (scatter only)
     // device-side record total (nullptr: n_rec/tile/grid are exact); the
-    // kernels then derive tile and tile count themselves, up to max_tiles
+    // kernels then derive tile and tile count themselves, up to max_tiles.  A
+    // total above rec_cap (the partition buffers' capacity) routes nothing and
+    // sets bit 1 of *err.
     const uint64_t* n_rec_dev;
     uint32_t max_tiles;
+    uint64_t rec_cap;
+    uint32_t* err;
 };
 
 // Phase-2 work item: records [x, y) of consecutive sub-partitions; records with
 // (split bits & z) == w are counted by this item.
 struct CountArgs {
-    const uint32_t* packed;
-    const uint2* rec;
+    const uint32_t* rec;       // partition-ordered fat records
     const uint4* items;
     uint32_t n_items;
     uint32_t k;
@@ -130,11 +153,13 @@ void launch_fill_strips(hipStream_t st, uint64_t n_chunks, const uint64_t* nstr_
 uint64_t scan_tmp_words(uint64_t n);
 void launch_scan_u64(hipStream_t st, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* tmp);
 void launch_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t n, uint64_t* out);
-void launch_superkmers(hipStream_t st, const SkArgs& a);
-void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles);
+// rw: words per record (2 = slim, 4 or 8 = fat)
+void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw);
+void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t rw);
 void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint64_t* part_sk,
                           uint64_t* part_km);
-void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint2* tmp);
+void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t* tmp,
+                          uint32_t rw);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n);
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw);
@@ -155,9 +180,10 @@ void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_
 void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
                        unsigned long long* job_sp);
 void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
-                       const uint64_t* bucket_item, const uint64_t* seg_x, const uint64_t* part_off,
-                       const unsigned long long* job_sp, const uint64_t* nkm_x, uint64_t* job_tot,
-                       uint64_t* job_item_tot, uint64_t* job_nsk, uint64_t* job_nkm);
+                       const uint64_t* bucket_item, const uint64_t* seg_x,
+                       const unsigned long long* job_sk, const unsigned long long* job_sp,
+                       const uint64_t* nkm_x, uint64_t* job_tot, uint64_t* job_item_tot,
+                       uint64_t* job_nsk, uint64_t* job_nkm);
 void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j,
                          const uint64_t* bucket_item, const uint64_t* seg_x,
                          const uint64_t* job_off, uint64_t* dst_keys, uint32_t* dst_cnt,
