@@ -60,7 +60,7 @@ def cpu_baseline(text, n_bases_total, threads):
     kache table, oracle/_ref) on a bounded sample of the same reads, split
     into `threads` files so both phases run on `threads` cores."""
     import oracle as O
-    reads_sample = 400000  # 60 Mbp: ~10-30 s of CPU work
+    reads_sample = 1666667  # the whole workload (250 Mbp): ~20 s of CPU work on 16 cores
     lines = text.split(b"\n")
     nrec = min(reads_sample, len(lines) // 4)
     tmp = tempfile.mkdtemp(prefix="tg_cpu_")

@@ -338,7 +338,8 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
     {
         Prof pr(prof, "count_items");
         tg::launch_build_items(st, b.part_km, b.part_off, c.nsp, c.xb, b.bucket_job, &b.desc->scale,
-                               b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS);
+                               b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS, b.part_sk,
+                               b.scan_tmp);
     }
     tg::CountArgs ca{};
     ca.rec = b.sorted;

@@ -692,6 +692,8 @@ constexpr uint32_t ROUTE_T = 1024;
 
 constexpr uint32_t ROUTE_RUNS = 1024;  // run descriptors cached per tile (LDS: 16 KiB)
 
+__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t& total);
+
 // Tile geometry: records per tile and live tiles.  With a device-side record
 // total the tile is the smallest power of two >= 16384 that keeps the tile
 // count within max_tiles (the rule the host applies for exact launches).
@@ -827,7 +829,6 @@ void set_route_attributes()
     const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
     const int cur = (int)((1u << 14) * sizeof(uint32_t));
     (void)hipFuncSetAttribute((const void*)k_route_hist<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
-    (void)hipFuncSetAttribute((const void*)k_route_coarse<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
     (void)hipFuncSetAttribute((const void*)k_route_fine<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
 }
 
@@ -896,30 +897,105 @@ void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
 }
 
 // Scatter, level 1: every tile writes its records into coarse buckets of F
-// consecutive sub-partitions.  A tile's runs are long (tile/nc records) and
-// the lines being filled at once fit in L2, so partial lines merge there.
+// consecutive sub-partitions (at most 256 buckets).  The tile goes through its
+// records in chunks that fit in LDS (64 KiB); a chunk is grouped by bucket in
+// LDS (histogram, scan, place) and then written out bucket by bucket, so
+// consecutive threads write consecutive addresses of one bucket.
+constexpr uint32_t COARSE_RUNS = 512;  // run descriptors cached per tile
+
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint32_t* tmp)
 {
-    extern __shared__ uint32_t cur[];  // nc cursors
-    __shared__ uint64_t s_off[ROUTE_RUNS + 1];
-    __shared__ int64_t s_base[ROUTE_RUNS];
+    constexpr uint32_t CH = 16384u / RW;  // records per chunk (64 KiB)
+    constexpr uint32_t PER = CH / ROUTE_T;
+    __shared__ Rec<RW> s_rec[CH];
+    __shared__ uint32_t s_hist[256], s_start[256], s_cur[256];
+    __shared__ uint64_t s_off[COARSE_RUNS + 1];
+    __shared__ int64_t s_base[COARSE_RUNS];
     __shared__ uint32_t s_meta[4];
+    __shared__ uint32_t s_red[ROUTE_T / 64];
     const RouteGeom g = route_geom(a);
     if (blockIdx.x >= g.ntiles) return;
     const uint32_t nc = a.nsp >> log2f, F = 1u << log2f;
+    const uint32_t tid = threadIdx.x;
     const uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * a.nsp;
-    for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+    for (uint32_t c = tid; c < nc; c += blockDim.x) {
         uint32_t o = (uint32_t)a.part_off[c * F];
         for (uint32_t f = 0; f < F; ++f) o += hs[c * F + f];
-        cur[c] = o;
+        s_cur[c] = o;
+    }
+    const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
+    const uint64_t hi = min(lo + g.tile, g.n_rec);
+    if (tid == 0) {
+        uint32_t l = 0, h = a.n_runs;
+        while (l < h) {  // first run with run_off[r+1] > lo
+            const uint32_t mid = (l + h) >> 1;
+            if (a.run_off[mid + 1] > lo) h = mid;
+            else l = mid + 1;
+        }
+        uint32_t e = l;
+        while (e < a.n_runs && a.run_off[e] < hi && e - l < COARSE_RUNS - 1) ++e;
+        s_meta[0] = l;
+        s_meta[1] = e - l;
+        s_meta[2] = (e < a.n_runs && a.run_off[e] < hi) ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t r0 = s_meta[0], nr = s_meta[1];
+    const bool cached = s_meta[2] == 0;
+    for (uint32_t i = tid; i <= nr; i += blockDim.x) {
+        s_off[i] = a.run_off[r0 + i];
+        if (i < nr) s_base[i] = (int64_t)a.run_start[r0 + i] - (int64_t)a.run_off[r0 + i];
     }
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb + log2f;
-    for_tile_records<RW>(a, g, s_off, s_base, s_meta, [&](const Rec<RW>& v) {
-        const uint32_t pos = atomicAdd(&cur[rec_meta<RW>(v) >> shift], 1u);
-        store_rec<RW>(tmp, pos, v);
-    });
+    uint32_t rr = 0;  // cached run of this thread's last record (indices only grow)
+    for (uint64_t cb = lo; cb < hi; cb += CH) {
+        for (uint32_t c = tid; c < nc; c += blockDim.x) s_hist[c] = 0;
+        __syncthreads();
+        Rec<RW> v[PER];
+        uint32_t bk[PER], pos[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint64_t i = cb + j * ROUTE_T + tid;
+            bk[j] = 0xFFFFFFFFu;
+            if (i < hi) {
+                int64_t base;
+                if (cached) {
+                    while (s_off[rr + 1] <= i) ++rr;
+                    base = s_base[rr];
+                } else {
+                    uint32_t l = 0, h = a.n_runs;
+                    while (l < h) {
+                        const uint32_t mid = (l + h) >> 1;
+                        if (a.run_off[mid + 1] > i) h = mid;
+                        else l = mid + 1;
+                    }
+                    base = (int64_t)a.run_start[l] - (int64_t)a.run_off[l];
+                }
+                v[j] = load_rec<RW>(a.recs, (uint64_t)(base + (int64_t)i));
+                bk[j] = rec_meta<RW>(v[j]) >> shift;
+                pos[j] = atomicAdd(&s_hist[bk[j]], 1u);
+            }
+        }
+        __syncthreads();
+        uint32_t tot;
+        const uint32_t ex = block_excl_u32(tid < nc ? s_hist[tid] : 0u, s_red, tot);
+        if (tid < nc) s_start[tid] = ex;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j)
+            if (bk[j] != 0xFFFFFFFFu) s_rec[s_start[bk[j]] + pos[j]] = v[j];
+        __syncthreads();
+        const uint32_t nch = (uint32_t)min((uint64_t)CH, hi - cb);
+        for (uint32_t i = tid; i < nch; i += ROUTE_T) {
+            const Rec<RW> r = s_rec[i];
+            const uint32_t b = rec_meta<RW>(r) >> shift;
+            store_rec<RW>(tmp, (uint64_t)s_cur[b] + (i - s_start[b]), r);
+        }
+        __syncthreads();
+        if (tid < nc) s_cur[tid] += s_hist[tid];
+        __syncthreads();
+    }
 }
 
 // Scatter, level 2: one workgroup per coarse bucket reads its region
@@ -961,7 +1037,7 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     while ((a.nsp >> log2f) > 256u) ++log2f;
     const uint32_t nc = a.nsp >> log2f;
     set_kernel_attributes();
-    const size_t lc = nc * sizeof(uint32_t), lf = (1u << log2f) * sizeof(uint32_t);
+    const size_t lc = 0, lf = (1u << log2f) * sizeof(uint32_t);
     if (rw == 2) {
         k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
         k_route_fine<2><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
@@ -1292,7 +1368,9 @@ __device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
     return v;
 }
 
-constexpr uint32_t COUNT_STAGE = 512;  // packed words staged per wave (64 records x <= 7, + 1)
+// packed words staged per wave: 64 records x (RW-1) words, + 1 read ahead
+template <int NW>
+constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8u; }
 
 // One workgroup per work item.  A wave takes 64 fat records at a time from a
 // workgroup dispenser (one coalesced vector load per lane), puts their bases
@@ -1311,7 +1389,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ __align__(16) uint64_t s_k0[CAP];
     __shared__ uint64_t s_k1[NW == 2 ? CAP : 1];
     __shared__ uint32_t s_cnt[CAP];
-    __shared__ uint32_t s_stage[NWAVE][COUNT_STAGE];
+    __shared__ uint32_t s_stage[NWAVE][count_stage<NW>()];
     __shared__ uint32_t s_incl[NWAVE][64];
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint32_t s_red[NWAVE];
@@ -1627,56 +1705,59 @@ void launch_batch_setup(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_
 // Work items of a batch, built on the device: every non-empty bucket is one
 // item, split by the low routing bits into r = 2^i items when its estimated
 // distinct k-mers (k-mers x scale, scale = rho / target) exceed one table.
-// Buckets never merge, so an item never spans two jobs.
-__global__ __launch_bounds__(1024) void k_build_items(const uint64_t* __restrict__ part_km,
-                                                      const uint64_t* __restrict__ part_off,
-                                                      uint32_t nsp, uint32_t xb,
-                                                      const uint32_t* __restrict__ bucket_job,
-                                                      const float* __restrict__ scale_p,
-                                                      uint4* __restrict__ items,
-                                                      uint32_t* __restrict__ item_job,
-                                                      uint64_t* __restrict__ bucket_item,
-                                                      uint32_t* __restrict__ n_items)
+// Buckets never merge, so an item never spans two jobs.  Three steps: split
+// counts per bucket, their exclusive scan (bucket_item), the items.
+__device__ __forceinline__ uint32_t bucket_splits(const uint64_t* part_km, const uint64_t* part_off,
+                                                  uint32_t b, uint32_t rmax, float scale)
 {
-    __shared__ uint32_t s_red[16];
-    const float scale = *scale_p;
-    const uint32_t per = (nsp + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = min(threadIdx.x * per, nsp), b1 = min(b0 + per, nsp);
-    const uint32_t rmax = 1u << xb;
-    auto splits = [&](uint32_t b) -> uint32_t {
-        if (part_off[b + 1] == part_off[b]) return 0u;
-        const float est = (float)part_km[b] * scale;
-        uint32_t r = 1;
-        while (r < rmax && est > (float)r) r <<= 1;
-        return r;
-    };
-    uint32_t mine = 0;
-    for (uint32_t b = b0; b < b1; ++b) mine += splits(b);
-    uint32_t total;
-    uint32_t cur = block_excl_u32(mine, s_red, total);
-    for (uint32_t b = b0; b < b1; ++b) {
-        bucket_item[b] = cur;
-        const uint32_t r = splits(b);
-        const uint32_t lo = (uint32_t)part_off[b], hi = (uint32_t)part_off[b + 1];
-        const uint32_t job = bucket_job[b];
-        for (uint32_t q = 0; q < r; ++q) {
-            items[cur + q] = make_uint4(lo, hi, r - 1u, q);
-            item_job[cur + q] = job;
-        }
-        cur += r;
-    }
-    if (threadIdx.x == 0) {
-        bucket_item[nsp] = total;
-        *n_items = total;
+    if (part_off[b + 1] == part_off[b]) return 0u;
+    const float est = (float)part_km[b] * scale;
+    uint32_t r = 1;
+    while (r < rmax && est > (float)r) r <<= 1;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_item_counts(const uint64_t* __restrict__ part_km,
+                                                     const uint64_t* __restrict__ part_off,
+                                                     uint32_t nsp, uint32_t xb,
+                                                     const float* __restrict__ scale_p,
+                                                     uint64_t* __restrict__ nitem)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nsp) nitem[b] = bucket_splits(part_km, part_off, b, 1u << xb, *scale_p);
+}
+
+__global__ __launch_bounds__(256) void k_items(const uint64_t* __restrict__ part_km,
+                                               const uint64_t* __restrict__ part_off, uint32_t nsp,
+                                               uint32_t xb, const uint32_t* __restrict__ bucket_job,
+                                               const float* __restrict__ scale_p,
+                                               const uint64_t* __restrict__ bucket_item,
+                                               uint4* __restrict__ items, uint32_t* __restrict__ item_job,
+                                               uint32_t* __restrict__ n_items)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *n_items = (uint32_t)bucket_item[nsp];
+    if (b >= nsp) return;
+    const uint32_t r = bucket_splits(part_km, part_off, b, 1u << xb, *scale_p);
+    const uint32_t lo = (uint32_t)part_off[b], hi = (uint32_t)part_off[b + 1];
+    const uint32_t job = bucket_job[b];
+    const uint64_t cur = bucket_item[b];
+    for (uint32_t q = 0; q < r; ++q) {
+        items[cur + q] = make_uint4(lo, hi, r - 1u, q);
+        item_job[cur + q] = job;
     }
 }
 
 void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
                         uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
-                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items)
+                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items,
+                        uint64_t* nitem, uint64_t* scan_tmp)
 {
-    k_build_items<<<1, 1024, 0, st>>>(part_km, part_off, nsp, xb, bucket_job, scale, items,
-                                      item_job, bucket_item, n_items);
+    const unsigned blocks = (nsp + 255) / 256;
+    k_item_counts<<<blocks, 256, 0, st>>>(part_km, part_off, nsp, xb, scale, nitem);
+    launch_scan_u64(st, nitem, nsp, bucket_item, scan_tmp);
+    k_items<<<blocks, 256, 0, st>>>(part_km, part_off, nsp, xb, bucket_job, scale, bucket_item, items,
+                                    item_job, n_items);
 }
 
 // Spill entries per job: distinct keys, and reported keys (ci/cx filter).

@@ -172,9 +172,11 @@ void launch_batch_setup(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_
                         uint64_t* boff, uint32_t* chunk_job, uint64_t cap_ch, uint32_t* bucket_job,
                         uint32_t nsp, uint32_t* err);
 void set_kernel_attributes();
+// nitem: nsp per-bucket item counts; scan_tmp >= scan_tmp_words(nsp)
 void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
                         uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
-                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items);
+                        uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items,
+                        uint64_t* nitem, uint64_t* scan_tmp);
 void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks);
 // spill entries of a batch: per-job distinct and reported counts, then placement
 void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
