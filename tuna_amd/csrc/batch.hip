@@ -66,6 +66,7 @@ struct BatchBufs {
     uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_start, *run_cnt, *flags;
     uint64_t *strip_slot, *run_cnt64, *run_off;
     uint32_t *recs, *sorted, *route_tmp;  // rw words per record
+    uint32_t* metas;                      // meta word per record slot
     uint32_t *hsk, *hkm, *bucket_job, *item_job;
     uint64_t *part_sk, *part_km, *part_off, *bucket_item;
     uint4* items;
@@ -196,7 +197,7 @@ int alloc_class(BatchState* s, const BatchClass& c)
         (rc = balloc(s, &b.flags, F_WORDS)) || (rc = balloc(s, &b.run_start, b.n_runs + 1)) ||
         (rc = balloc(s, &b.run_cnt, b.n_runs + 1)) || (rc = balloc(s, &b.run_cnt64, b.n_runs + 1)) ||
         (rc = balloc(s, &b.run_off, b.n_runs + 2)) ||
-        (rc = balloc(s, &b.recs, (slots + 1) * c.rw)) ||
+        (rc = balloc(s, &b.recs, (slots + 1) * c.rw)) || (rc = balloc(s, &b.metas, slots + 1)) ||
         (rc = balloc(s, &b.sorted, (c.rec_cap + 1) * c.rw)) ||
         (rc = balloc(s, &b.route_tmp, (c.rec_cap + 1) * c.rw)) ||
         (rc = balloc(s, &b.hsk, (uint64_t)c.max_tiles * c.nsp)) ||
@@ -276,6 +277,7 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
     a.fix_dev = 0;
     a.err = b.flags + F_SKERR;
     a.recs = b.recs;
+    a.metas = b.metas;
     a.fix_base = (uint32_t)c.cap_bases;
     a.run_start = b.run_start;
     a.run_cnt = b.run_cnt;
@@ -300,6 +302,7 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
     // routing into job buckets
     tg::RouteArgs r{};
     r.recs = b.recs;
+    r.metas = b.metas;
     r.run_start = b.run_start;
     r.run_cnt = b.run_cnt;
     r.run_off = b.run_off;

@@ -100,8 +100,8 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         const uint64_t rec_cap = o.n_kmers + fix_recs;
         const uint64_t run_cap = n_blocks + fixb + 1;
         if (rec_cap >= (1ull << 32)) return fail(TG_EINVAL, "input too large for one job's record space");
-        uint32_t* recs;
-        if ((rc = ensure_t(c, "recs", rec_cap * o.rw, &recs)) ||
+        uint32_t *recs, *metas;
+        if ((rc = ensure_t(c, "recs", rec_cap * o.rw, &recs)) || (rc = ensure_t(c, "metas", rec_cap, &metas)) ||
             (rc = ensure_t(c, "run_start", run_cap, &run_start)) ||
             (rc = ensure_t(c, "run_cnt", run_cap, &run_cnt)) ||
             (rc = ensure_t(c, "run_off", run_cap + 1, &run_off)))
@@ -130,6 +130,7 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         a.xb = tg::HSEL_BITS - o.log2n;
         a.err = flags + 1;
         a.recs = recs;
+        a.metas = metas;
         a.fix_base = (uint32_t)o.n_kmers;
         a.run_start = run_start;
         a.run_cnt = run_cnt;
@@ -203,6 +204,7 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
             return rc;
         tg::RouteArgs r{};
         r.recs = recs;
+        r.metas = metas;
         r.run_start = run_start;
         r.run_cnt = run_cnt;
         r.run_off = run_off;

@@ -310,7 +310,9 @@ template <int RW>
 __device__ __forceinline__ Rec<RW> load_rec(const uint32_t* base, uint64_t i)
 {
     Rec<RW> r;
-    if (RW == 2) {
+    if (RW == 1) {
+        r.w[0] = base[i];
+    } else if (RW == 2) {
         const uint2 v = reinterpret_cast<const uint2*>(base)[i];
         r.w[0] = v.x;
         r.w[1] = v.y;
@@ -342,6 +344,10 @@ __device__ __forceinline__ void store_rec(uint32_t* base, uint64_t i, const Rec<
 
 template <int RW>
 __device__ __forceinline__ uint32_t rec_meta(const Rec<RW>& r) { return RW == 2 ? r.w[1] : r.w[0]; }
+
+// the meta word alone (RW = 1 views the meta array)
+template <>
+__device__ __forceinline__ uint32_t rec_meta<1>(const Rec<1>& r) { return r.w[0]; }
 
 template <int RW>
 __device__ __forceinline__ uint32_t rec_len(uint32_t meta)
@@ -403,6 +409,7 @@ __device__ __forceinline__ void sk_flush(const SkStage& S, const SkArgs& a, uint
         } else {
             store_rec<RW>(a.recs, S.run + base + i, fat_record<RW>(a.packed, e.x, e.y));
         }
+        a.metas[S.run + base + i] = e.y;
     }
     __builtin_amdgcn_wave_barrier();
     *S.wn = 0;
@@ -761,7 +768,7 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
             const uint64_t b = max(a.run_off[r], lo), e = min(a.run_off[r + 1], hi);
             const int64_t base = (int64_t)a.run_start[r] - (int64_t)a.run_off[r];
             for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x)
-                f(load_rec<RW>(a.recs, (uint64_t)(base + (int64_t)i)));
+                f(load_rec<RW>(RW == 1 ? a.metas : a.recs, (uint64_t)(base + (int64_t)i)));
         }
         return;
     }
@@ -777,7 +784,7 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
             ok[u] = i < hi;
             if (ok[u]) {
                 while (s_off[rr + 1] <= i) ++rr;
-                v[u] = load_rec<RW>(a.recs, (uint64_t)(s_base[rr] + (int64_t)i));
+                v[u] = load_rec<RW>(RW == 1 ? a.metas : a.recs, (uint64_t)(s_base[rr] + (int64_t)i));
             }
         }
 #pragma unroll
@@ -786,6 +793,8 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
     }
 }
 
+// RW is the record kind (2 slim, 4/8 fat): it fixes the length field's width;
+// the histogram reads only the meta array.
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
 {
@@ -800,8 +809,8 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t k1 = a.k - 1u;
-    for_tile_records<RW>(a, g, s_off, s_base, s_meta, [&](const Rec<RW>& v) {
-        const uint32_t meta = rec_meta<RW>(v);
+    for_tile_records<1>(a, g, s_off, s_base, s_meta, [&](const Rec<1>& v) {
+        const uint32_t meta = v.w[0];
         const uint32_t pid = meta >> shift;
         atomicAdd(&sh[pid], 1u);
         atomicAdd(&sh[nsp + pid], rec_len<RW>(meta) - k1);

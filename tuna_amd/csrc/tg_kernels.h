@@ -60,6 +60,7 @@ struct SkArgs {
     uint32_t fix_dev;
     uint32_t* err;              // bit 0: fix-up capacity exceeded
     uint32_t* recs;             // record storage (RW words per record)
+    uint32_t* metas;            // the meta word of every record, same slots (routing histograms)
     uint32_t fix_base;          // fix-up pass: first record index of its regions
     uint32_t* run_start;        // run table, indexed from run_base + blockIdx.x
     uint32_t* run_cnt;
@@ -71,6 +72,7 @@ struct SkArgs {
 
 struct RouteArgs {
     const uint32_t* recs;      // record storage of the runs (RW words per record)
+    const uint32_t* metas;     // meta word per record slot (histograms read only these)
     const uint32_t* run_start;
     const uint32_t* run_cnt;
     const uint64_t* run_off;   // exclusive scan of run_cnt (n_runs+1)
