@@ -798,29 +798,30 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
 {
-    extern __shared__ uint32_t sh[];  // [0, nsp): superkmers, [nsp, 2nsp): k-mers
+    // one 64-bit counter per sub-partition: superkmers (low half) and k-mers
+    // (high half), so a record costs one LDS atomic
+    extern __shared__ unsigned long long sh64[];
     __shared__ uint64_t s_off[ROUTE_RUNS + 1];
     __shared__ int64_t s_base[ROUTE_RUNS];
     __shared__ uint32_t s_meta[4];
     const RouteGeom g = route_geom(a);
     if (blockIdx.x >= g.ntiles) return;
     const uint32_t nsp = a.nsp;
-    for (uint32_t i = threadIdx.x; i < 2u * nsp; i += blockDim.x) sh[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nsp; i += blockDim.x) sh64[i] = 0ull;
     __syncthreads();
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t k1 = a.k - 1u;
     for_tile_records<1>(a, g, s_off, s_base, s_meta, [&](const Rec<1>& v) {
         const uint32_t meta = v.w[0];
-        const uint32_t pid = meta >> shift;
-        atomicAdd(&sh[pid], 1u);
-        atomicAdd(&sh[nsp + pid], rec_len<RW>(meta) - k1);
+        atomicAdd(&sh64[meta >> shift], 1ull | ((unsigned long long)(rec_len<RW>(meta) - k1) << 32));
     });
     __syncthreads();
     uint32_t* hs = a.hsk + (uint64_t)blockIdx.x * nsp;
     uint32_t* hk = a.hkm + (uint64_t)blockIdx.x * nsp;
     for (uint32_t i = threadIdx.x; i < nsp; i += blockDim.x) {
-        hs[i] = sh[i];
-        hk[i] = sh[nsp + i];
+        const unsigned long long v = sh64[i];
+        hs[i] = (uint32_t)v;
+        hk[i] = (uint32_t)(v >> 32);
     }
 }
 
