@@ -1,0 +1,80 @@
+"""Full-size workloads, checked through size-independent properties (the
+oracle would take too long at these sizes):
+  * the counts add up to the number of k-mers, and every distinct k-mer is
+    reported when ci = 1;
+  * the multiset of (k-mer, count) pairs, reduced to an order-free checksum,
+    is the same whatever the partitioning (bucket size) and whichever engine
+    mode (batched launch sequence, plain host-driven sequence) counts it."""
+import sys
+
+import numpy as np
+import pytest
+
+import tuna_amd as T
+
+pytestmark = pytest.mark.gpu
+
+C2 = "kind=reads_fq genome=5000000 seed=2 gc=0.5 reads=1666667 rl=150 sub=0.005 nrate=0.0001"
+C3 = ("kind=repeat_fq genome=100000000 seed=3 gc=0.45 cov=20 rl=150 sub=0.003 "
+      "fam=50 flen=300 div=0.05 rep=0.35 tan=0.10")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def checksum(keys, cnt):
+    """Sum over pairs of a mixed 64-bit hash of (key words, count): independent
+    of the order of the pairs, sensitive to any changed, lost or split pair."""
+    h = np.zeros(len(cnt), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for w in range(keys.shape[1]):
+            h ^= keys[:, w] * np.uint64(0x9E3779B97F4A7C15 + 2 * w)
+            h = (h ^ (h >> np.uint64(31))) * np.uint64(0xBF58476D1CE4E5B9)
+        h ^= cnt.astype(np.uint64) * np.uint64(0x94D049BB133111EB)
+        h = (h ^ (h >> np.uint64(29))) * np.uint64(0xD6E8FEB86659FD93)
+        return int(h.sum(dtype=np.uint64))
+
+
+def summary(eng, k):
+    keys, cnt = eng.result(k)
+    return len(cnt), int(cnt.sum(dtype=np.uint64)), checksum(keys, cnt)
+
+
+@pytest.mark.parametrize("name,spec,k,m,ci", [
+    ("short reads 250 Mbp", C2, 31, 21, 2),
+    ("repetitive genome reads 2 Gbp", C3, 31, 23, 1),
+])
+def test_full_size_properties(name, spec, k, m, ci):
+    bases, off = T.synth_parsed(spec)
+    say(f"{name}: {len(bases)} bases, {len(off) - 1} chunks")
+    eng = T.Engine(0)
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    N, nch = len(bases), len(off) - 1
+    del bases
+    try:
+        st = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        ref = summary(eng, k)
+        say(f"  batched: {st.n_kmers} k-mers, {st.n_unique} distinct, {ref[0]} reported")
+        assert ref[0] == st.n_out
+        if ci == 1:
+            assert ref[1] == st.n_kmers and ref[0] == st.n_unique
+        else:
+            assert ref[1] <= st.n_kmers and ref[0] <= st.n_unique
+        # another partitioning of the same input
+        eng.set_batch_config(kmers_per_bucket=65536)
+        st2 = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        assert (st2.n_kmers, st2.n_unique, st2.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
+        assert summary(eng, k) == ref
+        say("  other bucket size: same multiset")
+        # the plain host-driven sequence
+        eng.set_plain_sequence(True)
+        st3 = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        assert (st3.n_kmers, st3.n_unique, st3.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
+        assert summary(eng, k) == ref
+        say("  plain sequence: same multiset")
+    finally:
+        eng.dev_free(d_b)
+        eng.dev_free(d_o)
+        eng.close()
