@@ -3,6 +3,7 @@ result must equal counting it alone (oracle and single-job path), bit-exact,
 whatever the batching, graph replay or overflow handling."""
 import random
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -154,6 +155,25 @@ def test_spill_and_retry_path(eng):
         check_jobs(eng, parts, 63, 21, label="spill k63")
     finally:
         eng.set_batch_config(kmers_per_bucket=old.kmers_per_bucket, rho=0.0)
+
+
+def test_ratio_learned_on_other_input(eng):
+    # the distinct/k-mer ratio learned on high-coverage reads (about 0.24)
+    # sizes the work items of a genome whose k-mers are all distinct: the
+    # items then hold about three tables' worth of keys each and overflow the
+    # spill table; the overflowed attempt must end quickly and the retry must
+    # re-split the items
+    reads = T.parse_buffer(T.synth("kind=reads_fq genome=200000 seed=2 cov=20 rl=150 sub=0.005 nrate=0.0001"))
+    genome = T.parse_buffer(T.synth("kind=genome_fa len=4600000 seed=1"))
+    assert eng.batch_config().rho == 0.0
+    check_jobs(eng, [reads], 31, 21, label="reads (learns the ratio)")
+    t0 = time.time()
+    check_jobs(eng, [genome], 31, 21, label="genome (all k-mers distinct)")
+    ms = eng.last_many
+    say(f"  retries {ms.n_retries}, {time.time() - t0:.1f} s with the oracle")
+    assert ms.n_retries <= 1
+    check_jobs(eng, [genome], 31, 21, label="genome again (ratio learned)")
+    assert eng.last_many.n_retries == 0
 
 
 def test_many_samples_workload(eng):

@@ -672,7 +672,13 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
                     want.fixb = (uint32_t)std::max<uint64_t>(
                         want.fixb * 2ull, (h_flags[F_FLAGS] + tg::SK_BLOCK - 1) / tg::SK_BLOCK);
                 if (h_flags[F_SKERR] & 2u) want.rec_cap = want.cap_bases;
-                if (h_flags[F_SPERR]) want.sp_slots <<= 2;
+                if (h_flags[F_SPERR]) {
+                    // items sized from a distinct/k-mer ratio learned on other
+                    // input can hold several tables' worth of distinct k-mers:
+                    // split them further as well as growing the spill table
+                    want.sp_slots <<= 2;
+                    s->rho = std::min(1.0, s->rho * 4.0);
+                }
                 s->last.n_retries += 1;
                 continue;
             }

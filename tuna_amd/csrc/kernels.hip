@@ -1088,10 +1088,16 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n)
 
 // ========================================================== counting ====
 constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
+constexpr uint32_t SPILL_PROBES = 4096;
 
 // Spill-table view passed by value to the out-of-line insert paths (a
 // reference to the kernel's argument struct would force a per-thread copy of
 // it into scratch memory).
+//
+// Overflow: a key that finds no slot within SPILL_PROBES sets *err and is
+// dropped; the host then counts the whole job again with a larger table, so
+// once *err is set every later insert returns at once instead of scanning a
+// full table.
 struct SpillRef {
     uint64_t* keys;
     uint32_t* cnt;
@@ -1107,8 +1113,9 @@ __device__ __forceinline__ SpillRef spill_ref(const CountArgs& a)
 // Global spill table, one 64-bit word per key: claim by CAS on the key.
 __device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x)
 {
+    if (__hip_atomic_load(t.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t slot = mix64(x ^ 0x9e3779b97f4a7c15ull) & t.mask;
-    for (uint32_t p = 0; p <= t.mask && p < (1u << 16); ++p) {
+    for (uint32_t p = 0; p <= t.mask && p < SPILL_PROBES; ++p) {
         unsigned long long* kp = (unsigned long long*)&t.keys[slot];
         const uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == x) {
@@ -1132,8 +1139,9 @@ __device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x)
 // (0 empty, LOCKED32 being written, otherwise published count).
 __device__ __noinline__ bool spill_insert2(SpillRef t, uint64_t hi, uint64_t lo)
 {
+    if (__hip_atomic_load(t.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t slot = mix64(lo ^ mix64(hi ^ 0x9e3779b97f4a7c15ull)) & t.mask;
-    for (uint32_t p = 0; p <= t.mask && p < (1u << 16);) {
+    for (uint32_t p = 0; p <= t.mask && p < SPILL_PROBES;) {
         uint32_t* cp = &t.cnt[slot];
         const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (c == 0u) {
@@ -1168,11 +1176,12 @@ struct SpKey {
 template <int SW>
 __device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
 {
+    if (__hip_atomic_load(t.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
     for (int i = 0; i < SW; ++i) h = mix64(h ^ key.w[i]);
     uint64_t slot = h & t.mask;
-    for (uint32_t p = 0; p <= t.mask && p < (1u << 16);) {
+    for (uint32_t p = 0; p <= t.mask && p < SPILL_PROBES;) {
         uint32_t* cp = &t.cnt[slot];
         const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (c == 0u) {
@@ -1203,62 +1212,71 @@ __device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
     return false;
 }
 
-// Slot of a key in an LDS table: multiplicative hash of the folded key (slot
-// placement never affects results).
-__device__ __forceinline__ uint32_t slot_hash(uint64_t x)
+// Slot of a key in an LDS table of CAP slots: the top bits of a
+// multiplicative hash of the folded key (slot placement never affects results).
+template <int NW, uint32_t CAP>
+__device__ __forceinline__ uint32_t slot_of(uint64_t xh, uint64_t xl)
 {
-    const uint32_t h = (uint32_t)x ^ (uint32_t)(x >> 29);
-    return (h * 0x9E3779B1u) ^ ((h * 0x85EBCA77u) >> 15);
+    uint32_t h = (uint32_t)xl ^ (uint32_t)(xl >> 29);
+    if (NW == 2) h ^= (uint32_t)xh ^ (uint32_t)(xh >> 32) * 0x85EBCA77u;
+    return (h * 0x9E3779B1u) >> (32 - __builtin_ctz(CAP));
 }
 
-// LDS table, one word per key.  Slots move EMPTY -> key exactly once, so a key
-// absent from a full probe window can never enter the table later: spilling
-// it keeps every occurrence of a key in exactly one table.
-__device__ __forceinline__ bool lds_insert1(uint64_t* keys, uint32_t* cnt, uint32_t mask,
-                                            uint64_t x, uint32_t probe_max)
+// Forward and reverse-complement 2k-bit k-mer whose first base is at bit
+// offset 2*sh of word wi of src (words wi .. wi+2, or wi+4 when NW == 2).
+// NW == 1: (fl, rl), k <= 32; NW == 2: (fh:fl, rh:rl), 32 < k <= 64.
+template <int NW>
+__device__ __forceinline__ void kmer_at(const uint32_t* src, uint32_t wi, uint32_t sh, uint32_t k,
+                                        uint64_t& fh, uint64_t& fl, uint64_t& rh, uint64_t& rl)
 {
-    // one compare-and-swap per probe: it claims an empty slot or returns the
-    // key already there
-    uint32_t slot = slot_hash(x) & mask;
-    uint32_t p = 0;
-    for (;;) {
-        const uint64_t old = atomicCAS((unsigned long long*)&keys[slot], (unsigned long long)EMPTY64,
-                                       (unsigned long long)x);
-        if (old == EMPTY64 || old == x) break;
-        if (++p == probe_max) return false;
-        slot = (slot + 1) & mask;
+    const uint64_t g = 16ull * wi + sh;
+    if (NW == 1) {
+        fl = load_kmer64(src, g, k);
+        rl = (~rev_pairs64(fl)) >> (64u - 2u * k);
+        fh = rh = 0;
+    } else {
+        load_kmer128(src, g, k, fh, fl);
+        rh = ~rev_pairs64(fl);
+        rl = ~rev_pairs64(fh);
+        const uint32_t r = 128u - 2u * k;
+        if (r) { rl = (rl >> r) | (rh << (64u - r)); rh >>= r; }
     }
-    atomicAdd(&cnt[slot], 1u);
-    return true;
 }
 
-// LDS table, two words per key, count word as slot state.
-__device__ __forceinline__ bool lds_insert2(uint64_t* khi, uint64_t* klo, uint32_t* cnt,
-                                            uint32_t mask, uint64_t hi, uint64_t lo,
-                                            uint32_t probe_max)
+// Roll base b (0..3) into the forward k-mer and its complement into the
+// reverse complement.  kmask: the valid bits of the forward word that holds
+// the first base (fl when NW == 1, fh when NW == 2); rsh: the bit offset of
+// the new reverse-complement base in that word.
+template <int NW>
+__device__ __forceinline__ void roll_base(uint64_t& fh, uint64_t& fl, uint64_t& rh, uint64_t& rl,
+                                          uint64_t b, uint64_t kmask, uint32_t rsh)
 {
-    uint32_t slot = slot_hash(lo ^ (hi * 0x9e3779b97f4a7c15ull)) & mask;
-    for (uint32_t p = 0; p < probe_max;) {
-        const uint32_t c = __hip_atomic_load(&cnt[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (c == 0u) {
-            if (atomicCAS(&cnt[slot], 0u, LOCKED32) == 0u) {
-                khi[slot] = hi;
-                klo[slot] = lo;
-                __hip_atomic_store(&cnt[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                return true;
-            }
-            continue;
-        }
-        if (c == LOCKED32) continue;
-        if (khi[slot] == hi && klo[slot] == lo) {
-            atomicAdd(&cnt[slot], 1u);
-            return true;
-        }
-        slot = (slot + 1) & mask;
-        ++p;
+    if (NW == 1) {
+        fl = ((fl << 2) | b) & kmask;
+        rl = (rl >> 2) | ((3u - b) << rsh);
+    } else {
+        fh = ((fh << 2) | (fl >> 62)) & kmask;
+        fl = (fl << 2) | b;
+        rl = (rl >> 2) | (rh << 62);
+        rh = (rh >> 2) | ((3u - b) << rsh);
     }
-    return false;
 }
+
+// Canonical k-mer: the smaller of forward and reverse complement.
+template <int NW>
+__device__ __forceinline__ void canon_pick(uint64_t fh, uint64_t fl, uint64_t rh, uint64_t rl,
+                                           uint64_t& xh, uint64_t& xl)
+{
+    if (NW == 1) {
+        xh = 0;
+        xl = fl < rl ? fl : rl;
+    } else {
+        const bool f = fh < rh || (fh == rh && fl <= rl);
+        xh = f ? fh : rh;
+        xl = f ? fl : rl;
+    }
+}
+
 
 // Block-wide exclusive scan of one u32 per thread (blockDim multiple of 64).
 __device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t& total)
@@ -1296,78 +1314,6 @@ __device__ __forceinline__ void emit_out(const CountArgs& a, uint64_t pos, uint6
     a.out_cnt[pos] = c;
 }
 
-// Rolling k-mer of one lane over words staged in LDS: forward and reverse
-// complement plus a reader positioned at the next base.
-template <int NW>
-struct KRoll {
-    uint64_t fh, fl, rh, rl;   // forward / reverse complement (hi used when NW == 2)
-    uint32_t wi;               // stage index of the word being read
-    uint32_t wrd, rem, nw;     // bits left of that word, their count, the next word
-};
-
-// 2k-bit k-mer whose first base is at bit offset 2*sh of stage word wi.
-template <int NW>
-__device__ __forceinline__ void kroll_init(KRoll<NW>& R, const uint32_t* src, uint32_t wi,
-                                           uint32_t sh, uint32_t k)
-{
-    const uint32_t bsh = 2u * sh;
-    uint64_t a0 = ((uint64_t)src[wi] << 32) | src[wi + 1];
-    const uint32_t c2 = src[wi + 2];
-    if (NW == 1) {
-        const uint64_t top = bsh ? (a0 << bsh) | ((uint64_t)c2 >> (32u - bsh)) : a0;
-        R.fl = top >> (64u - 2u * k);
-        R.fh = 0;
-        R.rl = (~rev_pairs64(R.fl)) >> (64u - 2u * k);
-        R.rh = 0;
-    } else {
-        uint64_t b0 = ((uint64_t)c2 << 32) | src[wi + 3];
-        const uint32_t c4 = src[wi + 4];
-        if (bsh) {
-            a0 = (a0 << bsh) | (b0 >> (64u - bsh));
-            b0 = (b0 << bsh) | ((uint64_t)c4 >> (32u - bsh));
-        }
-        const uint32_t r = 128u - 2u * k;
-        if (r == 0) { R.fh = a0; R.fl = b0; }
-        else { R.fl = (b0 >> r) | (a0 << (64u - r)); R.fh = a0 >> r; }
-        uint64_t rh = ~rev_pairs64(R.fl), rl = ~rev_pairs64(R.fh);
-        if (r) { rl = (rl >> r) | (rh << (64u - r)); rh >>= r; }
-        R.rh = rh;
-        R.rl = rl;
-    }
-    const uint32_t q = sh + k;  // next base, relative to word wi
-    R.wi = wi + (q >> 4);
-    R.wrd = src[R.wi] << (2u * (q & 15u));
-    R.nw = src[R.wi + 1];
-    R.rem = 16u - (q & 15u);
-}
-
-template <int NW>
-__device__ __forceinline__ void kroll_step(KRoll<NW>& R, const uint32_t* src, uint32_t k)
-{
-    if (R.rem == 0) {
-        R.wrd = R.nw;
-        ++R.wi;
-        R.nw = src[R.wi + 1u];
-        R.rem = 16u;
-    }
-    const uint64_t b = R.wrd >> 30;
-    R.wrd <<= 2;
-    --R.rem;
-    const uint64_t cb = 3u - b;
-    if (NW == 1) {
-        const uint64_t mask = k == 32 ? ~0ull : ((1ull << (2u * k)) - 1u);
-        R.fl = ((R.fl << 2) | b) & mask;
-        R.rl = (R.rl >> 2) | (cb << (2u * k - 2u));
-    } else {
-        const uint32_t hb = 2u * k - 64u;  // valid bits of the high word
-        const uint64_t hmask = hb == 64 ? ~0ull : ((1ull << hb) - 1u);
-        R.fh = ((R.fh << 2) | (R.fl >> 62)) & hmask;
-        R.fl = (R.fl << 2) | b;
-        R.rl = (R.rl >> 2) | (R.rh << 62);
-        R.rh = (R.rh >> 2) | (cb << (hb - 2u));
-    }
-}
-
 __device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
 {
 #pragma unroll
@@ -1383,11 +1329,24 @@ template <int NW>
 constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8u; }
 
 // One workgroup per work item.  A wave takes 64 fat records at a time from a
-// workgroup dispenser (one coalesced vector load per lane), puts their bases
-// into its LDS stage at a fixed stride, and then their k-mers form a flat index
-// space (wave scan) cut into 64 equal pieces, one per lane.  A lane finds its
-// first record once (binary search) and rolls through consecutive k-mers from
-// LDS, re-seeding only when it crosses into the next record.
+// workgroup dispenser (one coalesced vector load per lane), compacts the
+// records of the item's sub-partition into its LDS stage at a fixed stride,
+// and then their k-mers form a flat index space (wave scan) cut into 64 equal
+// pieces, one per lane.  A lane finds its first record once (binary search)
+// and rolls through consecutive k-mers from LDS, re-seeding from the next
+// record's first k-mer (computed once per record) when it crosses into it.
+//
+// Tables: NW == 1, one 64-bit word per key claimed by compare-and-swap, plus a
+// count word.  NW == 2, two key words per slot and the count word as the slot
+// state (0 empty, LOCKED32 being written, otherwise the count).  Either way a
+// slot goes from empty to one key exactly once, so a key absent from a full
+// probe window can never enter the table later: spilling it keeps every
+// occurrence of a key in exactly one table.
+//
+// Each loop iteration makes ONE probe for each lane's current k-mer; a lane
+// whose probe ends (counted or spilled) rolls to its next k-mer while the
+// others keep probing, so a wave pays the probes of its lanes' sums, not the
+// worst lane at every k-mer.
 //
 // BATCH: a persistent launch; workgroups take items from a device queue until
 // the device-side item count is reached, spill keys carry the item's job, and
@@ -1403,7 +1362,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ uint32_t s_incl[NWAVE][64];
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint32_t s_red[NWAVE];
-    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_next, s_stop;
     __shared__ uint32_t s_item;
     __shared__ unsigned long long s_base;
 
@@ -1411,6 +1370,11 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     const uint32_t k = a.k;
     constexpr uint32_t RW = NW == 1 ? 4u : 8u, PW = RW - 1u;  // record words, base words
     const uint32_t* stage = s_stage[wv];
+    // roll constants: valid bits of the word holding the first base, and the
+    // bit offset of a new reverse-complement base
+    const uint32_t topb = NW == 1 ? 2u * k : 2u * k - 64u;
+    const uint64_t kmask = topb == 64u ? ~0ull : ((1ull << topb) - 1u);
+    const uint32_t rsh = topb - 2u;
     uint32_t item = blockIdx.x;
     uint32_t n_items = a.n_items;
     if (BATCH) {
@@ -1426,10 +1390,14 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             if (NW == 1) s_k0[i] = EMPTY64;
             s_cnt[i] = 0;
         }
-        if (threadIdx.x == 0) s_next = it.x;
+        if (threadIdx.x == 0) {
+            s_next = it.x;
+            // after a spill overflow the job is counted again: skip the rest
+            s_stop = __hip_atomic_load(a.sp_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         __syncthreads();
 
-        const uint32_t r1 = it.y, smask = it.z, sval = it.w;
+        const uint32_t r1 = s_stop ? it.x : it.y, smask = it.z, sval = it.w;
         uint32_t spilled = 0;
         // records are loaded one batch ahead: the next 64 are in flight while
         // the current ones are counted
@@ -1448,32 +1416,28 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                 rb = __shfl(rn, 0, 64);
                 if (rb + lane < r1) nxt = load_rec<RW>(a.rec, rb + lane);
             }
-            uint32_t nk = 0;
-            if (i < r1) {
-                if (((rr.w[0] >> REC_HSEL_SHIFT) & smask) == sval) {
-                    nk = (rr.w[0] & REC_FAT_LEN_MASK) - k + 1u;
-#pragma unroll
-                    for (uint32_t q = 0; q < PW; ++q) s_stage[wv][lane * PW + q] = rr.w[1 + q];
-                }
-            }
+            // the item's records, compacted to the front of the wave's stage
+            const uint32_t len = rr.w[0] & REC_FAT_LEN_MASK;
+            const bool live = i < r1 && len >= k && ((rr.w[0] >> REC_HSEL_SHIFT) & smask) == sval;
+            const uint32_t nk = live ? len - k + 1u : 0u;
+            const uint64_t lm = __ballot(live);
+            const uint32_t ci = lane_rank(lm), nlive = (uint32_t)__popcll(lm);
             const uint32_t incl = wave_incl_u32(nk, lane);
-            s_incl[wv][lane] = incl;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            {
-                // seed of this lane's own record, computed by every lane at once so
-                // that crossing into a record later is two LDS reads
-                KRoll<NW> Q;
-                kroll_init<NW>(Q, stage, lane * PW, 0u, k);
-                s_seed[wv][lane][0] = Q.fl;
-                s_seed[wv][lane][1] = Q.rl;
+            const uint32_t total = __shfl(incl, 63, 64);
+            if (live) {
+#pragma unroll
+                for (uint32_t q = 0; q < PW; ++q) s_stage[wv][ci * PW + q] = rr.w[1 + q];
+                s_incl[wv][ci] = incl;
+                uint64_t fh, fl, rh, rl;  // the record's first k-mer, from registers
+                kmer_at<NW>(rr.w + 1, 0u, 0u, k, fh, fl, rh, rl);
+                s_seed[wv][ci][0] = fl;
+                s_seed[wv][ci][1] = rl;
                 if (NW == 2) {
-                    s_seed[wv][lane][NW == 2 ? 2 : 0] = Q.fh;
-                    s_seed[wv][lane][NW == 2 ? 3 : 1] = Q.rh;
+                    s_seed[wv][ci][NW == 2 ? 2 : 0] = fh;
+                    s_seed[wv][ci][NW == 2 ? 3 : 1] = rh;
                 }
             }
-            const uint32_t total = __shfl(incl, 63, 64);
+            if (lane >= nlive) s_incl[wv][lane] = total;  // keeps the search in range
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1486,49 +1450,97 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                     if (s_incl[wv][s + step - 1u] <= j0) s += step;
                 const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
                 uint32_t end = s_incl[wv][s];
-                KRoll<NW> R;
-                {
-                    const uint32_t gb = j0 - ex;  // k-mer offset inside record s
-                    kroll_init<NW>(R, stage, s * PW + (gb >> 4), gb & 15u, k);
-                }
+                const uint32_t gb = j0 - ex;  // k-mer offset inside record s
+                uint64_t fh, fl, rh, rl;
+                kmer_at<NW>(stage, s * PW + (gb >> 4), gb & 15u, k, fh, fl, rh, rl);
+                uint32_t pos = s * (16u * PW) + gb + k;  // stage base rolled in next
+                uint64_t xh, xl;
+                canon_pick<NW>(fh, fl, rh, rl, xh, xl);
+                uint32_t slot = slot_of<NW, CAP>(xh, xl), p = 0;
+                // settle the set-up reads here: a read still in flight at the loop
+                // entry would otherwise be waited for inside every iteration
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
                 for (uint32_t j = j0;;) {
-                    bool ok;
-                    if (NW == 1) {
-                        const uint64_t x = R.fl < R.rl ? R.fl : R.rl;
-                        ok = lds_insert1(s_k0, s_cnt, CAP - 1u, x, a.probe_max);
-                        if (!ok) {
-                            if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
-                            else spill_insert1(spill_ref(a), x);
-                        }
-                    } else {
-                        const bool f = R.fh < R.rh || (R.fh == R.rh && R.fl <= R.rl);
-                        const uint64_t xh = f ? R.fh : R.rh, xl = f ? R.fl : R.rl;
-                        ok = lds_insert2(s_k0, s_k1, s_cnt, CAP - 1u, xh, xl, a.probe_max);
-                        if (!ok) {
-                            if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
-                            else spill_insert2(spill_ref(a), xh, xl);
+                    // the LDS reads an advance may need are issued with the probe,
+                    // so that one wait covers them all
+                    const uint32_t wn = stage[pos >> 4];
+                    const bool xing = j + 1u == end;  // advancing would cross
+                    uint32_t nend = 0;
+                    uint64_t nfl = 0, nrl = 0, nfh = 0, nrh = 0;
+                    if (xing) {
+                        const uint32_t sn = min(s + 1u, 63u);
+                        nend = s_incl[wv][sn];
+                        nfl = s_seed[wv][sn][0];
+                        nrl = s_seed[wv][sn][1];
+                        if (NW == 2) {
+                            nfh = s_seed[wv][sn][NW == 2 ? 2 : 0];
+                            nrh = s_seed[wv][sn][NW == 2 ? 3 : 1];
                         }
                     }
-                    spilled += ok ? 0u : 1u;
-                    if (++j == j1) break;
-                    if (j == end) {
-                        // next record with k-mers (filtered records have none)
-                        do {
-                            ++s;
-                            end = s_incl[wv][s];
-                        } while (end == j);
-                        R.fl = s_seed[wv][s][0];
-                        R.rl = s_seed[wv][s][1];
-                        if (NW == 2) {
-                            R.fh = s_seed[wv][s][NW == 2 ? 2 : 0];
-                            R.rh = s_seed[wv][s][NW == 2 ? 3 : 1];
+                    bool adv = false, miss = false;
+                    if (NW == 1) {
+                        const uint64_t old = atomicCAS((unsigned long long*)&s_k0[slot],
+                                                       (unsigned long long)EMPTY64, (unsigned long long)xl);
+                        if (old == EMPTY64 || old == xl) {
+                            atomicAdd(&s_cnt[slot], 1u);
+                            adv = true;
+                        } else {
+                            miss = true;
                         }
-                        R.wi = s * PW + (k >> 4);
-                        R.wrd = stage[R.wi] << (2u * (k & 15u));
-                        R.nw = stage[R.wi + 1];
-                        R.rem = 16u - (k & 15u);
                     } else {
-                        kroll_step<NW>(R, stage, k);
+                        const uint32_t c = __hip_atomic_load(&s_cnt[slot], __ATOMIC_ACQUIRE,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (c == 0u) {
+                            // claim; a lost race retries the same slot next time
+                            if (atomicCAS(&s_cnt[slot], 0u, LOCKED32) == 0u) {
+                                s_k0[slot] = xh;
+                                s_k1[slot] = xl;
+                                __hip_atomic_store(&s_cnt[slot], 1u, __ATOMIC_RELEASE,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                                adv = true;
+                            }
+                        } else if (c != LOCKED32) {
+                            if (s_k0[slot] == xh && s_k1[slot] == xl) {
+                                atomicAdd(&s_cnt[slot], 1u);
+                                adv = true;
+                            } else {
+                                miss = true;
+                            }
+                        }
+                    }
+                    if (miss) {
+                        if (++p == a.probe_max) {
+                            if (NW == 1) {
+                                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, xl, 0ull}});
+                                else spill_insert1(spill_ref(a), xl);
+                            } else {
+                                if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
+                                else spill_insert2(spill_ref(a), xh, xl);
+                            }
+                            ++spilled;
+                            adv = true;
+                        } else {
+                            slot = (slot + 1u) & (CAP - 1u);
+                        }
+                    }
+                    if (adv) {
+                        if (++j == j1) break;
+                        if (xing) {
+                            ++s;
+                            end = nend;
+                            fh = nfh;
+                            fl = nfl;
+                            rh = nrh;
+                            rl = nrl;
+                            pos = s * (16u * PW) + k;
+                        } else {
+                            const uint64_t b = (wn >> (30u - 2u * (pos & 15u))) & 3u;
+                            ++pos;
+                            roll_base<NW>(fh, fl, rh, rl, b, kmask, rsh);
+                        }
+                        canon_pick<NW>(fh, fl, rh, rl, xh, xl);
+                        slot = slot_of<NW, CAP>(xh, xl);
+                        p = 0;
                     }
                 }
             }
