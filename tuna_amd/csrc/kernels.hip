@@ -1249,16 +1249,29 @@ __device__ __forceinline__ void kmer_at(const uint32_t* src, uint32_t wi, uint32
 // the new reverse-complement base in that word.
 template <int NW>
 __device__ __forceinline__ void roll_base(uint64_t& fh, uint64_t& fl, uint64_t& rh, uint64_t& rl,
-                                          uint64_t b, uint64_t kmask, uint32_t rsh)
+                                          uint32_t b, uint64_t kmask, uint32_t rsh)
 {
     if (NW == 1) {
-        fl = ((fl << 2) | b) & kmask;
-        rl = (rl >> 2) | ((3u - b) << rsh);
+        // 32-bit halves: funnel shifts instead of 64-bit shifts and masks
+        uint32_t flo = (uint32_t)fl, fhi = (uint32_t)(fl >> 32);
+        uint32_t rlo = (uint32_t)rl, rhi = (uint32_t)(rl >> 32);
+        const uint32_t cb = b ^ 3u;
+        if (rsh >= 32u) {  // k >= 17: the low forward word needs no mask
+            fhi = __builtin_amdgcn_alignbit(fhi, flo, 30u) & (uint32_t)(kmask >> 32);
+            flo = (flo << 2) | b;
+            rlo = __builtin_amdgcn_alignbit(rhi, rlo, 2u);
+            rhi = (rhi >> 2) | (cb << (rsh - 32u));
+        } else {           // k <= 16: both fit in the low word
+            flo = ((flo << 2) | b) & (uint32_t)kmask;
+            rlo = (rlo >> 2) | (cb << rsh);
+        }
+        fl = ((uint64_t)fhi << 32) | flo;
+        rl = ((uint64_t)rhi << 32) | rlo;
     } else {
         fh = ((fh << 2) | (fl >> 62)) & kmask;
         fl = (fl << 2) | b;
         rl = (rl >> 2) | (rh << 62);
-        rh = (rh >> 2) | ((3u - b) << rsh);
+        rh = (rh >> 2) | ((uint64_t)(b ^ 3u) << rsh);
     }
 }
 
@@ -1338,10 +1351,12 @@ constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8
 //
 // Tables: NW == 1, one 64-bit word per key claimed by compare-and-swap, plus a
 // count word.  NW == 2, two key words per slot and the count word as the slot
-// state (0 empty, LOCKED32 being written, otherwise the count).  Either way a
-// slot goes from empty to one key exactly once, so a key absent from a full
-// probe window can never enter the table later: spilling it keeps every
-// occurrence of a key in exactly one table.
+// state (0 empty, LOCKED32 being written, otherwise the count).  A key probes
+// slot h, h+1, h+3, h+6, ... (triangular steps, which visit every slot of a
+// power-of-two table and break up the clusters of linear probing).  Either
+// way a slot goes from empty to one key exactly once, so a key absent from
+// its full probe window can never enter the table later: spilling it keeps
+// every occurrence of a key in exactly one table.
 //
 // Each loop iteration makes ONE probe for each lane's current k-mer; a lane
 // whose probe ends (counted or spilled) rolls to its next k-mer while the
@@ -1453,7 +1468,9 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                 const uint32_t gb = j0 - ex;  // k-mer offset inside record s
                 uint64_t fh, fl, rh, rl;
                 kmer_at<NW>(stage, s * PW + (gb >> 4), gb & 15u, k, fh, fl, rh, rl);
-                uint32_t pos = s * (16u * PW) + gb + k;  // stage base rolled in next
+                // stage bit index, plus one, of the base rolled in next (odd, so
+                // that its complement's low five bits give the base's shift)
+                uint32_t pc = 2u * (s * (16u * PW) + gb + k) + 1u;
                 uint64_t xh, xl;
                 canon_pick<NW>(fh, fl, rh, rl, xh, xl);
                 uint32_t slot = slot_of<NW, CAP>(xh, xl), p = 0;
@@ -1463,10 +1480,10 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                 for (uint32_t j = j0;;) {
                     // the LDS reads an advance may need are issued with the probe,
                     // so that one wait covers them all
-                    const uint32_t wn = stage[pos >> 4];
+                    const uint32_t wn = stage[pc >> 5];
                     const bool xing = j + 1u == end;  // advancing would cross
-                    uint32_t nend = 0;
-                    uint64_t nfl = 0, nrl = 0, nfh = 0, nrh = 0;
+                    uint32_t nend;
+                    uint64_t nfl, nrl, nfh, nrh;  // read only when xing
                     if (xing) {
                         const uint32_t sn = min(s + 1u, 63u);
                         nend = s_incl[wv][sn];
@@ -1520,7 +1537,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                             ++spilled;
                             adv = true;
                         } else {
-                            slot = (slot + 1u) & (CAP - 1u);
+                            slot = (slot + p) & (CAP - 1u);  // triangular steps
                         }
                     }
                     if (adv) {
@@ -1528,14 +1545,16 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                         if (xing) {
                             ++s;
                             end = nend;
-                            fh = nfh;
                             fl = nfl;
-                            rh = nrh;
                             rl = nrl;
-                            pos = s * (16u * PW) + k;
+                            if (NW == 2) {
+                                fh = nfh;
+                                rh = nrh;
+                            }
+                            pc = 2u * (s * (16u * PW) + k) + 1u;
                         } else {
-                            const uint64_t b = (wn >> (30u - 2u * (pos & 15u))) & 3u;
-                            ++pos;
+                            const uint32_t b = __builtin_amdgcn_ubfe(wn, ~pc, 2u);
+                            pc += 2u;
                             roll_base<NW>(fh, fl, rh, rl, b, kmask, rsh);
                         }
                         canon_pick<NW>(fh, fl, rh, rl, xh, xl);
