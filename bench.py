@@ -290,6 +290,42 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
     }
 
 
+def bench_output(T, eng, k):
+    """TSV output of the last result: device formatting (alone, then with the
+    copy into host memory) against the single-threaded host formatter."""
+    lib, h = T.lib(), eng._h
+    n = int(lib.tg_result_size(h))
+    size = int(lib.tg_result_tsv(h, k, 0, n, None, 0))
+    buf = np.empty(max(size, 1), dtype=np.uint8)
+    ptr = buf.ctypes.data_as(T.ctypes.c_char_p)
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+
+    t_fmt = timed(lambda: lib.tg_result_tsv(h, k, 0, n, None, 0), 5)
+    t_dev = timed(lambda: lib.tg_result_tsv(h, k, 0, n, ptr, size), 5)
+    keys, cnt = eng.result(k)
+    ka = np.ascontiguousarray(keys.reshape(-1))
+    hbuf = np.empty(max(n, 1) * (k + 12), dtype=np.uint8)
+    hres = []
+    t_host = timed(lambda: hres.append(lib.tg_format_tsv(k, ka.ctypes.data_as(T._u64p),
+                                                          cnt.ctypes.data_as(T._u32p), n,
+                                                          hbuf.ctypes.data_as(T.ctypes.c_char_p),
+                                                          len(hbuf))), 3)
+    same = hres[-1] == size and bytes(buf[:size]) == bytes(hbuf[:size])
+    return {"lines": n, "bytes": size, "identical_to_host": bool(same),
+            "device_format_ms": round(t_fmt * 1e3, 3), "device_format_gb_s": round(size / t_fmt / 1e9, 2),
+            "device_format_and_copy_ms": round(t_dev * 1e3, 3),
+            "device_format_and_copy_gb_s": round(size / t_dev / 1e9, 2),
+            "host_format_ms": round(t_host * 1e3, 3), "host_format_gb_s": round(size / t_host / 1e9, 2),
+            "note": "TSV of the job's result; host = one thread; copy = pageable host memory"}
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload, from the committed PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel)."""
@@ -427,6 +463,7 @@ def main():
         "engine_config": dict(tuned, autotune_s=round(t_tune, 2)),
         "roofline": roof,
     }
+    out["output"] = bench_output(T, eng, K)
     if args.samples > 0 and args.only == "all":
         out["batched"] = bench_batched(T, eng, args.samples, threads, not args.no_graph)
     if args.requests > 0 and args.only == "all":

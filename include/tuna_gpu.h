@@ -283,6 +283,23 @@ int64_t tg_format_tsv(uint32_t k, const uint64_t* keys, const uint32_t* counts, 
 int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint32_t* counts,
                  uint64_t n);
 
+/* The same formats built on the device from the context's last single-job
+ * result (src/count.hpp:180-265 write_counts / write_counts_kff, whose
+ * per-k-mer formatting they replace); the host only copies bytes.
+ *   tg_result_tsv: TSV lines of result entries [first, first + n) into out
+ *     (cap bytes); returns the byte count, TG_ECAP if it exceeds cap, or with
+ *     out == NULL just the byte count.
+ *   tg_result_kff: the KFF raw-section records of those entries, ceil(k/4)
+ *     key bytes + 4-byte big-endian count each (no file header).
+ *   tg_result_write: the whole result as a TSV file (kff = 0) or a KFF file
+ *     (kff != 0, the bytes of tg_write_kff), formatted on the device in chunks
+ *     whose copies overlap the file writes.
+ * k must match the result's key width; output order is the result order. */
+int64_t tg_result_tsv(tg_ctx* ctx, uint32_t k, uint64_t first, uint64_t n, char* out, uint64_t cap);
+int64_t tg_result_kff(tg_ctx* ctx, uint32_t k, uint64_t first, uint64_t n, uint8_t* out,
+                      uint64_t cap);
+int tg_result_write(tg_ctx* ctx, uint32_t k, const char* path, int kff);
+
 /* ---- synthetic inputs (seeded, deterministic) ------------------------- */
 /* Generate the FASTA/FASTQ text of a named workload into a library-owned
  * buffer: "genome_fa" (uniform genome), "reads_fq" (short reads from a uniform

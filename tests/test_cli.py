@@ -258,6 +258,16 @@ def test_cli_empty_input_error(tmp_path):
     assert r.returncode == 1 and f"tuna: error: Empty file: {e}" in r.stderr
 
 
+@pytest.mark.gpu
+def test_cli_unwritable_output_reports_reference_message(tmp_path):
+    f = tmp_path / "g.fa"
+    f.write_text(">g\n" + "ACGTTGCA" * 50 + "\n")
+    for out in (tmp_path / "no_dir" / "o.tsv", tmp_path / "no_dir" / "o.kff"):
+        r = run_cli("-k", "21", "-m", "11", str(f), str(out))
+        # src/pipeline.hpp:151
+        assert r.returncode == 1 and f"tuna: error: cannot open output file: {out}" in r.stderr
+
+
 CPP_TEST = r'''
 #include "tuna/tuna.hpp"
 #include <cstdio>

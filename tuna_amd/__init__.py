@@ -35,6 +35,7 @@ EXPORTS = [
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
+    "tg_result_tsv", "tg_result_kff", "tg_result_write",
 ]
 
 
@@ -163,6 +164,13 @@ def lib():
     L.tg_format_tsv.argtypes = [ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, ctypes.c_char_p,
                                 ctypes.c_uint64]
     L.tg_write_kff.argtypes = [ctypes.c_char_p, ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64]
+    L.tg_result_tsv.restype = ctypes.c_int64
+    L.tg_result_tsv.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p,
+                                ctypes.c_uint64]
+    L.tg_result_kff.restype = ctypes.c_int64
+    L.tg_result_kff.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, _u8p,
+                                ctypes.c_uint64]
+    L.tg_result_write.argtypes = [_vp, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_int]
     L.tg_synth.restype = ctypes.c_int64
     L.tg_synth.argtypes = [ctypes.c_char_p]
     L.tg_synth_copy.argtypes = [_u8p, ctypes.c_uint64]
@@ -210,6 +218,13 @@ def _p(a, t):
 def _check(rc):
     if rc != 0:
         raise TunaGpuError(f"tuna_gpu error {rc}: {lib().tg_last_error().decode()}")
+
+
+def _check_size(v):
+    """Entry points that return a byte count: negative values are errors."""
+    if v < 0:
+        _check(v)
+    return v
 
 
 def _u8(x):
@@ -472,6 +487,28 @@ class Engine:
         cnt = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib().tg_result_copy(self._h, _p(keys, _u64p), _p(cnt, _u32p), n))
         return keys[: n * nw].reshape(n, nw), cnt[:n]
+
+    def result_tsv(self, k, first=0, n=None):
+        """TSV bytes of result entries [first, first+n), formatted on the device."""
+        if n is None:
+            n = lib().tg_result_size(self._h) - first
+        size = _check_size(lib().tg_result_tsv(self._h, k, first, n, None, 0))
+        buf = ctypes.create_string_buffer(max(size, 1))
+        got = _check_size(lib().tg_result_tsv(self._h, k, first, n, buf, size))
+        return buf.raw[:got]
+
+    def result_kff(self, k, first=0, n=None):
+        """KFF raw-section records of result entries [first, first+n)."""
+        if n is None:
+            n = lib().tg_result_size(self._h) - first
+        size = _check_size(lib().tg_result_kff(self._h, k, first, n, None, 0))
+        out = np.zeros(max(size, 1), dtype=np.uint8)
+        got = _check_size(lib().tg_result_kff(self._h, k, first, n, _p(out, _u8p), size))
+        return out[:got].tobytes()
+
+    def result_write(self, k, path, kff=False):
+        """Write the whole result as a TSV or KFF file, formatted on the device."""
+        _check(lib().tg_result_write(self._h, k, os.fsencode(path), 1 if kff else 0))
 
     # -- many jobs per launch sequence
     def batch_config(self):

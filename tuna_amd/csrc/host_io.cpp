@@ -5,6 +5,7 @@
 // (src/seq_source.hpp:33-39): headers ignored, sequence lines joined,
 // every byte outside [ACGTacgt] ends a chunk.
 #include "../../include/tuna_gpu.h"
+#include "kff.h"
 
 #include <sys/resource.h>
 #include <sys/stat.h>
@@ -362,6 +363,51 @@ void put_be(std::string& o, uint64_t v, int bytes)
 
 }  // namespace
 
+namespace tgi {
+
+std::string kff_header(uint32_t k, uint64_t n)
+{
+    std::string h;
+    h += "KFF";
+    h += (char)1;                                   // version 1.0
+    h += (char)0;
+    h += (char)((0 << 6) | (1 << 4) | (2 << 2) | 3);  // A=0 C=1 G=2 T=3
+    h += (char)1;                                   // unique
+    h += (char)1;                                   // canonical
+    put_be(h, 0, 4);                                // no metadata
+    // global variables: k, max = 1, data_size = 4
+    h += 'v';
+    put_be(h, 3, 8);
+    for (const auto& kv : {std::make_pair("k", (uint64_t)k), std::make_pair("max", (uint64_t)1),
+                           std::make_pair("data_size", (uint64_t)4)}) {
+        h += kv.first;
+        h += '\0';
+        put_be(h, kv.second, 8);
+    }
+    // raw section: one block per k-mer (max = 1: no per-block count)
+    h += 'r';
+    put_be(h, n, 8);
+    return h;
+}
+
+std::string kff_footer()
+{
+    std::string t;
+    t += 'v';
+    put_be(t, 2, 8);
+    t += "first_index";
+    t += '\0';
+    put_be(t, 0, 8);
+    t += "footer_size";
+    t += '\0';
+    const uint64_t footer = 1 + 8 + 12 + 8 + 12 + 8 + 3;
+    put_be(t, footer, 8);
+    t += "KFF";
+    return t;
+}
+
+}  // namespace tgi
+
 extern "C" {
 
 int tg_read_file(const char* path, uint8_t** data, uint64_t* len)
@@ -518,26 +564,7 @@ int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint3
         set_host_error(std::string("cannot open output file: ") + path);
         return TG_EINPUT;
     }
-    std::string h;
-    h += "KFF";
-    h += (char)1;                                   // version 1.0
-    h += (char)0;
-    h += (char)((0 << 6) | (1 << 4) | (2 << 2) | 3);  // A=0 C=1 G=2 T=3
-    h += (char)1;                                   // unique
-    h += (char)1;                                   // canonical
-    put_be(h, 0, 4);                                // no metadata
-    // global variables: k, max = 1, data_size = 4
-    h += 'v';
-    put_be(h, 3, 8);
-    for (const auto& kv : {std::make_pair("k", (uint64_t)k), std::make_pair("max", (uint64_t)1),
-                           std::make_pair("data_size", (uint64_t)4)}) {
-        h += kv.first;
-        h += '\0';
-        put_be(h, kv.second, 8);
-    }
-    // raw section: one block per k-mer (max = 1: no per-block count)
-    h += 'r';
-    put_be(h, n, 8);
+    const std::string h = tgi::kff_header(k, n);
     fwrite(h.data(), 1, h.size(), f);
     const uint32_t kb = (k + 3) / 4, nw = k > 32 ? 2 : 1;
     std::vector<uint8_t> buf;
@@ -561,18 +588,7 @@ int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint3
         }
     }
     if (!buf.empty()) fwrite(buf.data(), 1, buf.size(), f);
-    // footer: a variables section locating the first section, then the magic
-    std::string t;
-    t += 'v';
-    put_be(t, 2, 8);
-    t += "first_index";
-    t += '\0';
-    put_be(t, 0, 8);
-    t += "footer_size";
-    t += '\0';
-    const uint64_t footer = 1 + 8 + 12 + 8 + 12 + 8 + 3;
-    put_be(t, footer, 8);
-    t += "KFF";
+    const std::string t = tgi::kff_footer();
     fwrite(t.data(), 1, t.size(), f);
     const bool ok = ferror(f) == 0;
     fclose(f);

@@ -335,31 +335,11 @@ int run(Config& cfg)
         std::cerr << "[2/2] counting  (gfx950, in-memory) ...\n";
     }
     const auto t2 = std::chrono::steady_clock::now();
-    const uint64_t n = tg_result_size(ctx);
-    const uint32_t nw = cfg.k > 32 ? 2 : 1;
-    std::vector<uint64_t> keys(n * nw + 1);
-    std::vector<uint32_t> counts(n + 1);
-    check(tg_result_copy(ctx, keys.data(), counts.data(), n));
-    if (cfg.output_kff) {
-        check(tg_write_kff(cfg.output_file.c_str(), cfg.k, keys.data(), counts.data(), n));
-    } else {
-        std::ofstream out(cfg.output_file, std::ios::binary);
-        if (!out) {
-            std::cerr << "tuna: error: cannot open output file: " << cfg.output_file << "\n";
-            return 1;
-        }
-        constexpr uint64_t BATCH = 1u << 16;
-        std::vector<char> buf(BATCH * (cfg.k + 12));
-        for (uint64_t i = 0; i < n; i += BATCH) {
-            const uint64_t m = std::min(BATCH, n - i);
-            const int64_t w = tg_format_tsv(cfg.k, keys.data() + i * nw, counts.data() + i, m, buf.data(), buf.size());
-            if (w < 0) throw std::runtime_error("TSV formatting failed");
-            out.write(buf.data(), w);
-        }
-        if (!out) {
-            std::cerr << "tuna: error: failed while writing output file: " << cfg.output_file << "\n";
-            return 1;
-        }
+    // the counts are formatted on the device (TSV lines or KFF records) and
+    // only their bytes cross to the host, overlapping the file writes
+    if (tg_result_write(ctx, cfg.k, cfg.output_file.c_str(), cfg.output_kff ? 1 : 0) != TG_OK) {
+        std::cerr << "tuna: error: " << tg_last_error() << "\n";
+        return 1;
     }
     if (cfg.keep_tmp && write_partition_files(cfg, ctx, p, h)) return 1;
     const double t_phase2 = st.ms_phase2 * 1e-3 + since(t2);
