@@ -1088,6 +1088,8 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n)
 
 // ========================================================== counting ====
 constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
+// never in a one-word table: slots hold EMPTY64 or a key below 2^64 - 2^62
+constexpr uint64_t NOMATCH64 = 0xFFFFFFFFFFFFFFFEull;
 constexpr uint32_t SPILL_PROBES = 4096;
 
 // Spill-table view passed by value to the out-of-line insert paths (a
@@ -1337,6 +1339,62 @@ __device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
     return v;
 }
 
+// Keys waiting on a wave's stack for their next probe (k <= 32): the count
+// loop makes exactly one probe per k-mer and pushes the misses, so its lanes
+// stay in step; the stack is then worked off 64 entries per pass, one probe
+// per entry, the misses pushed back one step further along their sequence.
+constexpr uint32_t DEFER_CAP = 128;  // entries per wave (>= 64 + 64)
+constexpr uint32_t DEFER_PSH = 12;   // entry state: slot | probes << 12 (CAP <= 4096)
+
+template <uint32_t CAP, bool BATCH>
+__device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, uint64_t* qk,
+                                                 uint32_t* qs, uint32_t qn, bool all,
+                                                 const CountArgs& a, uint32_t tag, uint32_t& spilled,
+                                                 uint32_t lane)
+{
+    static_assert(CAP <= (1u << DEFER_PSH), "slot field");
+    while (qn >= 64u || (all && qn)) {
+        const uint32_t take = min(qn, 64u), base = qn - take;
+        const bool has = lane < take;
+        uint64_t x = 0;
+        uint32_t e = 0;
+        if (has) {
+            x = qk[base + lane];
+            e = qs[base + lane];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t slot = e & ((1u << DEFER_PSH) - 1u), p = e >> DEFER_PSH;
+        bool again = false;
+        if (has) {
+            const uint64_t old = atomicCAS((unsigned long long*)&keys[slot], (unsigned long long)EMPTY64,
+                                           (unsigned long long)x);
+            if (old == EMPTY64 || old == x) {
+                atomicAdd(&cnt[slot], 1u);
+            } else if (++p == a.probe_max) {
+                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
+                else spill_insert1(spill_ref(a), x);
+                ++spilled;
+            } else {
+                again = true;
+                slot = (slot + p) & (CAP - 1u);  // triangular steps
+            }
+        }
+        const uint64_t am = __ballot(again);
+        if (again) {
+            const uint32_t at = base + lane_rank(am);
+            qk[at] = x;
+            qs[at] = slot | (p << DEFER_PSH);
+        }
+        qn = base + (uint32_t)__popcll(am);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return qn;
+}
+
 // packed words staged per wave: 64 records x (RW-1) words, + 1 read ahead
 template <int NW>
 constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8u; }
@@ -1376,6 +1434,8 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ uint32_t s_stage[NWAVE][count_stage<NW>()];
     __shared__ uint32_t s_incl[NWAVE][64];
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
+    __shared__ uint64_t s_qk[NWAVE][NW == 1 ? DEFER_CAP : 1];  // deferred keys (k <= 32)
+    __shared__ uint32_t s_qs[NWAVE][NW == 1 ? DEFER_CAP : 1];  // their probe state
     __shared__ uint32_t s_red[NWAVE];
     __shared__ uint32_t s_next, s_stop;
     __shared__ uint32_t s_item;
@@ -1414,6 +1474,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 
         const uint32_t r1 = s_stop ? it.x : it.y, smask = it.z, sval = it.w;
         uint32_t spilled = 0;
+        uint32_t qn = 0;  // deferred keys on this wave's stack (k <= 32)
         // records are loaded one batch ahead: the next 64 are in flight while
         // the current ones are counted
         uint32_t rb = 0;
@@ -1458,7 +1519,93 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const uint32_t q = (total + 63u) >> 6;
             const uint32_t j0 = lane * q, j1 = min(j0 + q, total);
-            if (j0 < j1) {
+            if constexpr (NW == 1) {
+                // k <= 32: every lane advances every iteration; a probe that
+                // misses moves the key to the wave's stack
+                const uint32_t nmine = j1 > j0 ? j1 - j0 : 0u;
+                uint32_t s = 0, end = 0, pc = 1u;
+                uint64_t fh = 0, fl = 0, rh = 0, rl = 0;
+                if (nmine) {
+#pragma unroll
+                    for (uint32_t step = 32; step; step >>= 1)
+                        if (s_incl[wv][s + step - 1u] <= j0) s += step;
+                    const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
+                    end = s_incl[wv][s];
+                    const uint32_t gb = j0 - ex;
+                    kmer_at<1>(stage, s * PW + (gb >> 4), gb & 15u, k, fh, fl, rh, rl);
+                    pc = 2u * (s * (16u * PW) + gb + k) + 1u;
+                }
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): settle the set-up reads
+                // A probe result is used one iteration after the probe is issued,
+                // so two probes per wave are in flight.  hit: count; miss: push
+                // the key with its next slot.
+                auto settle = [&](bool pa, uint64_t px, uint32_t ps, uint64_t po) {
+                    const bool hit = pa && (po == EMPTY64 || po == px);
+                    const bool miss = pa && !hit;
+                    if (hit) atomicAdd(&s_cnt[ps], 1u);
+                    const uint64_t mm = __ballot(miss);
+                    if (mm) {
+                        if (miss) {
+                            const uint32_t at = qn + lane_rank(mm);
+                            s_qk[wv][at] = px;
+                            s_qs[wv][at] = ((ps + 1u) & (CAP - 1u)) | (1u << DEFER_PSH);
+                        }
+                        qn += (uint32_t)__popcll(mm);
+                    }
+                    if (qn > DEFER_CAP - 64u)
+                        qn = defer_passes<CAP, BATCH>(s_k0, s_cnt, s_qk[wv], s_qs[wv], qn, false, a, tag,
+                                                      spilled, lane);
+                };
+                // k-mer i: issue its probe, then roll the lane on to k-mer i+1
+                auto step = [&](uint32_t i, uint64_t& x, uint32_t& slot, uint64_t& old, bool& act) {
+                    act = i < nmine;
+                    const uint32_t wn = stage[pc >> 5];
+                    const bool xing = act && j0 + i + 1u == end && i + 1u < nmine;
+                    uint32_t nend;
+                    uint64_t nfl, nrl;  // read only when xing
+                    if (xing) {
+                        const uint32_t sn = min(s + 1u, 63u);
+                        nend = s_incl[wv][sn];
+                        nfl = s_seed[wv][sn][0];
+                        nrl = s_seed[wv][sn][1];
+                    }
+                    x = fl < rl ? fl : rl;
+                    slot = slot_of<1, CAP>(0ull, x);
+                    // every lane probes; a lane without a k-mer compares against
+                    // a value no slot holds and writes nothing
+                    old = atomicCAS((unsigned long long*)&s_k0[slot],
+                                    (unsigned long long)(act ? EMPTY64 : NOMATCH64), (unsigned long long)x);
+                    const uint32_t b = __builtin_amdgcn_ubfe(wn, ~pc, 2u);
+                    if (act && i + 1u < nmine) {
+                        if (xing) {
+                            ++s;
+                            end = nend;
+                            fl = nfl;
+                            rl = nrl;
+                            pc = 2u * (s * (16u * PW) + k) + 1u;
+                        } else {
+                            pc += 2u;
+                            roll_base<1>(fh, fl, rh, rl, b, kmask, rsh);
+                        }
+                    }
+                };
+                // two k-mers per trip with their own result registers, so a probe
+                // result crosses the loop edge without a copy (which would wait)
+                uint64_t xB = 0, oB = 0;
+                uint32_t sB = 0;
+                bool aB = false;
+                for (uint32_t i = 0; i < q; i += 2u) {
+                    uint64_t xA, oA;
+                    uint32_t sA;
+                    bool aA;
+                    step(i, xA, sA, oA, aA);
+                    settle(aB, xB, sB, oB);
+                    aB = false;
+                    if (i + 1u < q) step(i + 1u, xB, sB, oB, aB);
+                    settle(aA, xA, sA, oA);
+                }
+                settle(aB, xB, sB, oB);
+            } else if (j0 < j1) {
                 uint32_t s = 0;
 #pragma unroll
                 for (uint32_t step = 32; step; step >>= 1)
@@ -1566,6 +1713,9 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if constexpr (NW == 1) {
+            if (qn) defer_passes<CAP, BATCH>(s_k0, s_cnt, s_qk[wv], s_qs[wv], qn, true, a, tag, spilled, lane);
         }
         __syncthreads();
 
