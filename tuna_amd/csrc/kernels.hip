@@ -386,13 +386,17 @@ __device__ __forceinline__ Rec<RW> fat_record(const uint32_t* packed, uint32_t s
 // fat record gathers its bases, outside the divergent per-k-mer loop.
 struct SkStage {
     uint2* buf;        // this wave's buffer (LDS)
-    uint32_t* wn;      // this wave's fill count (LDS)
     uint32_t* wg_n;    // workgroup run fill (LDS)
     uint64_t run;      // workgroup run: first record index
+    // The wave's fill is pushed - flushed, kept in registers.  Both counters
+    // only grow, and every lane still in the loop has taken part in every
+    // push and flush so far, so the active lanes agree; a lane that left the
+    // loop keeps older values, and the wave's counts are then the maxima.
+    uint32_t pushed, flushed;
 };
 
 template <int RW>
-__device__ __forceinline__ void sk_flush(const SkStage& S, const SkArgs& a, uint32_t cnt)
+__device__ __forceinline__ void sk_flush(SkStage& S, const SkArgs& a, uint32_t cnt)
 {
     const uint64_t act = __ballot(1);
     const uint32_t nact = (uint32_t)__popcll(act), rank = lane_rank(act);
@@ -411,25 +415,29 @@ __device__ __forceinline__ void sk_flush(const SkStage& S, const SkArgs& a, uint
         }
         a.metas[S.run + base + i] = e.y;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    *S.wn = 0;
-    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    S.flushed = S.pushed;
 }
 
 template <int RW>
-__device__ __forceinline__ void sk_push(const SkStage& S, const SkArgs& a, bool e, uint2 rec)
+__device__ __forceinline__ void sk_push(SkStage& S, const SkArgs& a, bool e, uint2 rec)
 {
     const uint64_t mask = __ballot(e);
     if (!mask) return;
-    const uint32_t base = __builtin_amdgcn_readfirstlane(*S.wn);
+    const uint32_t base = S.pushed - S.flushed;
     // lanes that do not emit write the scratch slot past the buffer end
     const uint32_t slot = e ? base + lane_rank(mask) : SK_WAVE_BUF + (threadIdx.x & 63u);
     S.buf[slot] = rec;
-    const uint32_t cnt = base + (uint32_t)__popcll(mask);
-    __builtin_amdgcn_wave_barrier();
-    *S.wn = cnt;
-    __builtin_amdgcn_wave_barrier();
-    if (cnt > SK_WAVE_BUF - 64u) sk_flush<RW>(S, a, cnt);
+    S.pushed += (uint32_t)__popcll(mask);
+    const uint32_t cnt = S.pushed - S.flushed;
+    if (cnt > SK_WAVE_BUF - 64u) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        sk_flush<RW>(S, a, cnt);
+    }
 }
 
 // Staged entry of the piece [p0, t_end) of the current superkmer: global start
@@ -451,7 +459,7 @@ __device__ __forceinline__ uint2 sk_entry(const SkLane& L, const SkArgs& a, uint
 // records a piece ends after a.piece k-mers (a cut that starts no new
 // superkmer).
 template <int RW>
-__device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, const SkStage& S, uint32_t t,
+__device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, uint32_t t,
                                         uint64_t Hc, uint32_t Pc)
 {
     const bool first = t == L.ts;
@@ -559,15 +567,13 @@ __device__ __forceinline__ void sk_backward(uint64_t (&sv)[WMAX], uint8_t (&sp)[
 }
 
 template <int WMAX, int RW>
-__global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
+__global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <= 12 ? 5 : 1))) void k_superkmers(SkArgs a)
 {
     __shared__ uint64_t tf[16], tr[16], s_init[2];
     __shared__ uint2 s_buf[SK_BLOCK / 64][SK_WAVE_BUF + 64];
-    __shared__ uint32_t s_wn[SK_BLOCK / 64];
     __shared__ uint32_t s_wgn;
     sk_tables(a, tf, tr, s_init);
     const uint32_t wv = threadIdx.x >> 6;
-    if (threadIdx.x < SK_BLOCK / 64) s_wn[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_wgn = 0;
     __syncthreads();
 
@@ -583,8 +589,9 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
     }
     SkStage S;
     S.buf = s_buf[wv];
-    S.wn = &s_wn[wv];
     S.wg_n = &s_wgn;
+    S.pushed = 0;
+    S.flushed = 0;
     // main pass: the run is the slot region of this block's strips; fix-up:
     // a private region of SK_BLOCK*S records per block
     const uint32_t first = blockIdx.x * SK_BLOCK;
@@ -655,10 +662,21 @@ __global__ __launch_bounds__(SK_BLOCK) void k_superkmers(SkArgs a)
         nsk = L.nsk + ((tail && L.t0p == L.t0) ? 1u : 0u);
         job = L.job;
     }
-    // every lane of the wave is here again: the chunk-end records, then flush
+    // every lane of the wave is here again: the wave's counts, the chunk-end
+    // records, then flush
+#pragma unroll
+    for (uint32_t d = 32; d; d >>= 1) {
+        S.pushed = max(S.pushed, (uint32_t)__shfl_xor((int)S.pushed, d, 64));
+        S.flushed = max(S.flushed, (uint32_t)__shfl_xor((int)S.flushed, d, 64));
+    }
     sk_push<RW>(S, a, tail, tail_rec);
-    const uint32_t left = s_wn[wv];
-    if (left) sk_flush<RW>(S, a, left);
+    const uint32_t left = S.pushed - S.flushed;
+    if (left) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        sk_flush<RW>(S, a, left);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         a.run_start[a.run_base + blockIdx.x] = (uint32_t)S.run;
@@ -684,6 +702,7 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
 #define TG_SK_RW(WM) \
     do { if (rw == 2) TG_SK(WM, 2); else if (rw == 4) TG_SK(WM, 4); else TG_SK(WM, 8); } while (0)
     if (a.W <= 8) TG_SK_RW(8);
+    else if (a.W <= 12) TG_SK_RW(12);
     else if (a.W <= 16) TG_SK_RW(16);
     else if (a.W <= 32) TG_SK_RW(32);
     else TG_SK_RW(64);
