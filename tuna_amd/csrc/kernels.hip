@@ -850,13 +850,16 @@ template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp);
 
 // Dynamic-LDS limits of the routing kernels, set once per process at their
-// largest sizes (2^14 sub-partitions), so no attribute call happens while a
-// launch sequence is being captured.
+// largest sizes, so no attribute call happens while a launch sequence is
+// being captured: the histogram at 2^14 sub-partitions, the fine scatter at
+// 3 arrays of F = 2^HSEL_BITS / 256 per-sub-partition words.
+constexpr uint32_t FINE_F_MAX = 1u << (HSEL_BITS - 8);
+
 template <int RW>
 void set_route_attributes()
 {
     const int hist = (int)(2u * (1u << 14) * sizeof(uint32_t));
-    const int cur = (int)((1u << 14) * sizeof(uint32_t));
+    const int cur = (int)(3u * FINE_F_MAX * sizeof(uint32_t));
     (void)hipFuncSetAttribute((const void*)k_route_hist<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, hist);
     (void)hipFuncSetAttribute((const void*)k_route_fine<RW>, hipFuncAttributeMaxDynamicSharedMemorySize, cur);
 }
@@ -1028,33 +1031,64 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
 }
 
 // Scatter, level 2: one workgroup per coarse bucket reads its region
-// (contiguous) and places records by sub-partition inside the same region.
+// (contiguous) in chunks that fit in LDS (64 KiB), groups each chunk by
+// sub-partition there (histogram, scan, place) and writes the groups out as
+// runs, so consecutive threads write consecutive addresses.
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp)
 {
-    extern __shared__ uint32_t cur[];  // F cursors
-    const uint32_t F = 1u << log2f, c = blockIdx.x;
-    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[c * F + f];
-    __syncthreads();
+    constexpr uint32_t CH = 16384u / RW;  // records per chunk (64 KiB)
+    constexpr uint32_t PER = CH / ROUTE_T;
+    __shared__ Rec<RW> s_rec[CH];
+    __shared__ uint32_t s_red[ROUTE_T / 64];
+    extern __shared__ uint32_t s_dyn[];  // F cursors, F counts, F group starts
+    const uint32_t F = 1u << log2f, c = blockIdx.x, tid = threadIdx.x;
+    uint32_t* cur = s_dyn;
+    uint32_t* hist = s_dyn + F;
+    uint32_t* gst = s_dyn + 2 * F;
+    for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[c * F + f];
     const uint64_t b = a.part_off[c * F], e = a.part_off[(c + 1) * F];
     const uint32_t shift = REC_HSEL_SHIFT + a.sb;
     const uint32_t fm = F - 1u;
-    constexpr uint32_t U = RW == 8 ? 4 : 8;
-    uint64_t i = b + threadIdx.x;
-    for (; i + (U - 1) * (uint64_t)blockDim.x < e; i += U * (uint64_t)blockDim.x) {
-        Rec<RW> v[U];
+    for (uint64_t cb = b; cb < e; cb += CH) {
+        for (uint32_t f = tid; f < F; f += blockDim.x) hist[f] = 0;
+        __syncthreads();
+        Rec<RW> v[PER];
+        uint32_t fk[PER], pos[PER];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) v[u] = load_rec<RW>(tmp, i + u * blockDim.x);
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t pos = atomicAdd(&cur[(rec_meta<RW>(v[u]) >> shift) & fm], 1u);
-            store_rec<RW>(a.dst, pos, v[u]);
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint64_t i = cb + j * ROUTE_T + tid;
+            fk[j] = 0xFFFFFFFFu;
+            if (i < e) {
+                v[j] = load_rec<RW>(tmp, i);
+                fk[j] = (rec_meta<RW>(v[j]) >> shift) & fm;
+                pos[j] = atomicAdd(&hist[fk[j]], 1u);
+            }
         }
-    }
-    for (; i < e; i += blockDim.x) {
-        const Rec<RW> v = load_rec<RW>(tmp, i);
-        const uint32_t pos = atomicAdd(&cur[(rec_meta<RW>(v) >> shift) & fm], 1u);
-        store_rec<RW>(a.dst, pos, v);
+        __syncthreads();
+        // exclusive scan of the F counts (F <= blockDim.x * 4 handled in a loop)
+        uint32_t carry = 0;
+        for (uint32_t f0 = 0; f0 < F; f0 += blockDim.x) {
+            const uint32_t f = f0 + tid;
+            uint32_t tot;
+            const uint32_t ex = block_excl_u32(f < F ? hist[f] : 0u, s_red, tot);
+            if (f < F) gst[f] = carry + ex;
+            carry += tot;
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j)
+            if (fk[j] != 0xFFFFFFFFu) s_rec[gst[fk[j]] + pos[j]] = v[j];
+        __syncthreads();
+        const uint32_t nch = (uint32_t)min((uint64_t)CH, e - cb);
+        for (uint32_t i = tid; i < nch; i += ROUTE_T) {
+            const Rec<RW> r = s_rec[i];
+            const uint32_t f = (rec_meta<RW>(r) >> shift) & fm;
+            store_rec<RW>(a.dst, (uint64_t)cur[f] + (i - gst[f]), r);
+        }
+        __syncthreads();
+        for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] += hist[f];
+        __syncthreads();
     }
 }
 
@@ -1065,8 +1099,9 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     uint32_t log2f = 0;
     while ((a.nsp >> log2f) > 256u) ++log2f;
     const uint32_t nc = a.nsp >> log2f;
+    // nsp <= 2^HSEL_BITS, so F = nsp / nc <= FINE_F_MAX (the fine scatter's LDS)
     set_kernel_attributes();
-    const size_t lc = 0, lf = (1u << log2f) * sizeof(uint32_t);
+    const size_t lc = 0, lf = 3ull * (1u << log2f) * sizeof(uint32_t);
     if (rw == 2) {
         k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
         k_route_fine<2><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
