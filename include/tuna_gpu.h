@@ -259,12 +259,24 @@ int tg_host_free(void* p);
  * or a negative status (TG_ECAP when off_cap is short, TG_EINPUT on empty). */
 int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases,
                         uint64_t* out_off, uint64_t off_cap);
+/* The same parse on `threads` host threads, with the same output (bytes,
+ * offsets and status) as tg_parse_buffer: the input is cut into sections at
+ * FASTQ record starts (every fourth line) or anywhere in FASTA (a chunk that
+ * crosses a cut is joined); inputs whose FASTQ records do not follow the
+ * lines are parsed sequentially.  Replaces the reference's parallel readers
+ * (src/seq_source.hpp:48-205). */
+int64_t tg_parse_buffer_mt(const uint8_t* data, uint64_t len, uint8_t* out_bases, uint64_t* out_off,
+                           uint64_t off_cap, uint32_t threads);
 
 /* Read a whole input file into a malloc'd buffer (gzip-compressed when the
  * name ends in ".gz", as src/seq_source.hpp:196-204).  Errors follow the
  * reference messages: "Cannot open file: <path>", "Empty file: <path>". */
 int tg_read_file(const char* path, uint8_t** data, uint64_t* len);
 void tg_free(void* p);
+/* A plain (not gzip) input file as a read-only mapping, parsed in place with
+ * no copy; same error messages as tg_read_file.  Release with tg_unmap_file. */
+int tg_map_file(const char* path, const uint8_t** data, uint64_t* len);
+void tg_unmap_file(const uint8_t* data, uint64_t len);
 
 /* Reference auto-tuned partition count (src/setup.hpp:82-196): depends on the
  * host's L2/L3 sizes, RLIMIT_NOFILE, the input sizes, k and threads.  Not yet

@@ -153,6 +153,23 @@ def test_empty_and_missing_inputs_report_reference_messages(tmp_path):
         T.read_file(m)
 
 
+def test_mapped_input_equals_read_input(tmp_path):
+    data = T.synth("kind=reads_fq genome=20000 seed=4 reads=300 rl=150 sub=0.01 nrate=0.001").tobytes()
+    p = tmp_path / "r.fq"
+    p.write_bytes(data)
+    L = T.lib()
+    d, n = T._u8p(), T.ctypes.c_uint64()
+    assert L.tg_map_file(str(p).encode(), T.ctypes.byref(d), T.ctypes.byref(n)) == 0
+    assert n.value == len(data) and T.ctypes.string_at(d, n.value) == data
+    L.tg_unmap_file(d, n)
+    for name, content, msg in (("missing.fq", None, "Cannot open file"), ("empty.fq", b"", "Empty file")):
+        q = tmp_path / name
+        if content is not None:
+            q.write_bytes(content)
+        assert L.tg_map_file(str(q).encode(), T.ctypes.byref(d), T.ctypes.byref(n)) == -5
+        assert L.tg_last_error().decode() == f"{msg}: {q}"
+
+
 def test_auto_partitions_bounds(tmp_path):
     f = tmp_path / "g.fa"
     f.write_text(">a\n" + "ACGT" * 1000000 + "\n")

@@ -64,6 +64,71 @@ def test_host_parser_matches_oracle_on_random_text():
         assert np.array_equal(o1, o2) and np.array_equal(b1, b2)
 
 
+def _quirk_fastq(rng):
+    """FASTQ where some records start with an empty header line whose '\n'
+    sits at block offset 63 (the reference lexer then loses that line feed and
+    the record spans five lines), and whose DNA may end at a multiple of 64."""
+    out = ""
+    for i in range(rng.randint(2, 12)):
+        if rng.random() < 0.5:
+            # a filler record that brings the next record start to offset 63
+            pad = (63 - len(out)) % 64
+            sl = rng.randint(0, 20)
+            h = (pad - (2 * sl + 6)) % 64
+            out += "@%s\n%s\n+\n%s\n" % ("x" * h, "A" * sl, "?" * sl)
+            assert len(out) % 64 == 63
+            hdr = ""
+        else:
+            hdr = "@r%d" % i
+        seq = "".join(rng.choice("ACGTN") for _ in range(rng.randint(0, 90)))
+        out += "%s\n%s\n+\n%s\n" % (hdr, seq, "?" * len(seq))
+    if rng.random() < 0.5:
+        out = out.rstrip("\n")
+    if rng.random() < 0.3:
+        out += "@z\n" + "A" * ((-len(out) - 3) % 64)  # DNA that ends the text at a multiple of 64
+    return out
+
+
+def test_parallel_parser_matches_sequential():
+    """Any number of threads gives the sequential parse: cuts inside records,
+    inside headers, inside chunks that continue over many lines, around the
+    64-byte-block behaviours and at the text's ends."""
+    rng = random.Random(12)
+    texts = [GOLD["inputs"][n].encode() for n in sorted(GOLD["parse"])]
+    texts += [_random_text(rng).encode() for _ in range(300)]
+    texts += [_quirk_fastq(rng).encode() for _ in range(200)]
+    # long multi-line and single-line FASTA records, headers across cuts
+    for _ in range(20):
+        g = "".join(rng.choice("ACGTN" if rng.random() < 0.01 else "ACGT") for _ in range(rng.randint(500, 5000)))
+        w = rng.choice([0, 60, 80])
+        body = "\n".join(g[j:j + w] for j in range(0, len(g), w)) if w else g
+        texts.append(("junk\n>h1 " + "x" * rng.randint(0, 300) + "\n" + body + "\n>h2\n" + body[:777]).encode())
+    for t in texts:
+        if not t:
+            continue
+        b1, o1 = T.parse_buffer(t)
+        for th in (2, 3, 5, 8, 16, 37, 64):
+            b2, o2 = T.parse_buffer(t, threads=th)
+            assert np.array_equal(o1, o2) and np.array_equal(b1, b2), (th, t[:80])
+
+
+def test_host_parser_matches_oracle_on_block_boundary_fastq():
+    rng = random.Random(3)
+    for _ in range(200):
+        t = _quirk_fastq(rng).encode()
+        b1, o1 = T.parse_buffer(t)
+        b2, o2 = O.parse(t)
+        assert np.array_equal(o1, o2) and np.array_equal(b1, b2)
+
+
+def test_parallel_parser_on_a_large_fastq():
+    t = T.synth("kind=reads_fq genome=200000 seed=5 reads=20000 rl=150 sub=0.005 nrate=0.001")
+    b1, o1 = T.parse_buffer(t)
+    for th in (4, 16):
+        b2, o2 = T.parse_buffer(t, threads=th)
+        assert np.array_equal(o1, o2) and np.array_equal(b1, b2)
+
+
 def test_host_parser_empty_raises():
     with pytest.raises(T.TunaGpuError):
         T.parse_buffer(b"")

@@ -35,7 +35,8 @@ EXPORTS = [
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
-    "tg_result_tsv", "tg_result_kff", "tg_result_write",
+    "tg_result_tsv", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
+    "tg_unmap_file",
 ]
 
 
@@ -154,7 +155,12 @@ def lib():
                                      ctypes.c_uint64, _u64p, _u64p]
     L.tg_parse_buffer.restype = ctypes.c_int64
     L.tg_parse_buffer.argtypes = [_u8p, ctypes.c_uint64, _u8p, _u64p, ctypes.c_uint64]
+    L.tg_parse_buffer_mt.restype = ctypes.c_int64
+    L.tg_parse_buffer_mt.argtypes = [_u8p, ctypes.c_uint64, _u8p, _u64p, ctypes.c_uint64, ctypes.c_uint32]
     L.tg_read_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p), _u64p]
+    L.tg_map_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p), _u64p]
+    L.tg_unmap_file.argtypes = [_u8p, ctypes.c_uint64]
+    L.tg_unmap_file.restype = None
     L.tg_free.argtypes = [_vp]
     L.tg_free.restype = None
     L.tg_auto_partitions.restype = ctypes.c_uint32
@@ -236,13 +242,17 @@ def _u8(x):
 
 
 # ------------------------------------------------------------------ host ----
-def parse_buffer(data):
-    """FASTA/FASTQ bytes -> (bases uint8, chunk offsets uint64) on the host."""
+def parse_buffer(data, threads=1):
+    """FASTA/FASTQ bytes -> (bases uint8, chunk offsets uint64) on the host,
+    on `threads` host threads (the output does not depend on it)."""
     d = _u8(data)
     out = np.zeros(max(len(d), 1), dtype=np.uint8)
     cap = len(d) // 2 + 2
     off = np.zeros(cap, dtype=np.uint64)
-    n = lib().tg_parse_buffer(_p(d, _u8p), len(d), _p(out, _u8p), _p(off, _u64p), cap)
+    if threads > 1:
+        n = lib().tg_parse_buffer_mt(_p(d, _u8p), len(d), _p(out, _u8p), _p(off, _u64p), cap, threads)
+    else:
+        n = lib().tg_parse_buffer(_p(d, _u8p), len(d), _p(out, _u8p), _p(off, _u64p), cap)
     if n < 0:
         raise TunaGpuError("Empty file" if n == -5 else f"parse failed ({n})")
     off = off[: n + 1].copy()

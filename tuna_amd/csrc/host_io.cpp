@@ -7,8 +7,11 @@
 #include "../../include/tuna_gpu.h"
 #include "kff.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/resource.h>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -18,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -40,51 +44,120 @@ inline uint64_t lf_at_or_after(const uint8_t* d, uint64_t len, uint64_t from)
     return q ? (uint64_t)((const uint8_t*)q - d) : len;
 }
 
+// Output of a scan: DNA runs and chunk ends, in order.  The direct sink
+// writes into the caller's arrays (TG_ECAP when the offsets do not fit); the
+// section sink keeps a section's output for the merge of a parallel parse.
+struct DirectSink {
+    uint8_t* ob;
+    uint64_t* off;
+    uint64_t cap, nb = 0, nc = 0;
+    bool full = false;
+    bool bases(const uint8_t* p, uint64_t n)
+    {
+        memcpy(ob + nb, p, n);
+        nb += n;
+        return true;
+    }
+    bool end()
+    {
+        if (nc + 1 >= cap) {
+            full = true;
+            return false;
+        }
+        off[++nc] = nb;
+        return true;
+    }
+};
+
+// A section of a parallel parse is scanned twice: once to count its bases
+// and chunk ends, then again to write them at offsets known from the counts.
+struct CountSink {
+    uint64_t nb = 0, ne = 0;
+    bool bases(const uint8_t*, uint64_t n)
+    {
+        nb += n;
+        return true;
+    }
+    bool end()
+    {
+        ++ne;
+        return true;
+    }
+};
+
+struct PlaceSink {
+    uint8_t* ob;     // the section's first base
+    uint64_t* off;   // the section's first chunk-end slot
+    uint64_t base;   // the section's first base, as a global offset
+    uint64_t nb = 0, ne = 0;
+    bool bases(const uint8_t* p, uint64_t n)
+    {
+        memcpy(ob + nb, p, n);
+        nb += n;
+        return true;
+    }
+    bool end()
+    {
+        off[ne++] = base + nb;
+        return true;
+    }
+};
+
 // FASTA: include/helicase_cpp/lexer.hpp:46-71 (a '>' opens a header that runs
 // through the next '\n'), parser_fasta.hpp:233-316 (bytes before the first
 // header are skipped; '\n' inside sequence is transparent; any other
-// non-ACGT byte or a header closes the chunk).
-int64_t parse_fasta(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap)
+// non-ACGT byte or a header closes the chunk).  Scans [i, stop) with the chunk
+// state `in_chunk`; a run is cut at `stop` (the next section continues it).
+// `closed` reports a chunk-closing byte ('>' or non-ACGT, non-'\n') seen
+// before the first base, which a parallel merge needs.
+template <class Sink>
+bool fasta_scan(const uint8_t* d, uint64_t len, uint64_t i, uint64_t stop, Sink& sink, bool& in_chunk,
+                bool& closed)
 {
-    const void* first = memchr(d, '>', len);
-    if (!first) return 0;
-    uint64_t i = (uint64_t)((const uint8_t*)first - d);
-    uint64_t nb = 0, nc = 0;
-    bool in_chunk = false;
-    off[0] = 0;
-    while (i < len) {
+    bool seen_base = false;
+    while (i < stop) {
         const uint8_t c = d[i];
         if (c == '>') {
+            if (!seen_base) closed = true;
             if (in_chunk) {
-                if (nc + 1 >= cap) return TG_ECAP;
-                off[++nc] = nb;
+                if (!sink.end()) return false;
                 in_chunk = false;
             }
             i = lf_at_or_after(d, len, i) + 1;
             continue;
         }
         if (ACGT.t[c]) {
-            // copy the run up to the next non-ACGT byte
             uint64_t j = i;
-            while (j < len && ACGT.t[d[j]]) ++j;
-            memcpy(ob + nb, d + i, j - i);
-            nb += j - i;
+            while (j < stop && ACGT.t[d[j]]) ++j;
+            if (!sink.bases(d + i, j - i)) return false;
             in_chunk = true;
+            seen_base = true;
             i = j;
             continue;
         }
-        if (c != '\n' && in_chunk) {
-            if (nc + 1 >= cap) return TG_ECAP;
-            off[++nc] = nb;
-            in_chunk = false;
+        if (c != '\n') {
+            if (!seen_base) closed = true;
+            if (in_chunk) {
+                if (!sink.end()) return false;
+                in_chunk = false;
+            }
         }
         ++i;
     }
-    if (in_chunk) {
-        if (nc + 1 >= cap) return TG_ECAP;
-        off[++nc] = nb;
-    }
-    return (int64_t)nc;
+    return true;
+}
+
+int64_t parse_fasta(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap)
+{
+    const void* first = memchr(d, '>', len);
+    if (!first) return 0;
+    DirectSink k{ob, off, cap};
+    off[0] = 0;
+    bool in_chunk = false, closed = false;
+    fasta_scan(d, len, (uint64_t)((const uint8_t*)first - d), len, k, in_chunk, closed);
+    if (k.full) return TG_ECAP;
+    if (in_chunk && !k.end()) return TG_ECAP;
+    return (int64_t)k.nc;
 }
 
 // FASTQ: include/helicase_cpp/parser_fastq.hpp:112-285.  Four-line records,
@@ -93,32 +166,33 @@ int64_t parse_fasta(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, 
 // over and, when that byte is a '\n' at block offset 63, the line feed is
 // lost with the block (:115, increment_pos); a DNA run that reaches end of
 // input exactly at a multiple of 64 bytes is not delivered (:181).
-int64_t parse_fastq(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap)
+//
+// Scans the records from `pos` (a record start) up to the first record that
+// starts at or after `stop`.  `quirk` reports a header whose first byte is a
+// '\n' at block offset 63: that record spans five lines.
+template <class Sink>
+void fastq_scan(const uint8_t* d, uint64_t len, uint64_t pos, uint64_t stop, Sink& sink, bool& quirk)
 {
-    uint64_t pos = 0, nb = 0, nc = 0;
-    off[0] = 0;
     for (;;) {
         // header line
-        if (pos + 1 >= len) return (int64_t)nc;
+        if (pos >= stop || pos + 1 >= len) return;
+        if (pos % 64 == 63 && d[pos] == '\n') quirk = true;
         uint64_t lf = lf_at_or_after(d, len, (pos % 64 == 63) ? pos + 1 : pos);
-        if (lf >= len) return (int64_t)nc;
+        if (lf >= len) return;
         pos = lf + 1;
         // sequence line
         for (;;) {
             uint64_t s = pos;
             while (s < len && !ACGT.t[d[s]] && d[s] != '\n') ++s;
-            if (s >= len) return (int64_t)nc;
+            if (s >= len) return;
             if (d[s] == '\n') {
                 pos = s + 1;
                 break;
             }
             uint64_t r = s;
             while (r < len && ACGT.t[d[r]]) ++r;
-            if (r == len && (len & 63u) == 0) return (int64_t)nc;
-            memcpy(ob + nb, d + s, r - s);
-            nb += r - s;
-            if (nc + 1 >= cap) return TG_ECAP;
-            off[++nc] = nb;
+            if (r == len && (len & 63u) == 0) return;
+            if (!sink.bases(d + s, r - s) || !sink.end()) return;
             if (r < len && d[r] == '\n') {
                 pos = r + 1;
                 break;
@@ -128,10 +202,186 @@ int64_t parse_fastq(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, 
         // '+' line, quality line
         for (int l = 0; l < 2; ++l) {
             lf = lf_at_or_after(d, len, pos);
-            if (lf >= len) return (int64_t)nc;
+            if (lf >= len) return;
             pos = lf + 1;
         }
     }
+}
+
+int64_t parse_fastq(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap)
+{
+    DirectSink k{ob, off, cap};
+    off[0] = 0;
+    bool quirk = false;
+    fastq_scan(d, len, 0, len, k, quirk);
+    return k.full ? TG_ECAP : (int64_t)k.nc;
+}
+
+// ---- parallel parse: the same output from S sections on S threads ----
+template <class F>
+void run_sections(uint32_t S, F&& f)
+{
+    std::vector<std::thread> th;
+    th.reserve(S);
+    for (uint32_t q = 0; q < S; ++q) th.emplace_back(f, q);
+    for (auto& t : th) t.join();
+}
+
+// Placement of the sections' outputs: base offset and first chunk-end slot of
+// each.  A FASTA chunk may continue across a cut (`carry`); when a closing
+// byte opens the next section, the carried chunk ends at that section's first
+// base, in an extra slot before the section's own.  Returns the chunk count.
+struct SectionPlan {
+    std::vector<uint64_t> bbase, ebase;
+    std::vector<uint8_t> close_carry;  // section q closes the carried chunk first
+    bool close_last = false;           // the text ends inside a chunk
+    uint64_t n_ends = 0;
+};
+
+SectionPlan plan_sections(const std::vector<CountSink>& cnt, const std::vector<uint8_t>& open_end,
+                          const std::vector<uint8_t>& closed_first, bool fasta)
+{
+    const size_t S = cnt.size();
+    SectionPlan p;
+    p.bbase.assign(S + 1, 0);
+    p.ebase.assign(S + 1, 0);
+    p.close_carry.assign(S, 0);
+    bool carry = false;
+    uint64_t e = 0;
+    for (size_t q = 0; q < S; ++q) {
+        p.bbase[q + 1] = p.bbase[q] + cnt[q].nb;
+        if (fasta && carry && closed_first[q]) {
+            p.close_carry[q] = 1;
+            ++e;
+            carry = false;
+        }
+        p.ebase[q] = e;
+        e += cnt[q].ne;
+        if (fasta && (cnt[q].ne || cnt[q].nb || closed_first[q])) carry = open_end[q];
+    }
+    p.ebase[S] = e;
+    if (fasta && carry) {
+        p.close_last = true;
+        ++e;
+    }
+    p.n_ends = e;
+    return p;
+}
+
+int64_t parse_fastq_mt(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap, uint32_t S)
+{
+    std::vector<uint64_t> st(S + 1), nl(S, 0), first(S + 1, len + 1);
+    for (uint32_t q = 0; q <= S; ++q) st[q] = len * q / S;
+    run_sections(S, [&](uint32_t q) {
+        uint64_t n = 0;
+        for (const uint8_t* p = d + st[q]; p < d + st[q + 1];) {
+            const void* x = memchr(p, '\n', (size_t)(d + st[q + 1] - p));
+            if (!x) break;
+            ++n;
+            p = (const uint8_t*)x + 1;
+        }
+        nl[q] = n;
+    });
+    // first record start of each section: the first line start at or after
+    // the section start whose line index is a multiple of 4
+    uint64_t lines = 0;
+    for (uint32_t q = 0; q < S; ++q) {
+        uint64_t p = st[q], idx = lines;
+        if (p > 0 && d[p - 1] != '\n') {
+            p = lf_at_or_after(d, len, p) + 1;
+            idx += 1;
+        }
+        while (p < st[q + 1] && (idx & 3u)) {
+            p = lf_at_or_after(d, len, p) + 1;
+            ++idx;
+        }
+        first[q] = p < st[q + 1] ? p : len + 1;
+        lines += nl[q];
+    }
+    // a section's records run up to the next section's first record
+    for (int q = (int)S - 1; q >= 0; --q)
+        if (first[q] > len) first[q] = first[q + 1];
+    std::vector<CountSink> cnt(S);
+    std::vector<uint8_t> quirk(S, 0);
+    run_sections(S, [&](uint32_t q) {
+        if (first[q] > len) return;
+        bool qk = false;
+        fastq_scan(d, len, first[q], first[q + 1], cnt[q], qk);
+        quirk[q] = qk;
+    });
+    for (uint32_t q = 0; q < S; ++q)
+        if (quirk[q]) return parse_fastq(d, len, ob, off, cap);  // records do not follow the lines
+    const std::vector<uint8_t> none(S, 0);
+    const SectionPlan pl = plan_sections(cnt, none, none, false);
+    if (pl.n_ends >= cap) return TG_ECAP;
+    off[0] = 0;
+    run_sections(S, [&](uint32_t q) {
+        if (first[q] > len) return;
+        PlaceSink k{ob + pl.bbase[q], off + 1 + pl.ebase[q], pl.bbase[q]};
+        bool qk = false;
+        fastq_scan(d, len, first[q], first[q + 1], k, qk);
+    });
+    return (int64_t)pl.n_ends;
+}
+
+int64_t parse_fasta_mt(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap, uint32_t S)
+{
+    const void* fp = memchr(d, '>', len);
+    if (!fp) return 0;
+    const uint64_t f0 = (uint64_t)((const uint8_t*)fp - d);
+    std::vector<uint64_t> st(S + 1);
+    for (uint32_t q = 0; q <= S; ++q) st[q] = std::max(f0, len * q / S);
+    // a section starts inside a header when the last '>' before it comes
+    // after the last '\n' before it: last of each per section, then maxima
+    std::vector<int64_t> lnl(S, -1), lgt(S, -1);
+    run_sections(S, [&](uint32_t q) {
+        const size_t n = (size_t)(st[q + 1] - st[q]);
+        const void* a = n ? memrchr(d + st[q], '\n', n) : nullptr;
+        const void* b = n ? memrchr(d + st[q], '>', n) : nullptr;
+        if (a) lnl[q] = (int64_t)((const uint8_t*)a - d);
+        if (b) lgt[q] = (int64_t)((const uint8_t*)b - d);
+    });
+    std::vector<uint8_t> in_hdr(S, 0);
+    {
+        int64_t nl_before = -1, gt_before = -1;
+        for (uint32_t q = 0; q < S; ++q) {
+            in_hdr[q] = gt_before > nl_before;
+            nl_before = std::max(nl_before, lnl[q]);
+            gt_before = std::max(gt_before, lgt[q]);
+        }
+    }
+    std::vector<uint64_t> from(S, 0);
+    std::vector<uint8_t> skip_hdr(S, 0);
+    for (uint32_t q = 0; q < S; ++q) {
+        from[q] = st[q];
+        if (st[q] < st[q + 1] && st[q] > f0 && in_hdr[q]) {
+            // the rest of the header is skipped (a closing event for a chunk
+            // carried into this section)
+            skip_hdr[q] = 1;
+            from[q] = lf_at_or_after(d, len, st[q]) + 1;
+        }
+    }
+    std::vector<CountSink> cnt(S);
+    std::vector<uint8_t> open_end(S, 0), closed_first(S, 0);
+    run_sections(S, [&](uint32_t q) {
+        if (st[q] >= st[q + 1]) return;
+        bool in_chunk = false, cl = false;
+        if (from[q] < st[q + 1]) fasta_scan(d, len, from[q], st[q + 1], cnt[q], in_chunk, cl);
+        open_end[q] = in_chunk;
+        closed_first[q] = skip_hdr[q] || cl;
+    });
+    const SectionPlan pl = plan_sections(cnt, open_end, closed_first, true);
+    if (pl.n_ends >= cap) return TG_ECAP;
+    off[0] = 0;
+    run_sections(S, [&](uint32_t q) {
+        if (pl.close_carry[q]) off[1 + pl.ebase[q] - 1] = pl.bbase[q];
+        if (st[q] >= st[q + 1] || from[q] >= st[q + 1]) return;
+        PlaceSink k{ob + pl.bbase[q], off + 1 + pl.ebase[q], pl.bbase[q]};
+        bool in_chunk = false, cl = false;
+        fasta_scan(d, len, from[q], st[q + 1], k, in_chunk, cl);
+    });
+    if (pl.close_last) off[pl.n_ends] = pl.bbase[S];
+    return (int64_t)pl.n_ends;
 }
 
 // ------------------------------------------------------------ synthetic ----
@@ -410,6 +660,41 @@ std::string kff_footer()
 
 extern "C" {
 
+int tg_map_file(const char* path, const uint8_t** data, uint64_t* len)
+{
+    if (!path || !data || !len) return TG_EINVAL;
+    *data = nullptr;
+    *len = 0;
+    const std::string p(path);
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) {
+        set_host_error("Cannot open file: " + p);
+        return TG_EINPUT;
+    }
+    struct stat st {};
+    if (fstat(fd, &st) != 0 || st.st_size == 0) {
+        close(fd);
+        set_host_error("Empty file: " + p);
+        return TG_EINPUT;
+    }
+    const uint64_t n = (uint64_t)st.st_size;
+    void* m = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (m == MAP_FAILED) {
+        set_host_error("cannot map file: " + p);
+        return TG_EINPUT;
+    }
+    (void)madvise(m, n, MADV_WILLNEED);
+    *data = (const uint8_t*)m;
+    *len = n;
+    return TG_OK;
+}
+
+void tg_unmap_file(const uint8_t* data, uint64_t len)
+{
+    if (data && len) munmap((void*)data, len);
+}
+
 int tg_read_file(const char* path, uint8_t** data, uint64_t* len)
 {
     if (!path || !data || !len) return TG_EINVAL;
@@ -606,6 +891,17 @@ int64_t tg_parse_buffer(const uint8_t* data, uint64_t len, uint8_t* out_bases, u
     if (off_cap < 1) return TG_ECAP;
     return data[0] == '@' ? parse_fastq(data, len, out_bases, out_off, off_cap)
                           : parse_fasta(data, len, out_bases, out_off, off_cap);
+}
+
+int64_t tg_parse_buffer_mt(const uint8_t* data, uint64_t len, uint8_t* out_bases, uint64_t* out_off,
+                           uint64_t off_cap, uint32_t threads)
+{
+    if (len == 0) return TG_EINPUT;
+    if (off_cap < 1) return TG_ECAP;
+    const uint32_t S = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(threads, 1u), std::max<uint64_t>(len / 64, 1));
+    if (S <= 1) return tg_parse_buffer(data, len, out_bases, out_off, off_cap);
+    return data[0] == '@' ? parse_fastq_mt(data, len, out_bases, out_off, off_cap, S)
+                          : parse_fasta_mt(data, len, out_bases, out_off, off_cap, S);
 }
 
 // Workloads (all sizes in bases; seed selects the stream):

@@ -237,26 +237,83 @@ std::string fmt_s(double s)
 }
 
 struct HostChunks {
-    std::vector<uint8_t> bases;
+    uint8_t* bases = nullptr;  // pinned (tg_host_alloc), so the upload runs at full rate
+    uint64_t n_bases = 0;
     std::vector<uint64_t> off{0};
     uint64_t seqs = 0;
+    HostChunks() = default;
+    HostChunks(const HostChunks&) = delete;
+    HostChunks& operator=(const HostChunks&) = delete;
+    ~HostChunks()
+    {
+        if (bases) tg_host_free(bases);
+    }
 };
 
+// Every input file is parsed (on -t threads) straight into one pinned buffer.
 void read_inputs(const Config& cfg, HostChunks& h)
 {
-    for (const auto& f : cfg.input_files) {
-        uint8_t* data = nullptr;
+    // plain files are mapped and parsed in place; gzip input is inflated
+    struct Input {
+        const uint8_t* d = nullptr;
         uint64_t len = 0;
-        if (tg_read_file(f.c_str(), &data, &len) != TG_OK) throw std::runtime_error(tg_last_error());
-        std::unique_ptr<uint8_t, void (*)(void*)> hold(data, tg_free);
-        std::vector<uint8_t> b(len);
-        std::vector<uint64_t> o(len / 2 + 2);
-        const int64_t n = tg_parse_buffer(data, len, b.data(), o.data(), o.size());
-        if (n < 0) throw std::runtime_error("cannot parse input: " + f);
-        const uint64_t base = h.bases.size();
-        h.bases.insert(h.bases.end(), b.begin(), b.begin() + (std::ptrdiff_t)o[n]);
-        for (int64_t i = 1; i <= n; ++i) h.off.push_back(base + o[i]);
-        h.seqs += (uint64_t)n;
+        bool mapped = false;
+        void release()
+        {
+            if (mapped) tg_unmap_file(d, len);
+            else tg_free((void*)d);
+            d = nullptr;
+        }
+        const uint8_t* get() const { return d; }
+    };
+    std::vector<Input> data;
+    std::vector<uint64_t> lens;
+    uint64_t total = 0;
+    struct Release {
+        std::vector<Input>& v;
+        ~Release()
+        {
+            for (auto& x : v)
+                if (x.d) x.release();
+        }
+    } release{data};
+    for (const auto& f : cfg.input_files) {
+        Input in;
+        const bool gz = f.size() > 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
+        if (gz) {
+            uint8_t* d = nullptr;
+            if (tg_read_file(f.c_str(), &d, &in.len) != TG_OK) throw std::runtime_error(tg_last_error());
+            in.d = d;
+        } else {
+            if (tg_map_file(f.c_str(), &in.d, &in.len) != TG_OK) throw std::runtime_error(tg_last_error());
+            in.mapped = true;
+        }
+        data.push_back(in);
+        lens.push_back(in.len);
+        total += in.len;
+    }
+    if (tg_host_alloc(std::max<uint64_t>(total, 1), (void**)&h.bases) != TG_OK)
+        throw std::runtime_error(tg_last_error());
+    const uint32_t threads = std::max(1u, cfg.num_threads);
+    for (size_t i = 0; i < data.size(); ++i) {
+        const uint64_t len = lens[i];
+        // chunk offsets: a first guess, then the bound (every chunk needs a
+        // base and a separator)
+        for (uint64_t cap = len / 64 + 1024;;) {
+            std::unique_ptr<uint64_t[]> o(new uint64_t[cap]);
+            const int64_t n = tg_parse_buffer_mt(data[i].get(), len, h.bases + h.n_bases, o.get(), cap, threads);
+            if (n == TG_ECAP && cap < len / 2 + 2) {
+                cap = len / 2 + 2;
+                continue;
+            }
+            if (n < 0) throw std::runtime_error("cannot parse input: " + cfg.input_files[i]);
+            h.off.reserve(h.off.size() + (size_t)n);
+            for (int64_t j = 1; j <= n; ++j) h.off.push_back(h.n_bases + o[j]);
+            h.n_bases += o[n];
+            h.seqs += (uint64_t)n;
+            break;
+        }
+        data[i].release();
     }
 }
 
@@ -268,12 +325,12 @@ void check(int rc)
 int write_partition_files(const Config& cfg, tg_ctx* ctx, const tg_params& p, const HostChunks& h)
 {
     // -kt: the reference keeps hash_<p>.superkmers (src/superkmer_io.hpp:30-33)
-    const uint64_t K = h.bases.size();
+    const uint64_t K = h.n_bases;
     std::vector<uint64_t> part_off(cfg.num_partitions + 1);
     std::vector<uint8_t> bytes(4 * K + 64);
     std::vector<uint64_t> rec_off(K + 2);
     uint64_t nsk = 0, nb = 0;
-    check(tg_partitions_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(), h.off.size() - 1,
+    check(tg_partitions_host(ctx, &p, h.bases, h.n_bases, h.off.data(), h.off.size() - 1,
                              part_off.data(), bytes.data(), bytes.size(), rec_off.data(), rec_off.size(),
                              &nsk, &nb));
     std::filesystem::create_directories(cfg.work_dir);
@@ -309,7 +366,7 @@ int run(Config& cfg)
         uint64_t nsk = 0;
         uint32_t nsp = 0;
         // counts only (no output buffers): TG_ECAP just reports that none were copied
-        const int rc = tg_superkmers_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(),
+        const int rc = tg_superkmers_host(ctx, &p, h.bases, h.n_bases, h.off.data(),
                                           h.off.size() - 1, nullptr, nullptr, nullptr, nullptr, 0, &nsk, &nsp);
         if (rc != TG_OK && rc != TG_ECAP) check(rc);
         if (cfg.keep_tmp && write_partition_files(cfg, ctx, p, h)) return 1;
@@ -326,7 +383,7 @@ int run(Config& cfg)
         return 0;
     }
     tg_stats st{};
-    check(tg_count_host(ctx, &p, h.bases.data(), h.bases.size(), h.off.data(), h.off.size() - 1, &st));
+    check(tg_count_host(ctx, &p, h.bases, h.n_bases, h.off.data(), h.off.size() - 1, &st));
     // phase split on the wall clock: parse + device phase 1 | device phase 2 + output
     const double t_all = since(t_part);
     const double t_phase1 = std::max(0.0, t_all - st.ms_phase2 * 1e-3);
