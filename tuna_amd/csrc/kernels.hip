@@ -1393,6 +1393,16 @@ __device__ __forceinline__ uint32_t wave_incl_u32(uint32_t v, uint32_t lane)
     return v;
 }
 
+// Occurrences of the key in LDS slot s.  One-word table: the count word holds
+// the occurrences after the one that claimed the slot (a claim needs no add).
+// Two-word table: the count word is the count (the claim writes 1).
+template <int NW>
+__device__ __forceinline__ uint32_t slot_count(const uint64_t* k0, const uint32_t* cnt, uint32_t s)
+{
+    if (NW == 1) return k0[s] != EMPTY64 ? cnt[s] + 1u : 0u;
+    return cnt[s];
+}
+
 // Keys waiting on a wave's stack for their next probe (k <= 32): the count
 // loop makes exactly one probe per k-mer and pushes the misses, so its lanes
 // stay in step; the stack is then worked off 64 entries per pass, one probe
@@ -1425,7 +1435,7 @@ __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, 
             const uint64_t old = atomicCAS((unsigned long long*)&keys[slot], (unsigned long long)EMPTY64,
                                            (unsigned long long)x);
             if (old == EMPTY64 || old == x) {
-                atomicAdd(&cnt[slot], 1u);
+                if (old == x) atomicAdd(&cnt[slot], 1u);  // a claim is counted by the drain
             } else if (++p == a.probe_max) {
                 if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
                 else spill_insert1(spill_ref(a), x);
@@ -1462,7 +1472,7 @@ constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8
 // record's first k-mer (computed once per record) when it crosses into it.
 //
 // Tables: NW == 1, one 64-bit word per key claimed by compare-and-swap, plus a
-// count word.  NW == 2, two key words per slot and the count word as the slot
+// count word for the occurrences after the claiming one.  NW == 2, two key words per slot and the count word as the slot
 // state (0 empty, LOCKED32 being written, otherwise the count).  A key probes
 // slot h, h+1, h+3, h+6, ... (triangular steps, which visit every slot of a
 // power-of-two table and break up the clusters of linear probing).  Either
@@ -1596,7 +1606,8 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                 auto settle = [&](bool pa, uint64_t px, uint32_t ps, uint64_t po) {
                     const bool hit = pa && (po == EMPTY64 || po == px);
                     const bool miss = pa && !hit;
-                    if (hit) atomicAdd(&s_cnt[ps], 1u);
+                    // the occurrence that claimed the slot is counted by the drain
+                    if (pa && po == px) atomicAdd(&s_cnt[ps], 1u);
                     const uint64_t mm = __ballot(miss);
                     if (mm) {
                         if (miss) {
@@ -1776,7 +1787,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         // drain: count, reserve once per workgroup, write
         uint32_t mine = 0, uniq = 0;
         for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
-            const uint32_t c = s_cnt[s];
+            const uint32_t c = slot_count<NW>(s_k0, s_cnt, s);
             if (c) {
                 ++uniq;
                 if (c >= a.ci && (uint64_t)c <= a.cx) ++mine;
@@ -1800,7 +1811,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         __syncthreads();
         uint64_t pos = s_base + my_off;
         for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
-            const uint32_t c = s_cnt[s];
+            const uint32_t c = slot_count<NW>(s_k0, s_cnt, s);
             if (c && c >= a.ci && (uint64_t)c <= a.cx) {
                 emit_out<NW>(a, pos, NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s], c);
                 ++pos;
