@@ -1501,8 +1501,10 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __shared__ uint64_t s_qk[NWAVE][NW == 1 ? DEFER_CAP : 1];  // deferred keys (k <= 32)
     __shared__ uint32_t s_qs[NWAVE][NW == 1 ? DEFER_CAP : 1];  // their probe state
     __shared__ uint32_t s_red[NWAVE];
-    __shared__ uint32_t s_next, s_stop;
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_next, s_tot_u, s_tot_s;
+    // the next work item, fetched by one thread while the current one is counted
+    __shared__ uint4 s_nit;
+    __shared__ uint32_t s_nitem, s_ntag, s_nstop;
     __shared__ unsigned long long s_base;
 
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -1514,29 +1516,42 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     const uint32_t topb = NW == 1 ? 2u * k : 2u * k - 64u;
     const uint64_t kmask = topb == 64u ? ~0ull : ((1ull << topb) - 1u);
     const uint32_t rsh = topb - 2u;
-    uint32_t item = blockIdx.x;
-    uint32_t n_items = a.n_items;
-    if (BATCH) {
-        n_items = *a.n_items_dev;
-        if (threadIdx.x == 0) s_item = atomicAdd(a.item_q, 1u);
-        __syncthreads();
-        item = s_item;
-    }
+    const uint32_t n_items = BATCH ? *a.n_items_dev : a.n_items;
+    // Work item fetch: the item (from the device queue when BATCH), its
+    // bounds, job and the spill-overflow flag (after an overflow the job is
+    // counted again, so the rest is skipped), into the s_n* slots.  One thread
+    // fetches the next item while the workgroup counts the current one, so
+    // the queue and item reads are off the critical path.
+    auto fetch = [&](uint32_t ni) {
+        if (BATCH) ni = atomicAdd(a.item_q, 1u);
+        uint4 nit = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t ntag = 0;
+        if (ni < n_items) {
+            nit = a.items[ni];
+            if (BATCH) ntag = a.item_job[ni];
+        }
+        s_nstop = __hip_atomic_load(a.sp_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_nitem = ni;
+        s_nit = nit;
+        s_ntag = ntag;
+    };
+    if (threadIdx.x == 0) fetch(blockIdx.x);
+    __syncthreads();
+    uint32_t item = s_nitem, tag = s_ntag, stop = s_nstop;
+    uint4 it = s_nit;
     while (item < n_items) {
-        const uint4 it = a.items[item];
-        const uint32_t tag = BATCH ? a.item_job[item] : 0u;
         for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) {
             if (NW == 1) s_k0[i] = EMPTY64;
             s_cnt[i] = 0;
         }
         if (threadIdx.x == 0) {
             s_next = it.x;
-            // after a spill overflow the job is counted again: skip the rest
-            s_stop = __hip_atomic_load(a.sp_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_tot_u = s_tot_s = 0;
         }
         __syncthreads();
+        if (BATCH && threadIdx.x == 0) fetch(0u);
 
-        const uint32_t r1 = s_stop ? it.x : it.y, smask = it.z, sval = it.w;
+        const uint32_t r1 = stop ? it.x : it.y, smask = it.z, sval = it.w;
         uint32_t spilled = 0;
         uint32_t qn = 0;  // deferred keys on this wave's stack (k <= 32)
         // records are loaded one batch ahead: the next 64 are in flight while
@@ -1784,19 +1799,43 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         }
         __syncthreads();
 
-        // drain: count, reserve once per workgroup, write
-        uint32_t mine = 0, uniq = 0;
-        for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
+        // Drain.  Wave w owns slots [w, w+1) * CAP / NWAVE, taken 64 at a time:
+        // the count pass keeps each slot's count and the ballot of the slots
+        // to write, the wave totals give each wave its output base (one
+        // reservation per workgroup), and the write pass compacts the written
+        // slots with the ballots, so consecutive lanes write consecutive
+        // entries (coalesced stores).
+        constexpr uint32_t DR = CAP / NWAVE / 64;  // drain rounds per wave
+        static_assert(CAP % (NWAVE * 64) == 0, "drain rounds");
+        uint32_t cv[DR];
+        uint64_t em[DR];
+        uint32_t ne = 0, nu = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < DR; ++r) {
+            const uint32_t s = (wv * DR + r) * 64u + lane;
             const uint32_t c = slot_count<NW>(s_k0, s_cnt, s);
-            if (c) {
-                ++uniq;
-                if (c >= a.ci && (uint64_t)c <= a.cx) ++mine;
-            }
+            cv[r] = c;
+            em[r] = __ballot(c && c >= a.ci && (uint64_t)c <= a.cx);
+            ne += (uint32_t)__popcll(em[r]);
+            nu += (uint32_t)__popcll(__ballot(c != 0u));
         }
-        uint32_t tot_out, tot_uniq, tot_sp;
-        const uint32_t my_off = block_excl_u32(mine, s_red, tot_out);
-        block_excl_u32(uniq, s_red, tot_uniq);
-        block_excl_u32(spilled, s_red, tot_sp);
+        uint32_t ss = spilled;
+#pragma unroll
+        for (uint32_t d = 32; d; d >>= 1) ss += __shfl_xor(ss, d, 64);
+        if (lane == 0) {
+            s_red[wv] = ne;
+            if (nu) atomicAdd(&s_tot_u, nu);
+            if (ss) atomicAdd(&s_tot_s, ss);
+        }
+        __syncthreads();
+        uint32_t wbase = 0, tot_out = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < NWAVE; ++w) {
+            const uint32_t n = s_red[w];
+            wbase += w < wv ? n : 0u;
+            tot_out += n;
+        }
+        const uint32_t tot_uniq = s_tot_u, tot_sp = s_tot_s;
         if (threadIdx.x == 0) {
             s_base = tot_out ? atomicAdd(&a.counters[0], (unsigned long long)tot_out) : 0ull;
             if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
@@ -1809,19 +1848,22 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             }
         }
         __syncthreads();
-        uint64_t pos = s_base + my_off;
-        for (uint32_t s = threadIdx.x; s < CAP; s += blockDim.x) {
-            const uint32_t c = slot_count<NW>(s_k0, s_cnt, s);
-            if (c && c >= a.ci && (uint64_t)c <= a.cx) {
-                emit_out<NW>(a, pos, NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s], c);
-                ++pos;
+        uint64_t pos = s_base + wbase;
+#pragma unroll
+        for (uint32_t r = 0; r < DR; ++r) {
+            if ((em[r] >> lane) & 1ull) {
+                const uint32_t s = (wv * DR + r) * 64u + lane;
+                emit_out<NW>(a, pos + lane_rank(em[r]), NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s],
+                             cv[r]);
             }
+            pos += (uint32_t)__popcll(em[r]);
         }
         if (!BATCH) break;
         __syncthreads();  // table reads done before the next item clears it
-        if (threadIdx.x == 0) s_item = atomicAdd(a.item_q, 1u);
-        __syncthreads();
-        item = s_item;
+        item = s_nitem;
+        tag = s_ntag;
+        stop = s_nstop;
+        it = s_nit;
     }
 }
 
