@@ -1539,11 +1539,12 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
     __syncthreads();
     uint32_t item = s_nitem, tag = s_ntag, stop = s_nstop;
     uint4 it = s_nit;
+    // the table starts empty; the drain empties every slot it reads
+    for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) {
+        if (NW == 1) s_k0[i] = EMPTY64;
+        s_cnt[i] = 0;
+    }
     while (item < n_items) {
-        for (uint32_t i = threadIdx.x; i < CAP; i += blockDim.x) {
-            if (NW == 1) s_k0[i] = EMPTY64;
-            s_cnt[i] = 0;
-        }
         if (threadIdx.x == 0) {
             s_next = it.x;
             s_tot_u = s_tot_s = 0;
@@ -1804,7 +1805,8 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         // to write, the wave totals give each wave its output base (one
         // reservation per workgroup), and the write pass compacts the written
         // slots with the ballots, so consecutive lanes write consecutive
-        // entries (coalesced stores).
+        // entries (coalesced stores), and empties the wave's slots for the
+        // next item (the next item's first barrier orders that).
         constexpr uint32_t DR = CAP / NWAVE / 64;  // drain rounds per wave
         static_assert(CAP % (NWAVE * 64) == 0, "drain rounds");
         uint32_t cv[DR];
@@ -1851,15 +1853,15 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         uint64_t pos = s_base + wbase;
 #pragma unroll
         for (uint32_t r = 0; r < DR; ++r) {
-            if ((em[r] >> lane) & 1ull) {
-                const uint32_t s = (wv * DR + r) * 64u + lane;
+            const uint32_t s = (wv * DR + r) * 64u + lane;
+            if ((em[r] >> lane) & 1ull)
                 emit_out<NW>(a, pos + lane_rank(em[r]), NW == 2 ? s_k0[s] : 0ull, NW == 2 ? s_k1[s] : s_k0[s],
                              cv[r]);
-            }
             pos += (uint32_t)__popcll(em[r]);
+            if (NW == 1) s_k0[s] = EMPTY64;
+            s_cnt[s] = 0;
         }
         if (!BATCH) break;
-        __syncthreads();  // table reads done before the next item clears it
         item = s_nitem;
         tag = s_ntag;
         stop = s_nstop;
