@@ -705,6 +705,7 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
     else if (a.W <= 12) TG_SK_RW(12);
     else if (a.W <= 16) TG_SK_RW(16);
     else if (a.W <= 32) TG_SK_RW(32);
+    else if (a.W <= 48) TG_SK_RW(48);
     else TG_SK_RW(64);
 #undef TG_SK_RW
 #undef TG_SK
