@@ -248,10 +248,15 @@ void launch_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t n, uint64_t*
 // the current superkmer already holds capk = 256-k k-mers (the 255-base cap of
 // the one-byte record header, :61-63,81).
 struct SkLane {
-    // packed-stream reader + rolling hash (include/nt_hash.hpp:114-118)
+    // packed-stream reader + rolling hash (include/nt_hash.hpp:114-118).  The
+    // next bases sit top-aligned in buf; every 16 rolls (the same rolls for
+    // all lanes of the wave still rolling) the word loaded 16 rolls earlier is
+    // appended at bit 2*phase and the next one is loaded, so the refill is a
+    // wave-uniform branch and its load has 16 rolls to arrive.
     const uint32_t* packed;
-    uint64_t widx;
-    uint32_t wrd, rem, nextw;   // nextw: the word after wrd, loaded one word ahead
+    uint64_t widx;              // index of the word in nxt
+    uint64_t buf;
+    uint32_t nxt, phase2, u;    // prefetched word, 2 * (start base & 15), rolls so far
     uint64_t f, rv, lmer;
     uint32_t osh;
     // cut state
@@ -267,14 +272,15 @@ struct SkLane {
 
 __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uint64_t* tr)
 {
-    if (L.rem == 0) {
-        L.wrd = L.nextw;
-        L.nextw = L.packed[++L.widx + 1];  // the stream keeps 8 spare words
-        L.rem = 16;
+    // the lanes that roll agree on u
+    const uint32_t u = __builtin_amdgcn_readfirstlane(L.u);
+    if (u && (u & 15u) == 0) {
+        L.buf |= (uint64_t)L.nxt << L.phase2;
+        L.nxt = L.packed[++L.widx];  // the stream keeps 8 spare words
     }
-    const uint32_t kb = L.wrd >> 30;
-    L.wrd <<= 2;
-    --L.rem;
+    L.u = u + 1u;
+    const uint32_t kb = (uint32_t)(L.buf >> 62);
+    L.buf <<= 2;
     const uint32_t in = kb ^ (kb >> 1);  // kache -> nt code
     const uint32_t o = (uint32_t)(L.lmer >> L.osh) & 3u;
     L.lmer = (L.lmer << 2) | in;
@@ -506,10 +512,12 @@ __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, 
     if (a.lookback != LOOKBACK_FULL && L.tb > a.lookback) L.ts = ((L.tb - a.lookback) / a.W) * a.W;
     L.packed = a.packed;
     const uint64_t gq = L.cb + L.ts;
-    L.widx = gq >> 4;
-    L.wrd = a.packed[L.widx] << (2u * (uint32_t)(gq & 15u));
-    L.nextw = a.packed[L.widx + 1];
-    L.rem = 16u - (uint32_t)(gq & 15u);
+    const uint64_t w0 = gq >> 4;
+    L.phase2 = 2u * (uint32_t)(gq & 15u);
+    L.buf = (((uint64_t)a.packed[w0] << 32) | a.packed[w0 + 1]) << L.phase2;
+    L.widx = w0 + 2;
+    L.nxt = a.packed[w0 + 2];
+    L.u = 0;
     L.f = s_init[0];
     L.rv = s_init[1];
     L.lmer = 0;
