@@ -1742,8 +1742,14 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                             miss = true;
                         }
                     } else {
-                        const uint32_t c = __hip_atomic_load(&s_cnt[slot], __ATOMIC_ACQUIRE,
+                        // the slot state, then its key words, read without a wait in
+                        // between: LDS serves a wave's accesses in order and a claim
+                        // writes the key before it publishes the count, so a published
+                        // count read first means the key read after it is complete
+                        const uint32_t c = __hip_atomic_load(&s_cnt[slot], __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __asm__ volatile("" ::: "memory");
+                        const uint64_t sk0 = s_k0[slot], sk1 = s_k1[slot];
                         if (c == 0u) {
                             // claim; a lost race retries the same slot next time
                             if (atomicCAS(&s_cnt[slot], 0u, LOCKED32) == 0u) {
@@ -1754,7 +1760,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                                 adv = true;
                             }
                         } else if (c != LOCKED32) {
-                            if (s_k0[slot] == xh && s_k1[slot] == xl) {
+                            if (sk0 == xh && sk1 == xl) {
                                 atomicAdd(&s_cnt[slot], 1u);
                                 adv = true;
                             } else {
