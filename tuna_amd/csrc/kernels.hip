@@ -401,7 +401,40 @@ struct SkStage {
     uint32_t pushed, flushed;
 };
 
+// The record writes of a flush, out of line (a flush comes every few dozen
+// k-mers and is inlined at every k-mer of the unrolled window loop otherwise);
+// its arguments are plain values so that nothing is copied to the stack.
 template <int RW>
+__device__ __forceinline__ void sk_write_recs(const uint2* buf, uint32_t cnt, uint32_t rank, uint32_t nact,
+                                              uint64_t at, uint32_t* recs, uint32_t* metas,
+                                              const uint32_t* packed)
+{
+    for (uint32_t i = rank; i < cnt; i += nact) {
+        const uint2 e = buf[i];
+        if (RW == 2) {
+            Rec<RW> r;
+            r.w[0] = e.x;
+            r.w[1] = e.y;
+            store_rec<RW>(recs, at + i, r);
+        } else {
+            store_rec<RW>(recs, at + i, fat_record<RW>(packed, e.x, e.y));
+        }
+        metas[at + i] = e.y;
+    }
+}
+
+template <int RW>
+__device__ __noinline__ void sk_write_out(const uint2* buf, uint32_t cnt, uint32_t rank, uint32_t nact,
+                                          uint64_t at, uint32_t* recs, uint32_t* metas,
+                                          const uint32_t* packed)
+{
+    sk_write_recs<RW>(buf, cnt, rank, nact, at, recs, metas, packed);
+}
+
+// OL: the record writes out of line (the large windows, whose unrolled loop
+// has many k-mers and registers to spare for a call; the small ones would
+// spill registers around it)
+template <int RW, bool OL>
 __device__ __forceinline__ void sk_flush(SkStage& S, const SkArgs& a, uint32_t cnt)
 {
     const uint64_t act = __ballot(1);
@@ -409,25 +442,15 @@ __device__ __forceinline__ void sk_flush(SkStage& S, const SkArgs& a, uint32_t c
     uint32_t base = 0;
     if (rank == 0) base = atomicAdd(S.wg_n, cnt);
     base = __builtin_amdgcn_readfirstlane(base);
-    for (uint32_t i = rank; i < cnt; i += nact) {
-        const uint2 e = S.buf[i];
-        if (RW == 2) {
-            Rec<RW> r;
-            r.w[0] = e.x;
-            r.w[1] = e.y;
-            store_rec<RW>(a.recs, S.run + base + i, r);
-        } else {
-            store_rec<RW>(a.recs, S.run + base + i, fat_record<RW>(a.packed, e.x, e.y));
-        }
-        a.metas[S.run + base + i] = e.y;
-    }
+    if (OL) sk_write_out<RW>(S.buf, cnt, rank, nact, S.run + base, a.recs, a.metas, a.packed);
+    else sk_write_recs<RW>(S.buf, cnt, rank, nact, S.run + base, a.recs, a.metas, a.packed);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     S.flushed = S.pushed;
 }
 
-template <int RW>
+template <int RW, bool OL>
 __device__ __forceinline__ void sk_push(SkStage& S, const SkArgs& a, bool e, uint2 rec)
 {
     const uint64_t mask = __ballot(e);
@@ -442,7 +465,7 @@ __device__ __forceinline__ void sk_push(SkStage& S, const SkArgs& a, bool e, uin
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        sk_flush<RW>(S, a, cnt);
+        sk_flush<RW, OL>(S, a, cnt);
     }
 }
 
@@ -464,7 +487,7 @@ __device__ __forceinline__ uint2 sk_entry(const SkLane& L, const SkArgs& a, uint
 // so the active lanes stay converged for the one ballot in sk_push.  With fat
 // records a piece ends after a.piece k-mers (a cut that starts no new
 // superkmer).
-template <int RW>
+template <int RW, bool OL>
 __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, uint32_t t,
                                         uint64_t Hc, uint32_t Pc)
 {
@@ -487,7 +510,7 @@ __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, 
     L.tail = last && L.have && L.t0 >= L.tb;   // [t0p, K) is emitted after the loop
     L.flagged = L.flagged || flag;
     L.done = stop || last;
-    sk_push<RW>(S, a, emit, rec);
+    sk_push<RW, OL>(S, a, emit, rec);
 }
 
 __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, const uint64_t* s_init)
@@ -644,7 +667,7 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
                         Hc = pv;
                         Pc = bs + W + pp;
                     }
-                    sk_kmer<RW>(L, a, S, t, Hc, Pc);
+                    sk_kmer<RW, (WMAX > 16)>(L, a, S, t, Hc, Pc);
                     if (!L.done) {
                         sk_roll(L, tf, tr);
                         const uint64_t h = L.f ^ L.rv;
@@ -677,13 +700,13 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
         S.pushed = max(S.pushed, (uint32_t)__shfl_xor((int)S.pushed, d, 64));
         S.flushed = max(S.flushed, (uint32_t)__shfl_xor((int)S.flushed, d, 64));
     }
-    sk_push<RW>(S, a, tail, tail_rec);
+    sk_push<RW, (WMAX > 16)>(S, a, tail, tail_rec);
     const uint32_t left = S.pushed - S.flushed;
     if (left) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        sk_flush<RW>(S, a, left);
+        sk_flush<RW, (WMAX > 16)>(S, a, left);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
