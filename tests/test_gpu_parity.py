@@ -107,7 +107,8 @@ def test_partition_records_match_reference(eng, name):
 
 
 @pytest.mark.parametrize("k,m", [(31, 21), (31, 23), (21, 15), (63, 21), (11, 5), (5, 3), (9, 8),
-                                 (64, 31), (32, 16), (17, 2), (3, 2), (45, 9), (33, 15), (7, 1)])
+                                 (64, 31), (32, 16), (17, 2), (3, 2), (45, 9), (33, 15), (7, 1),
+                                 (57, 6), (64, 1)])
 def test_superkmers_match_oracle(eng, k, m):
     rng = random.Random(k * 100 + m)
     strings = [rs(rng, rng.randint(1, 400)) for _ in range(300)]
@@ -129,7 +130,7 @@ def test_superkmers_match_oracle(eng, k, m):
 
 
 @pytest.mark.parametrize("k,m", [(31, 21), (21, 15), (63, 21), (2, 1), (5, 3), (15, 7), (32, 16),
-                                 (33, 15), (45, 9), (64, 31), (25, 11)])
+                                 (33, 15), (45, 9), (64, 31), (25, 11), (57, 6), (64, 1)])
 def test_counts_match_oracle_random_chunks(ceng, k, m):
     rng = random.Random(1000 + k)
     strings = [rs(rng, rng.randint(1, 300)) for _ in range(500)] + [rs(rng, 20000)]
