@@ -104,6 +104,19 @@ int tg_count_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases,
  * its steps (calibration and sizing read back in between), as
  * partition_kmers_mem then count_and_write_mem run in the reference. */
 int tg_set_plain_sequence(tg_ctx* ctx, int on);
+/* Debug statistics per reference partition (after src/count.hpp:42-60 and
+ * emit_debug_stats :478-565, which the reference collects under -dbg).  With
+ * on = 1, jobs counted with the plain sequence record, per partition: k-mers
+ * inserted (with multiplicity), distinct keys held in the LDS tables, k-mers
+ * sent to the HBM spill table (the reference's n_overflow) and the table slots
+ * used (work items x LDS table size; tables are fixed, never resized).  Work
+ * items then never span two partitions; results are unchanged. */
+int tg_set_debug_stats(tg_ctx* ctx, int on);
+/* Copies the statistics of the last such job; returns the number of
+ * partitions, TG_ECAP when n is smaller, TG_EINVAL when none were recorded.
+ * Null arrays are skipped. */
+int64_t tg_debug_partitions(tg_ctx* ctx, uint64_t* inserted, uint64_t* lds_unique, uint64_t* spilled,
+                            uint64_t* table_cap, uint64_t n);
 /* Result of the last job: n_out (key, count) pairs in unspecified order. */
 uint64_t tg_result_size(tg_ctx* ctx);
 /* keys: n*ceil(k/32) words; counts: n.  Copies min(n, cap) pairs. */

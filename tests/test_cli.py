@@ -223,6 +223,45 @@ def test_cli_counts_match_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_debug_table_stats(tmp_path):
+    # -dbg: the reference's per-partition table report (src/count.hpp:478-565),
+    # with per-partition insertions and distinct keys pinned to the oracle's
+    # superkmers (a k-mer's partition is its minimizer's), counts unchanged
+    text = bytes(T.synth("kind=reads_fq genome=40000 seed=3 cov=15 rl=150 sub=0.005 nrate=0"))
+    fq = tmp_path / "r.fq"
+    fq.write_bytes(text)
+    out, work = tmp_path / "o.tsv", tmp_path / "work"
+    n, k, m = 16, 31, 21
+    r = run_cli("-k", str(k), "-m", str(m), "-n", str(n), "-dbg", "-w", str(work), str(fq), str(out))
+    assert r.returncode == 0, r.stderr
+    assert read_tsv(out) == oracle_tsv([text], k)
+    err = r.stderr
+    for line in ("[debug] per-partition table stats (first 20):", "[debug] aggregate table stats (16 partitions):",
+                 "dbg_load_mean: ", "dbg_unique_mean: ", "dbg_n_resizes: 0", "[debug] resize summary:"):
+        assert line in err, line
+    rows = [l.split(",") for l in open(work / "debug_table_stats.csv").read().splitlines()]
+    assert rows[0] == ["partition_id", "init_sz", "table_cap", "n_inserted", "n_unique", "load_factor",
+                       "n_resizes", "resize_s", "n_overflow"]
+    rows = rows[1:]
+    assert [int(x[0]) for x in rows] == list(range(n))
+    b, o = O.parse(text)
+    st, ln, mp, hs, km = O.superkmers(b, o, k, m)
+    part = (hs.astype(np.uint64) & np.uint64(n - 1)).astype(np.int64)
+    for p_ in range(n):
+        sel = part == p_
+        ins = int((ln[sel].astype(np.int64) - k + 1).sum())
+        # the partition's distinct k-mers: count its superkmers as chunks
+        starts = st[sel].astype(np.uint64)
+        sb = np.concatenate([b[int(s0):int(s0) + int(l0)] for s0, l0 in zip(starts, ln[sel])]) if len(starts) else b[:0]
+        offs = np.concatenate([[0], np.cumsum(ln[sel].astype(np.uint64))]).astype(np.uint64)
+        _, _, tot, nu = O.count(sb, offs, k, 1, 2**64 - 1)
+        assert int(rows[p_][3]) == ins == tot, p_
+        assert int(rows[p_][8]) == 0
+        assert int(rows[p_][4]) == nu, p_
+        assert int(rows[p_][1]) == int(rows[p_][2]) and int(rows[p_][2]) % 4096 == 0
+
+
+@pytest.mark.gpu
 def test_cli_kff_list_gzip_and_multi_file(tmp_path):
     rng = random.Random(6)
     texts = [">a\n" + rs(rng, 3000) + "\n", ">b\n" + rs(rng, 2000) + "N" + rs(rng, 500) + "\n"]

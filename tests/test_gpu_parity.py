@@ -197,3 +197,34 @@ def test_repeated_calls_reuse_buffers(ceng):
         strings = [rs(rng, rng.randint(50, 2000)) for _ in range(50 - 20 * trial)]
         bases, off = O.chunks_of(strings)
         check_counts(ceng, bases, off, 31, 21, 16, label=f"reuse{trial}")
+
+
+@pytest.mark.parametrize("k,m,n", [(31, 21, 64), (63, 21, 16), (21, 15, 256)])
+def test_debug_partition_stats(k, m, n):
+    # per-partition statistics (the reference's -dbg table report): insertions
+    # and distinct keys per partition add up to the job's totals, the counts
+    # equal the oracle's with the statistics on, and none exist before
+    e = T.Engine(0)
+    try:
+        e.set_plain_sequence(True)
+        with pytest.raises(T.TunaGpuError, match="no debug statistics"):
+            e.debug_partitions(n)
+        e.set_debug_stats(True)
+        text = T.synth("kind=reads_fq genome=60000 seed=7 cov=12 rl=150 sub=0.005 nrate=0.001")
+        b, o = T.parse_buffer(text)
+        say(f"debug stats k={k} m={m} n={n}")
+        keys, cnt = e.count(b, o, k, m, n)
+        st = e.last_stats
+        d = e.debug_partitions(n)
+        assert len(d["inserted"]) == n
+        assert int(d["inserted"].sum()) == st.n_kmers
+        assert int(d["lds_unique"].sum()) <= st.n_unique
+        if int(d["spilled"].sum()) == 0:
+            assert int(d["lds_unique"].sum()) == st.n_unique
+        assert np.all(d["table_cap"] % 1024 == 0) and np.all(d["lds_unique"] <= d["table_cap"])
+        ok, oc, tot, nu = O.count(b, o, k, 1, 2**64 - 1)
+        assert st.n_unique == nu and st.n_kmers == tot
+        with pytest.raises(T.TunaGpuError):
+            e.debug_partitions(n // 2)
+    finally:
+        e.close()

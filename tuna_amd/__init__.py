@@ -35,6 +35,7 @@ EXPORTS = [
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
+    "tg_set_debug_stats", "tg_debug_partitions",
     "tg_result_tsv", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
     "tg_unmap_file",
 ]
@@ -197,6 +198,9 @@ def lib():
     L.tg_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(_vp)]
     L.tg_host_free.argtypes = [_vp]
     L.tg_set_plain_sequence.argtypes = [_vp, ctypes.c_int]
+    L.tg_set_debug_stats.argtypes = [_vp, ctypes.c_int]
+    L.tg_debug_partitions.argtypes = [_vp, _u64p, _u64p, _u64p, _u64p, ctypes.c_uint64]
+    L.tg_debug_partitions.restype = ctypes.c_int64
     L.tg_autotune_many.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
                                    ctypes.c_uint64, _u64p, _u64p, ctypes.c_uint32,
                                    ctypes.POINTER(TgBatchConfig), ctypes.POINTER(ctypes.c_double)]
@@ -451,6 +455,19 @@ class Engine:
         """Run single jobs with the plain host-driven launch sequence (True)
         or through the batched engine as a batch of one (False, default)."""
         _check(lib().tg_set_plain_sequence(self._h, 1 if on else 0))
+
+    def set_debug_stats(self, on=True):
+        """Record table statistics per reference partition for jobs counted
+        with the plain sequence (the reference's -dbg table report)."""
+        _check(lib().tg_set_debug_stats(self._h, 1 if on else 0))
+
+    def debug_partitions(self, n):
+        """Per-partition statistics of the last job: k-mers inserted,
+        distinct keys in the LDS tables, k-mers spilled, table slots."""
+        a = [np.zeros(n, np.uint64) for _ in range(4)]
+        got = lib().tg_debug_partitions(self._h, *[x.ctypes.data_as(_u64p) for x in a], n)
+        _check_size(got)
+        return dict(zip(("inserted", "lds_unique", "spilled", "table_cap"), [x[:got] for x in a]))
 
     def reset_kernel_times(self):
         lib().tg_reset_kernel_times(self._h)

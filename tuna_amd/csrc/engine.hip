@@ -243,9 +243,12 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
 // Work items over partitions [p_begin, p_end): consecutive sub-partitions
 // merged while their estimated distinct k-mers (k-mers x rho) fit one LDS table
 // at ~60% load; larger ones split by minimizer-hash bits.
+// part_shift: with item_part, groups stop at reference partitions (sub-partition
+// >> part_shift) and item_part gets each item's partition
 void build_items(const Phase1& o, const std::vector<uint64_t>& km, const std::vector<uint64_t>& off,
                  uint32_t cap, double rho, const std::vector<uint8_t>* skip, uint32_t p_begin,
-                 uint32_t p_end, uint32_t stride, std::vector<uint4>& items)
+                 uint32_t p_end, uint32_t stride, std::vector<uint4>& items,
+                 std::vector<uint32_t>* item_part = nullptr, uint32_t part_shift = 0)
 {
     const double target = 0.75 * cap;
     const uint32_t rmax = 1u << o.sb;
@@ -259,8 +262,10 @@ void build_items(const Phase1& o, const std::vector<uint64_t>& km, const std::ve
         if (est > target) {
             uint32_t r = 1;
             while (r < rmax && est / r > target) r <<= 1;
-            for (uint32_t j = 0; j < r; ++j)
+            for (uint32_t j = 0; j < r; ++j) {
                 items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[p + 1], r - 1, j));
+                if (item_part) item_part->push_back(p >> part_shift);
+            }
             p += stride;
             continue;
         }
@@ -270,17 +275,21 @@ void build_items(const Phase1& o, const std::vector<uint64_t>& km, const std::ve
         while (q < q_end && !(skip && (*skip)[q])) {
             const double e = (double)km[q] * rho;
             if (e > target || (q > p && acc + e > target)) break;
+            if (item_part && (q >> part_shift) != (p >> part_shift)) break;
             acc += e;
             ++q;
         }
         if (q == p) q = p + 1;
-        if (off[q] > off[p]) items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[q], 0u, 0u));
+        if (off[q] > off[p]) {
+            items.push_back(make_uint4((uint32_t)off[p], (uint32_t)off[q], 0u, 0u));
+            if (item_part) item_part->push_back(p >> part_shift);
+        }
         p = stride > 1 ? p + stride : q;
     }
 }
 
 int launch_items(tg_ctx* c, const char* buf, const std::vector<uint4>& items, tg::CountArgs& a,
-                 uint32_t nw, uint32_t** item_uniq)
+                 uint32_t nw, uint32_t** item_uniq, uint32_t** item_sp = nullptr, bool sample = false)
 {
     if (items.empty()) return TG_OK;
     int rc;
@@ -294,9 +303,14 @@ int launch_items(tg_ctx* c, const char* buf, const std::vector<uint4>& items, tg
     } else {
         a.item_uniq = nullptr;
     }
+    a.item_sp = nullptr;
+    if (item_sp) {
+        if ((rc = ensure_t(c, "item_sp", items.size() + 1, item_sp))) return rc;
+        a.item_sp = *item_sp;
+    }
     a.items = d_items;
     a.n_items = (uint32_t)items.size();
-    Prof pr(c, item_uniq ? "count_sample" : "count");
+    Prof pr(c, sample ? "count_sample" : "count");
     tg::launch_count(c->stream, a, nw);
     return TG_OK;
 }
@@ -355,14 +369,35 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
         std::vector<uint4> items;
         double rho = 1.0;
         const uint32_t stride = o.nsp >= 256 ? o.nsp / 64 : 0;
+        // debug statistics per reference partition (src/count.hpp:42-60)
+        const bool dbg = c->debug;
+        uint32_t pshift = 0;
+        while ((o.n << pshift) < o.nsp) ++pshift;
+        std::vector<uint64_t> d_ins(dbg ? o.n : 0), d_uni(dbg ? o.n : 0), d_sp(dbg ? o.n : 0),
+            d_cap(dbg ? o.n : 0);
+        std::vector<uint32_t> ipart;
+        auto dbg_take = [&](const std::vector<uint32_t>& uq, uint32_t* dev_sp) -> int {
+            std::vector<uint32_t> sp(ipart.size());
+            if (!sp.empty())
+                HIPOK(hipMemcpyAsync(sp.data(), dev_sp, sp.size() * 4, hipMemcpyDeviceToHost, st));
+            HIPOK(hipStreamSynchronize(st));
+            for (size_t i = 0; i < ipart.size(); ++i) {
+                d_uni[ipart[i]] += uq[i] - sp[i];
+                d_sp[ipart[i]] += sp[i];
+                d_cap[ipart[i]] += cap;
+            }
+            return TG_OK;
+        };
         if (stride) {
-            build_items(o, km, off, cap, 1.0, nullptr, 0, o.nsp, stride, items);
-            uint32_t* item_uniq = nullptr;
-            if ((rc = launch_items(c, "items_cal", items, a, nw, &item_uniq))) return rc;
+            build_items(o, km, off, cap, 1.0, nullptr, 0, o.nsp, stride, items, dbg ? &ipart : nullptr, pshift);
+            uint32_t *item_uniq = nullptr, *item_sp = nullptr;
+            if ((rc = launch_items(c, "items_cal", items, a, nw, &item_uniq, dbg ? &item_sp : nullptr, true)))
+                return rc;
             std::vector<uint32_t> uq(items.size());
             if (!items.empty())
                 HIPOK(hipMemcpyAsync(uq.data(), item_uniq, items.size() * 4, hipMemcpyDeviceToHost, st));
             HIPOK(hipStreamSynchronize(st));
+            if (dbg && (rc = dbg_take(uq, item_sp))) return rc;
             uint64_t su = 0, sk = 0;
             for (size_t i = 0; i < uq.size(); ++i) su += uq[i];
             for (uint32_t q = 0; q < o.nsp; q += stride) {
@@ -372,8 +407,12 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
             if (sk) rho = std::min(1.0, std::max(0.02, 1.15 * (double)su / (double)sk));
         }
         items.clear();
-        build_items(o, km, off, cap, rho, stride ? &done : nullptr, 0, o.nsp, 1, items);
-        if ((rc = launch_items(c, "items", items, a, nw, nullptr))) return rc;
+        ipart.clear();
+        build_items(o, km, off, cap, rho, stride ? &done : nullptr, 0, o.nsp, 1, items, dbg ? &ipart : nullptr,
+                    pshift);
+        uint32_t *main_uniq = nullptr, *main_sp = nullptr;
+        if ((rc = launch_items(c, "items", items, a, nw, dbg ? &main_uniq : nullptr, dbg ? &main_sp : nullptr)))
+            return rc;
         {
             Prof pr(c, "spill_drain");
             tg::launch_spill_drain(st, a, nw, sp_slots);
@@ -388,6 +427,17 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
             if (attempt >= 4) return fail(TG_ENOMEM, "spill table overflow after %d retries", attempt);
             sp_slots <<= 2;
             continue;
+        }
+        if (dbg) {
+            std::vector<uint32_t> uq(items.size());
+            if (!items.empty())
+                HIPOK(hipMemcpyAsync(uq.data(), main_uniq, items.size() * 4, hipMemcpyDeviceToHost, st));
+            if ((rc = dbg_take(uq, main_sp))) return rc;
+            for (uint32_t q = 0; q < o.nsp; ++q) d_ins[q >> pshift] += km[q];
+            c->dbg_inserted = d_ins;
+            c->dbg_unique = d_uni;
+            c->dbg_spilled = d_sp;
+            c->dbg_cap = d_cap;
         }
         c->n_out = hc[0];
         c->nw = nw;
@@ -582,6 +632,34 @@ int tg_set_plain_sequence(tg_ctx* c, int on)
     if (!c) return fail(TG_EINVAL, "null context");
     c->plain = on != 0;
     return TG_OK;
+}
+
+int tg_set_debug_stats(tg_ctx* c, int on)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    c->debug = on != 0;
+    c->dbg_inserted.clear();
+    c->dbg_unique.clear();
+    c->dbg_spilled.clear();
+    c->dbg_cap.clear();
+    return TG_OK;
+}
+
+int64_t tg_debug_partitions(tg_ctx* c, uint64_t* inserted, uint64_t* lds_unique, uint64_t* spilled,
+                            uint64_t* table_cap, uint64_t n)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    const uint64_t np = c->dbg_inserted.size();
+    if (np == 0) return fail(TG_EINVAL, "no debug statistics: enable them and count with the plain sequence");
+    if (n < np) return fail(TG_ECAP, "room for %llu partitions, %llu needed", (unsigned long long)n,
+                            (unsigned long long)np);
+    for (uint64_t i = 0; i < np; ++i) {
+        if (inserted) inserted[i] = c->dbg_inserted[i];
+        if (lds_unique) lds_unique[i] = c->dbg_unique[i];
+        if (spilled) spilled[i] = c->dbg_spilled[i];
+        if (table_cap) table_cap[i] = c->dbg_cap[i];
+    }
+    return (int64_t)np;
 }
 
 int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,

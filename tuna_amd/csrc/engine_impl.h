@@ -88,6 +88,10 @@ struct tg_ctx {
     // single jobs: false = through the batched engine (one captured launch
     // sequence, no host round trip); true = the plain host-driven sequence
     bool plain = false;
+    // debug statistics per reference partition of the last plain-sequence job
+    // (work items then never span two partitions)
+    bool debug = false;
+    std::vector<uint64_t> dbg_inserted, dbg_unique, dbg_spilled, dbg_cap;
 };
 
 namespace tgi {

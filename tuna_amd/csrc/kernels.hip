@@ -1881,6 +1881,7 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
             if (tot_sp) atomicAdd(&a.counters[2], (unsigned long long)tot_sp);
             if (a.item_uniq) a.item_uniq[item] = tot_uniq + tot_sp;
+            if (a.item_sp) a.item_sp[item] = tot_sp;
             if (BATCH) {
                 a.seg_pos[item] = s_base;
                 a.seg_n[item] = tot_out;

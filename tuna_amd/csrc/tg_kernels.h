@@ -115,6 +115,7 @@ struct CountArgs {
     unsigned long long* counters;  // [0] n_out, [1] n_unique, [2] spilled k-mers
     uint64_t out_cap;
     uint32_t* item_uniq;       // optional: distinct keys per item (calibration)
+    uint32_t* item_sp;         // optional: k-mers spilled per item (debug statistics)
     // batched jobs (persistent launch): items are taken from a device queue;
     // spill keys carry the job as a tag; each item reports its output segment
     const uint32_t* n_items_dev;

@@ -13,6 +13,8 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <iostream>
 #include <limits>
 #include <memory>
@@ -347,13 +349,110 @@ int write_partition_files(const Config& cfg, tg_ctx* ctx, const tg_params& p, co
     return 0;
 }
 
+// Table statistics per reference partition, printed and written as
+// src/count.hpp:478-565 (emit_debug_stats) does.  A partition's tables are its
+// work items' LDS tables: fixed in size (init_sz = table_cap, no resizes).
+// n_unique counts the keys held in them; keys that went to the HBM spill
+// table (n_overflow, the last CSV column) are reported in total only.
+int print_table_stats(const Config& cfg, tg_ctx* ctx)
+{
+    const uint64_t n = cfg.num_partitions;
+    std::vector<uint64_t> ins(n), uni(n), sp(n), cap(n);
+    const int64_t np = tg_debug_partitions(ctx, ins.data(), uni.data(), sp.data(), cap.data(), n);
+    if (np < 0) {
+        std::cerr << "tuna: error: " << tg_last_error() << "\n";
+        return 1;
+    }
+    std::ostringstream o;
+    o << "\n[debug] per-partition table stats (first 20):\n"
+      << "  part   init_sz   table_cap   n_inserted    n_unique  load%  n_resizes\n";
+    auto lf = [&](uint64_t i) { return cap[i] ? (double)uni[i] / (double)cap[i] : 0.0; };
+    for (int64_t i = 0; i < np && i < 20; ++i)
+        o << "  " << std::setw(5) << i << "  " << std::setw(8) << cap[i] << "  " << std::setw(9) << cap[i]
+          << "  " << std::setw(11) << ins[i] << "  " << std::setw(10) << uni[i] << "  " << std::fixed
+          << std::setprecision(1) << std::setw(5) << 100.0 * lf(i) << "%  0\n";
+    if (np > 20) o << "  ... (" << (np - 20) << " more partitions)\n";
+    double s_lf = 0, lo_lf = 2, hi_lf = -1, s_or = 0, lo_or = 1e18, hi_or = -1e18;
+    uint64_t s_u = 0, lo_u = UINT64_MAX, hi_u = 0, s_c = 0, lo_c = UINT64_MAX, hi_c = 0, spilled = 0;
+    int64_t valid = 0, nonempty = 0;
+    for (int64_t i = 0; i < np; ++i) {
+        spilled += sp[i];
+        if (!cap[i]) continue;
+        ++valid;
+        s_lf += lf(i);
+        lo_lf = std::min(lo_lf, lf(i));
+        hi_lf = std::max(hi_lf, lf(i));
+        s_u += uni[i];
+        lo_u = std::min(lo_u, uni[i]);
+        hi_u = std::max(hi_u, uni[i]);
+        s_c += cap[i];
+        lo_c = std::min(lo_c, cap[i]);
+        hi_c = std::max(hi_c, cap[i]);
+        if (uni[i]) {
+            ++nonempty;
+            const double r = (double)cap[i] / (double)uni[i];
+            s_or += r;
+            lo_or = std::min(lo_or, r);
+            hi_or = std::max(hi_or, r);
+        }
+    }
+    if (valid) {
+        o << "\n[debug] aggregate table stats (" << np << " partitions):\n" << std::fixed;
+        o << "  load_factor:    mean=" << std::setprecision(3) << s_lf / valid << "  min=" << lo_lf
+          << "  max=" << hi_lf << "\n";
+        if (nonempty)
+            o << "  oversize_ratio: mean=" << std::setprecision(1) << s_or / nonempty << "  min=" << lo_or
+              << "  max=" << hi_or << "  (init_sz / n_unique)\n";
+        o << "  n_unique/part:  mean=" << std::setprecision(0) << (double)s_u / valid << "  min=" << lo_u
+          << "  max=" << hi_u << "\n";
+        o << "  init_sz/part:   mean=" << (double)s_c / valid << "  min=" << lo_c << "  max=" << hi_c << "\n";
+        o << "dbg_load_mean: " << std::setprecision(6) << s_lf / valid << "\n"
+          << "dbg_load_min: " << lo_lf << "\n"
+          << "dbg_load_max: " << hi_lf << "\n"
+          << "dbg_oversize_mean: " << std::setprecision(2) << (nonempty ? s_or / nonempty : 0.0) << "\n"
+          << "dbg_unique_mean: " << std::setprecision(0) << (double)s_u / valid << "\n"
+          << "dbg_n_resizes: 0\n"
+          << "dbg_parts_resized: 0\n";
+    }
+    if (spilled)
+        o << "[debug] k-mers sent to the HBM spill table: " << spilled
+          << " (their distinct keys are in unique_kmers, not in n_unique per partition)\n";
+    std::error_code ec;
+    std::filesystem::create_directories(cfg.work_dir, ec);
+    const std::string path = cfg.work_dir + "debug_table_stats.csv";
+    std::ofstream csv(path);
+    if (csv) {
+        csv << "partition_id,init_sz,table_cap,n_inserted,n_unique,load_factor,n_resizes,resize_s,n_overflow\n";
+        for (int64_t i = 0; i < np; ++i)
+            csv << i << "," << cap[i] << "," << cap[i] << "," << ins[i] << "," << uni[i] << "," << std::fixed
+                << std::setprecision(6) << lf(i) << ",0," << 0.0 << "," << sp[i] << "\n";
+        o << "[debug] table stats CSV: " << path << "\n";
+    } else {
+        o << "[debug] warning: could not write table stats CSV to " << path << "\n";
+    }
+    o << "\n[debug] resize summary:\n"
+      << "  partitions with resizes : 0 / " << np << "\n"
+      << "  total resize events     : 0\n"
+      << "  total resize time       : 0.000s\n"
+      << "  overflow-triggered      : 0\n"
+      << "  load-triggered          : 0\n";
+    std::cerr << o.str();
+    return 0;
+}
+
 int run(Config& cfg)
 {
     const auto t_start = std::chrono::steady_clock::now();
     tg_ctx* ctx = nullptr;
     check(tg_ctx_create(0, &ctx));
     std::unique_ptr<tg_ctx, void (*)(tg_ctx*)> hold(ctx, tg_ctx_destroy);
-    if (cfg.debug_stats) tg_set_profiling(ctx, 1);
+    if (cfg.debug_stats) {
+        // per-kernel times, and table statistics per partition (the plain
+        // launch sequence records them; the counts are the same either way)
+        tg_set_profiling(ctx, 1);
+        check(tg_set_plain_sequence(ctx, 1));
+        check(tg_set_debug_stats(ctx, 1));
+    }
 
     if (!cfg.hide_progress)
         std::cerr << "[1/2] partitioning  (" << cfg.num_threads << " thread"
@@ -420,6 +519,7 @@ int run(Config& cfg)
             std::cerr << "[dbg] kernel " << q << " " << ms[i] << " ms x" << cnt[i] << "\n";
             q += strlen(q) + 1;
         }
+        if (print_table_stats(cfg, ctx)) return 1;
     }
     if (!cfg.hide_progress) std::cerr << "total: " << fmt_s(since(t_start)) << "\n";
     return 0;
