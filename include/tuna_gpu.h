@@ -117,6 +117,15 @@ int tg_set_debug_stats(tg_ctx* ctx, int on);
  * Null arrays are skipped. */
 int64_t tg_debug_partitions(tg_ctx* ctx, uint64_t* inserted, uint64_t* lds_unique, uint64_t* spilled,
                             uint64_t* table_cap, uint64_t n);
+/* Minimizer coverage histogram (src/count.hpp:84-171, 665-705 under -dbg): for
+ * every distinct minimizer (canonical ntHash of the m-mer at a superkmer's
+ * minimizer position) the number of k-mers of its superkmers; returns the
+ * number of distinct coverage values, with coverage[i] ascending and
+ * n_minimizers[i] minimizers having it (TG_ECAP when cap is smaller).  Runs
+ * Phase 1 again and groups on the host. */
+int64_t tg_debug_min_coverage(tg_ctx* ctx, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                              const uint64_t* off, uint64_t n_chunks, uint32_t* coverage,
+                              uint64_t* n_minimizers, uint64_t cap);
 /* Result of the last job: n_out (key, count) pairs in unspecified order. */
 uint64_t tg_result_size(tg_ctx* ctx);
 /* keys: n*ceil(k/32) words; counts: n.  Copies min(n, cap) pairs. */

@@ -237,8 +237,13 @@ def test_cli_debug_table_stats(tmp_path):
     assert read_tsv(out) == oracle_tsv([text], k)
     err = r.stderr
     for line in ("[debug] per-partition table stats (first 20):", "[debug] aggregate table stats (16 partitions):",
-                 "dbg_load_mean: ", "dbg_unique_mean: ", "dbg_n_resizes: 0", "[debug] resize summary:"):
+                 "dbg_load_mean: ", "dbg_unique_mean: ", "dbg_n_resizes: 0", "[debug] resize summary:",
+                 "[debug] minimizer coverage (k-mers sharing one minimizer):"):
         assert line in err, line
+    cov_rows = open(work / "debug_min_coverage.csv").read().splitlines()
+    assert cov_rows[0] == "coverage,n_minimizers,total_kmers"
+    assert sum(int(r.split(",")[2]) for r in cov_rows[1:]) == int(
+        next(l for l in err.splitlines() if l.startswith("  total k-mers covered")).split(":")[1])
     rows = [l.split(",") for l in open(work / "debug_table_stats.csv").read().splitlines()]
     assert rows[0] == ["partition_id", "init_sz", "table_cap", "n_inserted", "n_unique", "load_factor",
                        "n_resizes", "resize_s", "n_overflow"]

@@ -228,3 +228,26 @@ def test_debug_partition_stats(k, m, n):
             e.debug_partitions(n // 2)
     finally:
         e.close()
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (63, 21), (21, 15), (17, 2)])
+def test_debug_min_coverage_matches_oracle(eng, k, m):
+    # k-mers per distinct minimizer (the reference's -dbg coverage histogram),
+    # grouped here by the oracle's minimizer hash of each superkmer
+    text = T.synth("kind=reads_fq genome=50000 seed=8 cov=10 rl=150 sub=0.005 nrate=0.001")
+    b, o = T.parse_buffer(text)
+    b2 = b"".join([b">lc\n", b"A" * 2000, b"CA" * 800, b"\n"])
+    lb, lo = T.parse_buffer(b2)
+    for bases, off in ((b, o), (lb, lo)):
+        say(f"min coverage k={k} m={m} bases={len(bases)}")
+        cov, cnt = eng.debug_min_coverage(bases, off, k, m)
+        st, ln, mp, hs, km = O.superkmers(bases, off, k, m)
+        per = {}
+        for h, l in zip(hs.tolist(), ln.tolist()):
+            per[h] = per.get(h, 0) + l - k + 1
+        hist = {}
+        for v in per.values():
+            hist[v] = hist.get(v, 0) + 1
+        exp = sorted(hist.items())
+        assert list(zip(cov.tolist(), cnt.tolist())) == exp
+        assert int((cov.astype(np.int64) * cnt.astype(np.int64)).sum()) == km

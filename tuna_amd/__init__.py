@@ -35,7 +35,7 @@ EXPORTS = [
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
-    "tg_set_debug_stats", "tg_debug_partitions",
+    "tg_set_debug_stats", "tg_debug_partitions", "tg_debug_min_coverage",
     "tg_result_tsv", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
     "tg_unmap_file",
 ]
@@ -201,6 +201,9 @@ def lib():
     L.tg_set_debug_stats.argtypes = [_vp, ctypes.c_int]
     L.tg_debug_partitions.argtypes = [_vp, _u64p, _u64p, _u64p, _u64p, ctypes.c_uint64]
     L.tg_debug_partitions.restype = ctypes.c_int64
+    L.tg_debug_min_coverage.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
+                                        ctypes.c_uint64, _u32p, _u64p, ctypes.c_uint64]
+    L.tg_debug_min_coverage.restype = ctypes.c_int64
     L.tg_autotune_many.argtypes = [_vp, ctypes.POINTER(TgParams), _vp, ctypes.c_uint64, _vp,
                                    ctypes.c_uint64, _u64p, _u64p, ctypes.c_uint32,
                                    ctypes.POINTER(TgBatchConfig), ctypes.POINTER(ctypes.c_double)]
@@ -468,6 +471,19 @@ class Engine:
         got = lib().tg_debug_partitions(self._h, *[x.ctypes.data_as(_u64p) for x in a], n)
         _check_size(got)
         return dict(zip(("inserted", "lds_unique", "spilled", "table_cap"), [x[:got] for x in a]))
+
+    def debug_min_coverage(self, bases, off, k, m, partitions=16, cap=1 << 16):
+        """Minimizer coverage histogram: (coverage values ascending, number of
+        minimizers with each)."""
+        bases = _u8(bases)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        p = _params(k, m, partitions)
+        cov = np.zeros(cap, np.uint32)
+        cnt = np.zeros(cap, np.uint64)
+        got = lib().tg_debug_min_coverage(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
+                                          _p(off, _u64p), len(off) - 1, _p(cov, _u32p), _p(cnt, _u64p), cap)
+        _check_size(got)
+        return cov[:got], cnt[:got]
 
     def reset_kernel_times(self):
         lib().tg_reset_kernel_times(self._h)

@@ -5,8 +5,10 @@
 // src/pipeline.hpp:111-178 (partition_kmers_mem + count_and_write_mem).
 // Device buffers live in the context and are reused across jobs.
 #include "engine_impl.h"
+#include "tg_device.h"
 
 #include <chrono>
+#include <unordered_map>
 
 namespace tgi {
 thread_local std::string g_err;
@@ -736,6 +738,55 @@ int tg_superkmers_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     return o.n_sk > cap ? fail(TG_ECAP, "superkmer buffer too small (%llu needed)",
                                (unsigned long long)o.n_sk)
                         : TG_OK;
+}
+
+int64_t tg_debug_min_coverage(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
+                              const uint64_t* off, uint64_t n_chunks, uint32_t* coverage,
+                              uint64_t* n_minimizers, uint64_t cap)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    uint8_t* d_b;
+    uint64_t* d_o;
+    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    Phase1 o;
+    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
+    collect_prof(c);
+    std::vector<uint2> recs(o.n_sk);
+    if (o.n_sk) HIPOK(hipMemcpy(recs.data(), o.sorted, o.n_sk * sizeof(uint2), hipMemcpyDeviceToHost));
+    // k-mers per minimizer, keyed by the canonical ntHash of the m-mer at the
+    // superkmer's minimizer position, as count_partition does under -dbg
+    // (src/count.hpp:99-103,141-144 with Kmer_Window::init_packed_with_min,
+    // include/kache-hash/Streaming_Kmer_Hash_Table.hpp:585-598)
+    const uint32_t m = p->m;
+    std::unordered_map<uint64_t, uint32_t> per_min;
+    per_min.reserve(recs.size() / 2 + 16);
+    for (const uint2& r : recs) {
+        const uint32_t len = r.y & tg::REC_LEN_MASK, mp = (r.y >> tg::REC_MINPOS_SHIFT) & 63u;
+        uint64_t f = 0, rv = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t ch = bases[(uint64_t)r.x + mp + i];
+            const uint32_t kc = ((ch >> 2) ^ (ch >> 1)) & 3u, nt = kc ^ (kc >> 1);
+            f ^= tg::rotl64(tg::nt_seed(nt), m - 1u - i);
+            rv ^= tg::rotl64(tg::nt_seed(nt ^ 2u), i);
+        }
+        per_min[f ^ rv] += len - p->k + 1u;
+    }
+    std::map<uint32_t, uint64_t> hist;
+    for (const auto& e : per_min) ++hist[e.second];
+    uint64_t i = 0;
+    for (const auto& e : hist) {
+        if (i < cap) {
+            if (coverage) coverage[i] = e.first;
+            if (n_minimizers) n_minimizers[i] = e.second;
+        }
+        ++i;
+    }
+    if (i > cap) return fail(TG_ECAP, "room for %llu coverage values, %llu needed", (unsigned long long)cap,
+                             (unsigned long long)i);
+    return (int64_t)i;
 }
 
 int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,

@@ -14,7 +14,7 @@ constexpr uint64_t EMPTY64 = ~0ull;
 
 __device__ __forceinline__ uint32_t kache_code(uint32_t c) { return ((c >> 2) ^ (c >> 1)) & 3u; }
 
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t s)
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t s)
 {
     s &= 63u;
     return s ? (x << s) | (x >> (64u - s)) : x;
@@ -24,7 +24,7 @@ __device__ __forceinline__ uint64_t rotr1(uint64_t x) { return (x >> 1) | (x << 
 
 // Published ntHash seeds for nt codes A, C, T, G (Mohamadi et al. 2016),
 // the values of include/nt_hash.hpp:26-31.  The reverse strand uses seed[b^2].
-__device__ __forceinline__ uint64_t nt_seed(uint32_t b)
+__host__ __device__ __forceinline__ uint64_t nt_seed(uint32_t b)
 {
     const uint64_t s0 = 0x3c8bfbb395c60474ull, s1 = 0x3193c18562a02b4cull;
     const uint64_t s2 = 0x295549f54be24456ull, s3 = 0x20323ed082572324ull;

@@ -354,6 +354,48 @@ int write_partition_files(const Config& cfg, tg_ctx* ctx, const tg_params& p, co
 // work items' LDS tables: fixed in size (init_sz = table_cap, no resizes).
 // n_unique counts the keys held in them; keys that went to the HBM spill
 // table (n_overflow, the last CSV column) are reported in total only.
+// Minimizer coverage (src/count.hpp:665-705): k-mers per distinct minimizer,
+// summarised and written as debug_min_coverage.csv.
+int print_min_coverage(const Config& cfg, tg_ctx* ctx, const tg_params& p, const uint8_t* bases,
+                       uint64_t n_bases, const std::vector<uint64_t>& off)
+{
+    // distinct coverage values of N k-mers are fewer than sqrt(2N) + 1 <= 2^20 here
+    std::vector<uint32_t> cov(1u << 20);
+    std::vector<uint64_t> cnt(1u << 20);
+    const int64_t got = tg_debug_min_coverage(ctx, &p, bases, n_bases, off.data(), off.size() - 1, cov.data(),
+                                              cnt.data(), cov.size());
+    if (got < 0) {
+        std::cerr << "tuna: error: " << tg_last_error() << "\n";
+        return 1;
+    }
+    if (got == 0) return 0;
+    cov.resize(got);
+    cnt.resize(got);
+    uint64_t minimizers = 0, covered = 0;
+    for (int64_t i = 0; i < got; ++i) {
+        minimizers += cnt[i];
+        covered += (uint64_t)cov[i] * cnt[i];
+    }
+    std::ostringstream o;
+    o << "\n[debug] minimizer coverage (k-mers sharing one minimizer):\n"
+      << "  unique minimizers tracked : " << minimizers << "\n"
+      << "  total k-mers covered      : " << covered << "\n"
+      << "  avg k-mers / minimizer    : " << std::fixed << std::setprecision(2)
+      << (minimizers ? (double)covered / (double)minimizers : 0.0) << "\n"
+      << "  max k-mers / minimizer    : " << (got ? cov[got - 1] : 0u) << "\n";
+    const std::string path = cfg.work_dir + "debug_min_coverage.csv";
+    std::ofstream csv(path);
+    if (csv) {
+        csv << "coverage,n_minimizers,total_kmers\n";
+        for (int64_t i = 0; i < got; ++i) csv << cov[i] << "," << cnt[i] << "," << (uint64_t)cov[i] * cnt[i] << "\n";
+        o << "[debug] minimizer coverage CSV written to: " << path << "\n";
+    } else {
+        o << "[debug] warning: could not write CSV to " << path << "\n";
+    }
+    std::cerr << o.str();
+    return 0;
+}
+
 int print_table_stats(const Config& cfg, tg_ctx* ctx)
 {
     const uint64_t n = cfg.num_partitions;
@@ -520,6 +562,7 @@ int run(Config& cfg)
             q += strlen(q) + 1;
         }
         if (print_table_stats(cfg, ctx)) return 1;
+        if (print_min_coverage(cfg, ctx, p, h.bases, h.n_bases, h.off)) return 1;
     }
     if (!cfg.hide_progress) std::cerr << "total: " << fmt_s(since(t_start)) << "\n";
     return 0;
