@@ -1749,57 +1749,39 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                         nend = s_incl[wv][sn];
                         nfl = s_seed[wv][sn][0];
                         nrl = s_seed[wv][sn][1];
-                        if (NW == 2) {
-                            nfh = s_seed[wv][sn][NW == 2 ? 2 : 0];
-                            nrh = s_seed[wv][sn][NW == 2 ? 3 : 1];
-                        }
+                        nfh = s_seed[wv][sn][2];
+                        nrh = s_seed[wv][sn][3];
                     }
                     bool adv = false, miss = false;
-                    if (NW == 1) {
-                        const uint64_t old = atomicCAS((unsigned long long*)&s_k0[slot],
-                                                       (unsigned long long)EMPTY64, (unsigned long long)xl);
-                        if (old == EMPTY64 || old == xl) {
+                    // the slot state, then its key words, read without a wait in
+                    // between: LDS serves a wave's accesses in order and a claim
+                    // writes the key before it publishes the count, so a published
+                    // count read first means the key read after it is complete
+                    const uint32_t c = __hip_atomic_load(&s_cnt[slot], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __asm__ volatile("" ::: "memory");
+                    const uint64_t sk0 = s_k0[slot], sk1 = s_k1[slot];
+                    if (c == 0u) {
+                        // claim; a lost race retries the same slot next time
+                        if (atomicCAS(&s_cnt[slot], 0u, LOCKED32) == 0u) {
+                            s_k0[slot] = xh;
+                            s_k1[slot] = xl;
+                            __hip_atomic_store(&s_cnt[slot], 1u, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                            adv = true;
+                        }
+                    } else if (c != LOCKED32) {
+                        if (sk0 == xh && sk1 == xl) {
                             atomicAdd(&s_cnt[slot], 1u);
                             adv = true;
                         } else {
                             miss = true;
                         }
-                    } else {
-                        // the slot state, then its key words, read without a wait in
-                        // between: LDS serves a wave's accesses in order and a claim
-                        // writes the key before it publishes the count, so a published
-                        // count read first means the key read after it is complete
-                        const uint32_t c = __hip_atomic_load(&s_cnt[slot], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __asm__ volatile("" ::: "memory");
-                        const uint64_t sk0 = s_k0[slot], sk1 = s_k1[slot];
-                        if (c == 0u) {
-                            // claim; a lost race retries the same slot next time
-                            if (atomicCAS(&s_cnt[slot], 0u, LOCKED32) == 0u) {
-                                s_k0[slot] = xh;
-                                s_k1[slot] = xl;
-                                __hip_atomic_store(&s_cnt[slot], 1u, __ATOMIC_RELEASE,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-                                adv = true;
-                            }
-                        } else if (c != LOCKED32) {
-                            if (sk0 == xh && sk1 == xl) {
-                                atomicAdd(&s_cnt[slot], 1u);
-                                adv = true;
-                            } else {
-                                miss = true;
-                            }
-                        }
                     }
                     if (miss) {
                         if (++p == a.probe_max) {
-                            if (NW == 1) {
-                                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, xl, 0ull}});
-                                else spill_insert1(spill_ref(a), xl);
-                            } else {
-                                if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
-                                else spill_insert2(spill_ref(a), xh, xl);
-                            }
+                            if (BATCH) spill_insert_w<3>(spill_ref(a), SpKey{{xh, xl, (uint64_t)tag}});
+                            else spill_insert2(spill_ref(a), xh, xl);
                             ++spilled;
                             adv = true;
                         } else {
@@ -1813,10 +1795,8 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
                             end = nend;
                             fl = nfl;
                             rl = nrl;
-                            if (NW == 2) {
-                                fh = nfh;
-                                rh = nrh;
-                            }
+                            fh = nfh;
+                            rh = nrh;
                             pc = 2u * (s * (16u * PW) + k) + 1u;
                         } else {
                             const uint32_t b = __builtin_amdgcn_ubfe(wn, ~pc, 2u);
