@@ -1576,9 +1576,15 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         if (NW == 1) s_k0[i] = EMPTY64;
         s_cnt[i] = 0;
     }
+    // An item's first record batch is wave wv's records [x + 64 wv, +64) of
+    // the item, loaded before the item starts (for later items during the
+    // previous item's drain); the dispenser hands out the rest from x + 64 NWAVE.
+    uint32_t rb = it.x + 64u * wv;
+    Rec<RW> nxt;
+    if (item < n_items && rb + lane < (stop ? it.x : it.y)) nxt = load_rec<RW>(a.rec, rb + lane);
     while (item < n_items) {
         if (threadIdx.x == 0) {
-            s_next = it.x;
+            s_next = it.x + 64u * NWAVE;
             s_tot_u = s_tot_s = 0;
         }
         __syncthreads();
@@ -1589,11 +1595,6 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
         uint32_t qn = 0;  // deferred keys on this wave's stack (k <= 32)
         // records are loaded one batch ahead: the next 64 are in flight while
         // the current ones are counted
-        uint32_t rb = 0;
-        if (lane == 0) rb = atomicAdd(&s_next, 64u);
-        rb = __shfl(rb, 0, 64);
-        Rec<RW> nxt;
-        if (rb + lane < r1) nxt = load_rec<RW>(a.rec, rb + lane);
         for (;;) {
             if (rb >= r1) break;
             const uint32_t i = rb + lane;
@@ -1856,6 +1857,13 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
             tot_out += n;
         }
         const uint32_t tot_uniq = s_tot_u, tot_sp = s_tot_s;
+        if (BATCH) {
+            // the next item (fetched during this one, visible since the barrier
+            // above): this wave's first batch, in flight during the writes
+            const uint4 nit = s_nit;
+            rb = nit.x + 64u * wv;
+            if (s_nitem < n_items && rb + lane < (s_nstop ? nit.x : nit.y)) nxt = load_rec<RW>(a.rec, rb + lane);
+        }
         if (threadIdx.x == 0) {
             s_base = tot_out ? atomicAdd(&a.counters[0], (unsigned long long)tot_out) : 0ull;
             if (tot_uniq) atomicAdd(&a.counters[1], (unsigned long long)tot_uniq);
