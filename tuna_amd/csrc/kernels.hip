@@ -1440,7 +1440,7 @@ __device__ __forceinline__ uint32_t slot_count(const uint64_t* k0, const uint32_
 // stay in step; the stack is then worked off 64 entries per pass, one probe
 // per entry, the misses pushed back one step further along their sequence.
 constexpr uint32_t DEFER_CAP = 128;  // entries per wave (>= 64 + 64)
-constexpr uint32_t DEFER_PSH = 12;   // entry state: slot | probes << 12 (CAP <= 4096)
+constexpr uint32_t DEFER_PSH = 13;   // entry state: slot | probes << 13 (CAP <= 8192)
 
 template <uint32_t CAP, bool BATCH>
 __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, uint64_t* qk,
@@ -1521,9 +1521,9 @@ constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8
 // the device-side item count is reached, spill keys carry the item's job, and
 // every item records its output segment for the per-job gather.
 template <int NW, uint32_t CAP, bool BATCH>
-__global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
+__global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
 {
-    constexpr uint32_t NWAVE = COUNT_BLOCK / 64;
+    constexpr uint32_t NWAVE = count_block(NW) / 64;
     __shared__ __align__(16) uint64_t s_k0[CAP];
     __shared__ uint64_t s_k1[NW == 2 ? CAP : 1];
     __shared__ uint32_t s_cnt[CAP];
@@ -1899,8 +1899,8 @@ __global__ __launch_bounds__(COUNT_BLOCK) void k_count(CountArgs a)
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw)
 {
     if (a.n_items == 0) return;
-    if (nw == 1) k_count<1, LDS_CAP_NW1, false><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
-    else k_count<2, LDS_CAP_NW2, false><<<a.n_items, COUNT_BLOCK, 0, st>>>(a);
+    if (nw == 1) k_count<1, LDS_CAP_NW1, false><<<a.n_items, count_block(1), 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, false><<<a.n_items, count_block(2), 0, st>>>(a);
 }
 
 // Resident workgroups of the persistent count launch: occupancy x CUs.
@@ -1911,10 +1911,10 @@ uint32_t count_persistent_blocks(uint32_t nw)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (nw == 1)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, (const void*)k_count<1, LDS_CAP_NW1, true>, COUNT_BLOCK, 0);
+            &per, (const void*)k_count<1, LDS_CAP_NW1, true>, count_block(1), 0);
     else
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, (const void*)k_count<2, LDS_CAP_NW2, true>, COUNT_BLOCK, 0);
+            &per, (const void*)k_count<2, LDS_CAP_NW2, true>, count_block(2), 0);
     if (cus <= 0) cus = 256;
     if (per <= 0) per = 1;
     return (uint32_t)(cus * per);
@@ -1922,8 +1922,8 @@ uint32_t count_persistent_blocks(uint32_t nw)
 
 void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks)
 {
-    if (nw == 1) k_count<1, LDS_CAP_NW1, true><<<blocks, COUNT_BLOCK, 0, st>>>(a);
-    else k_count<2, LDS_CAP_NW2, true><<<blocks, COUNT_BLOCK, 0, st>>>(a);
+    if (nw == 1) k_count<1, LDS_CAP_NW1, true><<<blocks, count_block(1), 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, true><<<blocks, count_block(2), 0, st>>>(a);
 }
 
 template <int NW>

@@ -198,9 +198,11 @@ void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_
                         unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt);
 
 // table geometry of the LDS counting kernels
-constexpr uint32_t LDS_CAP_NW1 = 4096;  // 4096 x (8+4) B = 48 KiB
+constexpr uint32_t LDS_CAP_NW1 = 8192;  // 8192 x (8+4) B = 96 KiB
 constexpr uint32_t LDS_CAP_NW2 = 2048;  // 2048 x (16+4) B = 40 KiB
-constexpr uint32_t COUNT_BLOCK = 512;
+// count workgroup: k <= 32, 16 waves around a 96 KiB table (one per CU);
+// k > 32, 8 waves around a 40 KiB table (two per CU)
+constexpr uint32_t count_block(int nw) { return nw == 1 ? 1024u : 512u; }
 constexpr uint32_t ROUTE_MAX_TILES = 512;
 constexpr uint32_t FIX_BLOCKS = 8;      // fix-up capacity of a batch: 8 x SK_BLOCK strips
 
