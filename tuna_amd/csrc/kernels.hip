@@ -748,7 +748,10 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
 // [tile][sub-partition] matrix.
 constexpr uint32_t ROUTE_T = 1024;
 
-constexpr uint32_t ROUTE_RUNS = 1024;  // run descriptors cached per tile (LDS: 16 KiB)
+// Run descriptors cached per tile (LDS: 16 KB).  With 8192 sub-partitions the
+// histogram block (64 KiB of counters + this cache) must stay within half of
+// the CU's 160 KiB, so that two blocks share a CU.
+constexpr uint32_t ROUTE_RUNS = 1008;
 
 __device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t& total);
 
