@@ -785,6 +785,48 @@ __device__ __forceinline__ RouteGeom route_geom(const RouteArgs& a)
     return g;
 }
 
+// First r in [l, h) with run_off[r + d] >= x, or h when there is none
+// (run_off does not decrease).  Wave 0 searches 64-ary: every round probes 64
+// evenly spaced indices at once, so a few thousand runs take three rounds of
+// loads instead of a dozen dependent ones.
+__device__ __forceinline__ uint32_t wave_run_search(const uint64_t* run_off, uint32_t d, uint64_t x,
+                                                    uint32_t l, uint32_t h)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    while (l < h) {
+        const uint64_t span = h - l;
+        const uint32_t p = l + (uint32_t)(((uint64_t)(lane + 1u) * span) / 65u);
+        const uint64_t hit = __ballot(p < h && run_off[p + d] >= x);
+        if (!hit) {
+            l = __builtin_amdgcn_readlane(p, 63) + 1u;
+        } else {
+            const uint32_t j = (uint32_t)__builtin_ctzll(hit);
+            h = __builtin_amdgcn_readlane(p, j);
+            if (j) l = __builtin_amdgcn_readlane(p, j - 1u) + 1u;
+        }
+    }
+    return l;
+}
+
+// The runs of the tile [lo, hi): s_meta[0] = the first run that ends after lo,
+// s_meta[1] = the runs cached from there (at most cap - 1), s_meta[2] = 1 when
+// the tile has more runs than that.  Ends with a barrier.
+__device__ __forceinline__ void tile_runs(const RouteArgs& a, uint64_t lo, uint64_t hi, uint32_t cap,
+                                          uint32_t* s_meta)
+{
+    if (threadIdx.x < 64) {
+        const uint32_t l = wave_run_search(a.run_off, 1u, lo + 1u, 0u, a.n_runs);
+        const uint32_t ef = wave_run_search(a.run_off, 0u, hi, l, a.n_runs);
+        const uint32_t e = min(ef, l + cap - 1u);
+        if (threadIdx.x == 0) {
+            s_meta[0] = l;
+            s_meta[1] = e - l;
+            s_meta[2] = ef > e ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+}
+
 // Calls f(record) for every record of this block's tile.  The tile's run
 // descriptors (record index of the run's first record minus its offset in the
 // run space) are cached in LDS; each wave walks 64*U consecutive records at a
@@ -796,20 +838,7 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
 {
     const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
     const uint64_t hi = min(lo + g.tile, g.n_rec);
-    if (threadIdx.x == 0) {
-        uint32_t l = 0, h = a.n_runs;
-        while (l < h) {  // first run with run_off[r+1] > lo
-            const uint32_t mid = (l + h) >> 1;
-            if (a.run_off[mid + 1] > lo) h = mid;
-            else l = mid + 1;
-        }
-        uint32_t e = l;
-        while (e < a.n_runs && a.run_off[e] < hi && e - l < ROUTE_RUNS - 1) ++e;
-        s_meta[0] = l;
-        s_meta[1] = e - l;  // runs cached
-        s_meta[2] = (e < a.n_runs && a.run_off[e] < hi) ? 1u : 0u;  // more runs than cached
-    }
-    __syncthreads();
+    tile_runs(a, lo, hi, ROUTE_RUNS, s_meta);
     const uint32_t r0 = s_meta[0], nr = s_meta[1];
     for (uint32_t i = threadIdx.x; i <= nr; i += blockDim.x) {
         s_off[i] = a.run_off[r0 + i];
@@ -993,20 +1022,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     }
     const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
     const uint64_t hi = min(lo + g.tile, g.n_rec);
-    if (tid == 0) {
-        uint32_t l = 0, h = a.n_runs;
-        while (l < h) {  // first run with run_off[r+1] > lo
-            const uint32_t mid = (l + h) >> 1;
-            if (a.run_off[mid + 1] > lo) h = mid;
-            else l = mid + 1;
-        }
-        uint32_t e = l;
-        while (e < a.n_runs && a.run_off[e] < hi && e - l < COARSE_RUNS - 1) ++e;
-        s_meta[0] = l;
-        s_meta[1] = e - l;
-        s_meta[2] = (e < a.n_runs && a.run_off[e] < hi) ? 1u : 0u;
-    }
-    __syncthreads();
+    tile_runs(a, lo, hi, COARSE_RUNS, s_meta);
     const uint32_t r0 = s_meta[0], nr = s_meta[1];
     const bool cached = s_meta[2] == 0;
     for (uint32_t i = tid; i <= nr; i += blockDim.x) {
