@@ -74,6 +74,7 @@ struct BatchBufs {
     uint64_t* sp_keys;
     uint32_t* sp_cnt;
     unsigned long long* counters;
+    uint8_t* back;             // the readback, packed for one copy
     uint64_t *out_keys, *fin_keys;
     uint32_t *out_cnt, *fin_cnt;
     unsigned long long* job_acc;  // [0,J): distinct, [J,2J): spill reported, [2J,3J): spill cursor,
@@ -86,6 +87,10 @@ struct BatchBufs {
 // capacity, bit 1 partition-buffer capacity, bit 2 offsets), [2] spill
 // overflow, [3] work items, [4] item queue
 enum { F_FLAGS = 0, F_SKERR = 1, F_SPERR = 2, F_NITEMS = 3, F_ITEMQ = 4, F_WORDS = 16 };
+
+// Readback of a batch (device, packed by tg::launch_pack_back, and its pinned
+// host copy): job offsets, superkmers, k-mers, distinct, flags, counters.
+inline uint64_t back_bytes(uint32_t cap_jobs) { return 8ull * (4ull * cap_jobs + 1) + 4ull * F_WORDS + 64; }
 
 }  // namespace
 
@@ -171,13 +176,14 @@ int alloc_class(BatchState* s, const BatchClass& c)
             return fail(TG_ENOMEM, "pinned host allocation failed");
         s->h_tab_bytes = tab_bytes;
     }
-    const uint64_t back_bytes = 8ull * (4ull * c.cap_jobs + 1) + 4ull * F_WORDS + 64;
-    if (s->h_back_bytes < back_bytes) {
+    const uint64_t back_n = back_bytes(c.cap_jobs);
+    if ((rc = balloc(s, &b.back, back_n))) return rc;
+    if (s->h_back_bytes < back_n) {
         if (s->h_back) hipHostFree(s->h_back);
         s->h_back = nullptr;
-        if (hipHostMalloc((void**)&s->h_back, back_bytes, hipHostMallocDefault) != hipSuccess)
+        if (hipHostMalloc((void**)&s->h_back, back_n, hipHostMallocDefault) != hipSuccess)
             return fail(TG_ENOMEM, "pinned host allocation failed");
-        s->h_back_bytes = back_bytes;
+        s->h_back_bytes = back_n;
     }
     b.cap_words = c.cap_bases / 16 + 16;
     const uint64_t nch1 = c.cap_ch + 1, ns1 = c.cap_strips + 1;
@@ -382,6 +388,8 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
                                 b.fin_cnt, GATHER_BLOCKS);
         tg::launch_spill_place(st, ca, c.nw, c.sp_slots, b.job_off, b.job_item_tot,
                                b.job_acc + 2ull * J, b.fin_keys, b.fin_cnt);
+        tg::launch_pack_back(st, b.desc, J, b.job_off, b.job_nsk, b.job_nkm, b.job_acc, b.flags, F_WORDS,
+                             b.counters, b.back);
     }
 }
 
@@ -642,13 +650,8 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
             uint64_t* h_nkm = h_nsk + c.cap_jobs;
             uint64_t* h_unq = h_nkm + c.cap_jobs;
             uint32_t* h_flags = reinterpret_cast<uint32_t*>(h_unq + c.cap_jobs);
-            HIPOK(hipMemcpyAsync(h_off, s->b.job_off, (nj + 1) * 8ull, hipMemcpyDeviceToHost, st));
-            HIPOK(hipMemcpyAsync(h_nsk, s->b.job_nsk, nj * 8ull, hipMemcpyDeviceToHost, st));
-            HIPOK(hipMemcpyAsync(h_nkm, s->b.job_nkm, nj * 8ull, hipMemcpyDeviceToHost, st));
-            HIPOK(hipMemcpyAsync(h_unq, s->b.job_acc, nj * 8ull, hipMemcpyDeviceToHost, st));
-            HIPOK(hipMemcpyAsync(h_flags, s->b.flags, 4ull * F_WORDS, hipMemcpyDeviceToHost, st));
             unsigned long long* h_ctr = reinterpret_cast<unsigned long long*>(h_flags + F_WORDS);
-            HIPOK(hipMemcpyAsync(h_ctr, s->b.counters, 8ull * 8, hipMemcpyDeviceToHost, st));
+            HIPOK(hipMemcpyAsync(s->h_back, s->b.back, back_bytes(c.cap_jobs), hipMemcpyDeviceToHost, st));
             HIPOK(hipStreamSynchronize(st));
             {
                 float a1 = 0, a2 = 0;

@@ -2210,6 +2210,43 @@ void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJ
         k_gather_items<2><<<blocks, 256, 0, st>>>(a, j, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
 }
 
+__global__ __launch_bounds__(256) void k_pack_back(const BatchDesc* __restrict__ d, uint32_t cap_jobs,
+                                                   const uint64_t* __restrict__ job_off,
+                                                   const uint64_t* __restrict__ job_nsk,
+                                                   const uint64_t* __restrict__ job_nkm,
+                                                   const unsigned long long* __restrict__ job_uniq,
+                                                   const uint32_t* __restrict__ flags, uint32_t n_flags,
+                                                   const unsigned long long* __restrict__ counters,
+                                                   uint8_t* __restrict__ back)
+{
+    const uint32_t nj = min(d->n_jobs, cap_jobs);
+    uint64_t* o = reinterpret_cast<uint64_t*>(back);
+    uint64_t* nsk = o + cap_jobs + 1;
+    uint64_t* nkm = nsk + cap_jobs;
+    uint64_t* unq = nkm + cap_jobs;
+    uint32_t* fl = reinterpret_cast<uint32_t*>(unq + cap_jobs);
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(fl + n_flags);
+    for (uint32_t i = threadIdx.x; i <= nj; i += blockDim.x) {
+        o[i] = job_off[i];
+        if (i < nj) {
+            nsk[i] = job_nsk[i];
+            nkm[i] = job_nkm[i];
+            unq[i] = job_uniq[i];
+        }
+    }
+    if (threadIdx.x < n_flags) fl[threadIdx.x] = flags[threadIdx.x];
+    if (threadIdx.x < 8) ctr[threadIdx.x] = counters[threadIdx.x];
+}
+
+void launch_pack_back(hipStream_t st, const BatchDesc* d, uint32_t cap_jobs, const uint64_t* job_off,
+                      const uint64_t* job_nsk, const uint64_t* job_nkm,
+                      const unsigned long long* job_uniq, const uint32_t* flags, uint32_t n_flags,
+                      const unsigned long long* counters, uint8_t* back)
+{
+    k_pack_back<<<1, 256, 0, st>>>(d, cap_jobs, job_off, job_nsk, job_nkm, job_uniq, flags, n_flags,
+                                   counters, back);
+}
+
 template <int SW>
 __global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slots,
                                                      const uint64_t* __restrict__ job_off,

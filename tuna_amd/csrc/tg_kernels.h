@@ -184,6 +184,13 @@ void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_
 // spill entries of a batch: per-job distinct and reported counts, then placement
 void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
                        unsigned long long* job_sp);
+// The batch readback in one buffer, laid out as [job offsets: cap_jobs + 1]
+// [superkmers: cap_jobs][k-mers: cap_jobs][distinct: cap_jobs] (u64), then
+// n_flags u32 flags and 8 u64 counters, so the host needs one copy.
+void launch_pack_back(hipStream_t st, const BatchDesc* d, uint32_t cap_jobs, const uint64_t* job_off,
+                      const uint64_t* job_nsk, const uint64_t* job_nkm,
+                      const unsigned long long* job_uniq, const uint32_t* flags, uint32_t n_flags,
+                      const unsigned long long* counters, uint8_t* back);
 void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t cap_jobs,
                        const uint64_t* bucket_item, const uint64_t* seg_x,
                        const unsigned long long* job_sk, const unsigned long long* job_sp,
