@@ -279,12 +279,12 @@ __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uin
         L.nxt = L.packed[++L.widx];  // the stream keeps 8 spare words
     }
     L.u = u + 1u;
+    // kache codes in and out: the tables are indexed by them
     const uint32_t kb = (uint32_t)(L.buf >> 62);
     L.buf <<= 2;
-    const uint32_t in = kb ^ (kb >> 1);  // kache -> nt code
     const uint32_t o = (uint32_t)(L.lmer >> L.osh) & 3u;
-    L.lmer = (L.lmer << 2) | in;
-    const uint32_t ix = (o << 2) | in;
+    L.lmer = (L.lmer << 2) | kb;
+    const uint32_t ix = (o << 2) | kb;
     L.f = rotl1(L.f) ^ tf[ix];
     L.rv = rotr1(L.rv) ^ tr[ix];
 }
@@ -555,7 +555,9 @@ __device__ __forceinline__ void sk_tables(const SkArgs& a, uint64_t* tf, uint64_
 {
     const uint32_t m = a.m;
     if (threadIdx.x < 16) {
-        const uint32_t o = threadIdx.x >> 2, i = threadIdx.x & 3u;
+        // entry (out, in) in kache codes; the seeds take nt codes (c ^ c >> 1)
+        const uint32_t ko = threadIdx.x >> 2, ki = threadIdx.x & 3u;
+        const uint32_t o = ko ^ (ko >> 1), i = ki ^ (ki >> 1);
         // include/nt_hash.hpp:80-93
         tf[threadIdx.x] = rotl64(nt_seed(o), m) ^ nt_seed(i);
         tr[threadIdx.x] = rotl64(nt_seed(o ^ 2u), 63u) ^ rotl64(nt_seed(i ^ 2u), m - 1u);

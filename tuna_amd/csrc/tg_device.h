@@ -19,8 +19,18 @@ __host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t s)
     s &= 63u;
     return s ? (x << s) | (x >> (64u - s)) : x;
 }
-__device__ __forceinline__ uint64_t rotl1(uint64_t x) { return (x << 1) | (x >> 63); }
-__device__ __forceinline__ uint64_t rotr1(uint64_t x) { return (x >> 1) | (x << 63); }
+// One-bit rotations as two funnel shifts of the 32-bit halves (alignbit(a, b, s)
+// is the low word of the 64-bit a:b shifted right by s).
+__device__ __forceinline__ uint64_t rotl1(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return ((uint64_t)__builtin_amdgcn_alignbit(hi, lo, 31u) << 32) | __builtin_amdgcn_alignbit(lo, hi, 31u);
+}
+__device__ __forceinline__ uint64_t rotr1(uint64_t x)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return ((uint64_t)__builtin_amdgcn_alignbit(lo, hi, 1u) << 32) | __builtin_amdgcn_alignbit(hi, lo, 1u);
+}
 
 // Published ntHash seeds for nt codes A, C, T, G (Mohamadi et al. 2016),
 // the values of include/nt_hash.hpp:26-31.  The reverse strand uses seed[b^2].
