@@ -270,7 +270,9 @@ struct SkLane {
     uint64_t cb;
 };
 
-__device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uint64_t* tr)
+// tab: the 16 forward entries, then the 16 reverse ones, indexed by
+// (outgoing << 2 | incoming) in kache codes
+__device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tab)
 {
     // the lanes that roll agree on u
     const uint32_t u = __builtin_amdgcn_readfirstlane(L.u);
@@ -279,14 +281,17 @@ __device__ __forceinline__ void sk_roll(SkLane& L, const uint64_t* tf, const uin
         L.nxt = L.packed[++L.widx];  // the stream keeps 8 spare words
     }
     L.u = u + 1u;
-    // kache codes in and out: the tables are indexed by them
-    const uint32_t kb = (uint32_t)(L.buf >> 62);
+    // byte offset of entry (o, in) = 8 (o << 2 | in), in one funnel shift:
+    // the outgoing base's code in the low bits of the m-mer shifted down,
+    // the incoming one in the stream's top bits
+    const uint32_t hi = (uint32_t)(L.buf >> 32);
+    const uint32_t kb = hi >> 30;
+    const uint32_t off = __builtin_amdgcn_alignbit((uint32_t)(L.lmer >> L.osh), hi, 27u) & 0x78u;
     L.buf <<= 2;
-    const uint32_t o = (uint32_t)(L.lmer >> L.osh) & 3u;
     L.lmer = (L.lmer << 2) | kb;
-    const uint32_t ix = (o << 2) | kb;
-    L.f = rotl1(L.f) ^ tf[ix];
-    L.rv = rotr1(L.rv) ^ tr[ix];
+    const uint64_t* e = reinterpret_cast<const uint64_t*>(reinterpret_cast<const char*>(tab) + off);
+    L.f = rotl1(L.f) ^ e[0];
+    L.rv = rotr1(L.rv) ^ e[16];
 }
 
 // Routing field of a record (18 bits of the minimizer hash H): the bucket
@@ -602,10 +607,20 @@ __device__ __forceinline__ void sk_backward(uint64_t (&sv)[WMAX], uint8_t (&sp)[
 template <int WMAX, int RW>
 __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <= 12 ? 5 : 1))) void k_superkmers(SkArgs a)
 {
-    __shared__ uint64_t tf[16], tr[16], s_init[2];
-    __shared__ uint2 s_buf[SK_BLOCK / 64][SK_WAVE_BUF + 64];
-    __shared__ uint32_t s_wgn;
-    sk_tables(a, tf, tr, s_init);
+    // one LDS object, so the hash tables sit at address 0 and a table read
+    // needs no base added to its offset
+    struct SkShared {
+        uint64_t tab[32];  // forward [0, 16), reverse [16, 32)
+        uint64_t init[2];
+        uint2 buf[SK_BLOCK / 64][SK_WAVE_BUF + 64];
+        uint32_t wgn;
+    };
+    __shared__ SkShared sh;
+    const uint64_t* tab = sh.tab;
+    const uint64_t* s_init = sh.init;
+    auto& s_buf = sh.buf;
+    uint32_t& s_wgn = sh.wgn;
+    sk_tables(a, sh.tab, sh.tab + 16, sh.init);
     const uint32_t wv = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_wgn = 0;
     __syncthreads();
@@ -638,13 +653,13 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
         SkLane L;
         sk_init(L, a, s, s_init);
         const uint32_t W = a.W;
-        for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, tf, tr);
+        for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, tab);
         uint64_t sv[WMAX];
         uint8_t sp[WMAX];
 #pragma unroll
         for (int r = 0; r < WMAX; ++r) {
             if ((uint32_t)r < W) {
-                sk_roll(L, tf, tr);
+                sk_roll(L, tab);
                 sv[r] = L.f ^ L.rv;
             }
         }
@@ -671,7 +686,7 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
                     }
                     sk_kmer<RW, (WMAX > 16)>(L, a, S, t, Hc, Pc);
                     if (!L.done) {
-                        sk_roll(L, tf, tr);
+                        sk_roll(L, tab);
                         const uint64_t h = L.f ^ L.rv;
                         sv[r] = h;
                         if (r == 0 || h < pv) {
