@@ -167,9 +167,19 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint64_t* __restri
     __shared__ uint64_t sh[SCAN_T / 64];
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_I;
     uint64_t s = 0;
+    if (base + SCAN_I <= n && (reinterpret_cast<uintptr_t>(in) & 15u) == 0) {
+        // whole run of this thread: 16-byte loads
+        const ulonglong2* p = reinterpret_cast<const ulonglong2*>(in + base);
 #pragma unroll
-    for (uint32_t i = 0; i < SCAN_I; ++i)
-        if (base + i < n) s += in[base + i];
+        for (uint32_t i = 0; i < SCAN_I / 2; ++i) {
+            const ulonglong2 v = p[i];
+            s += v.x + v.y;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < SCAN_I; ++i)
+            if (base + i < n) s += in[base + i];
+    }
     uint64_t tot;
     block_excl_scan_u64(s, sh, tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -183,17 +193,43 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint64_t* __restric
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_I;
     uint64_t v[SCAN_I];
     uint64_t s = 0;
+    // whole run of this thread with both arrays 16-byte aligned: 16-byte
+    // loads and stores
+    const bool vec = base + SCAN_I <= n &&
+                     ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    if (vec) {
+        const ulonglong2* p = reinterpret_cast<const ulonglong2*>(in + base);
 #pragma unroll
-    for (uint32_t i = 0; i < SCAN_I; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0;
-        s += v[i];
+        for (uint32_t i = 0; i < SCAN_I / 2; ++i) {
+            const ulonglong2 w = p[i];
+            v[2 * i] = w.x;
+            v[2 * i + 1] = w.y;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < SCAN_I; ++i) v[i] = base + i < n ? in[base + i] : 0;
     }
+#pragma unroll
+    for (uint32_t i = 0; i < SCAN_I; ++i) s += v[i];
     uint64_t tot;
     uint64_t run = block_excl_scan_u64(s, sh, tot) + (boff ? boff[blockIdx.x] : 0);
+    if (vec) {
+        ulonglong2* q = reinterpret_cast<ulonglong2*>(out + base);
 #pragma unroll
-    for (uint32_t i = 0; i < SCAN_I; ++i) {
-        if (base + i < n) out[base + i] = run;
-        run += v[i];
+        for (uint32_t i = 0; i < SCAN_I / 2; ++i) {
+            ulonglong2 w;
+            w.x = run;
+            run += v[2 * i];
+            w.y = run;
+            run += v[2 * i + 1];
+            q[i] = w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < SCAN_I; ++i) {
+            if (base + i < n) out[base + i] = run;
+            run += v[i];
+        }
     }
     if (base < n && base + SCAN_I >= n) out[n] = run;  // thread owning the last item
 }
