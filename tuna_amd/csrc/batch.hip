@@ -239,11 +239,14 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
     {
         // clear state (kernels only: the sequence may be captured) and inputs
         Prof pr(prof, "batch_init");
-        tg::launch_fill_u32(st, b.flags, 0u, F_WORDS);
-        tg::launch_fill_u64(st, (uint64_t*)b.counters, 0ull, 8);
-        tg::launch_fill_u32(st, b.sp_cnt, 0u, c.sp_slots);
-        tg::launch_fill_u64(st, b.seg_n, 0ull, ITEM_CAP + 1);
-        tg::launch_fill_u64(st, (uint64_t*)b.job_acc, 0ull, 4ull * J);
+        tg::ZeroRegions z{};
+        z.p[0] = b.flags;      z.n[0] = F_WORDS;          z.wide[0] = 0;
+        z.p[1] = b.counters;   z.n[1] = 8;                z.wide[1] = 1;
+        z.p[2] = b.sp_cnt;     z.n[2] = c.sp_slots;       z.wide[2] = 0;
+        z.p[3] = b.seg_n;      z.n[3] = ITEM_CAP + 1;     z.wide[3] = 1;
+        z.p[4] = b.job_acc;    z.n[4] = 4ull * J;         z.wide[4] = 1;
+        z.count = 5;
+        tg::launch_zero_regions(st, z);
         tg::launch_batch_setup(st, b.desc, jobs, J, b.boff, b.chunk_job, c.cap_ch, b.bucket_job, c.nsp,
                                b.flags + F_SKERR);
     }

@@ -1244,6 +1244,31 @@ void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n)
     k_fill_u64<<<(unsigned)b, 256, 0, st>>>(p, v, n);
 }
 
+__global__ __launch_bounds__(256) void k_zero_regions(ZeroRegions z)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint32_t q = 0; q < z.count; ++q) {
+        if (z.wide[q]) {
+            uint64_t* p = static_cast<uint64_t*>(z.p[q]);
+            for (uint64_t i = t0; i < z.n[q]; i += stride) p[i] = 0ull;
+        } else {
+            uint32_t* p = static_cast<uint32_t*>(z.p[q]);
+            for (uint64_t i = t0; i < z.n[q]; i += stride) p[i] = 0u;
+        }
+    }
+}
+
+void launch_zero_regions(hipStream_t st, const ZeroRegions& z)
+{
+    uint64_t mx = 0;
+    for (uint32_t q = 0; q < z.count; ++q) mx = std::max(mx, z.n[q]);
+    if (!mx) return;
+    uint64_t b = (mx + 255) / 256;
+    if (b > 4096) b = 4096;
+    k_zero_regions<<<(unsigned)b, 256, 0, st>>>(z);
+}
+
 // ========================================================== counting ====
 constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 // never in a one-word table: slots hold EMPTY64 or a key below 2^64 - 2^62

@@ -164,6 +164,15 @@ void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
 void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t* tmp,
                           uint32_t rw);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
+// Zeroes up to ZERO_MAX regions in one launch (elements of 4 or 8 bytes).
+constexpr int ZERO_MAX = 6;
+struct ZeroRegions {
+    void* p[ZERO_MAX];
+    uint64_t n[ZERO_MAX];
+    uint32_t wide[ZERO_MAX];  // 1: 8-byte elements
+    uint32_t count;
+};
+void launch_zero_regions(hipStream_t st, const ZeroRegions& z);
 void launch_fill_u64(hipStream_t st, uint64_t* p, uint64_t v, uint64_t n);
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw);
 void launch_spill_drain(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots);
