@@ -242,3 +242,21 @@ def test_autotune_installs_a_config_with_identical_results(eng):
         eng.dev_free(d_b)
         eng.dev_free(d_o)
         eng.set_batch_config(**{k_: getattr(old, k_) for k_, _ in old._fields_})
+
+
+@pytest.mark.gpu
+def test_single_result_survives_many_run(eng):
+    # a single job's result stays where the batch gathered it; a many-job run
+    # that follows must move it first, so reading it afterwards gives the same
+    parts = job_inputs(7)
+    b, o = parts[1]
+    eng.count(b, o, 31, 21, 16, 1, fetch=False)
+    k1, c1 = eng.result(31)
+    k1, c1 = _sorted(np.array(k1).reshape(len(c1), -1), np.array(c1))
+    bases, off, jc = concat(parts)
+    eng.count_many(bases, off, jc, 63, 21)
+    k2, c2 = eng.result(31)
+    k2, c2 = _sorted(np.array(k2).reshape(len(c2), -1), np.array(c2))
+    say(f"  single job: {len(c1)} k-mers before and {len(c2)} after a many-job run")
+    assert len(c1) > 0
+    assert np.array_equal(k1, k2) and np.array_equal(c1, c2)

@@ -99,6 +99,10 @@ inline uint64_t back_bytes(uint32_t cap_jobs) { return 8ull * (4ull * cap_jobs +
 struct ResultStore {
     uint64_t* keys = nullptr;
     uint32_t* cnt = nullptr;
+    // a single job counted in one batch keeps its result where the batch
+    // gathered it (valid until the next job on the context): no copy
+    uint64_t* view_keys = nullptr;
+    uint32_t* view_cnt = nullptr;
     uint64_t cap = 0, n = 0;
     uint32_t nw = 1;
     std::vector<uint64_t> off, len;
@@ -584,6 +588,27 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     HIPOK(hipSetDevice(ctx->device));
     BatchState* s = state(ctx);
     hipStream_t st = ctx->stream;
+    if (!single && s->one.view_keys) {
+        // a many-job run may reallocate the batch buffers: the single job's
+        // result that still lives there moves into its own store first
+        ResultStore& o = s->one;
+        uint64_t* vk = o.view_keys;
+        uint32_t* vc = o.view_cnt;
+        o.view_keys = nullptr;
+        o.view_cnt = nullptr;
+        const uint64_t n = o.n;
+        o.n = 0;
+        if ((rc = grow_store(o, st, std::max<uint64_t>(n, 1), o.nw))) return rc;
+        if (n) {
+            HIPOK(hipMemcpyAsync(o.keys, vk, n * o.nw * 8ull, hipMemcpyDeviceToDevice, st));
+            HIPOK(hipMemcpyAsync(o.cnt, vc, n * 4ull, hipMemcpyDeviceToDevice, st));
+        }
+        o.n = n;
+        if (ctx->d_keys == vk + o.off[0] * o.nw) {
+            ctx->d_keys = o.keys + o.off[0] * o.nw;
+            ctx->d_cnt = o.cnt + o.off[0];
+        }
+    }
     const auto t_start = std::chrono::steady_clock::now();
     s->last = tg_many_stats{};
     s->ms_p1 = s->ms_p2 = 0;
@@ -599,6 +624,9 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     out.off.assign(jt.n, 0);
     out.len.assign(jt.n, 0);
     out.n = 0;
+    out.view_keys = nullptr;
+    out.view_cnt = nullptr;
+    const bool view = single && cuts.size() == 2;
     if ((rc = grow_store(out, st, 1, nw))) return rc;
     uint64_t tot_kmers = 0, tot_uniq = 0;
     for (size_t bi = 0; bi + 1 < cuts.size(); ++bi) {
@@ -689,8 +717,14 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
                 continue;
             }
             const uint64_t n_out = h_off[nj];
-            if ((rc = grow_store(out, st, out.n + n_out, nw))) return rc;
-            if (n_out) {
+            if (view) {
+                out.view_keys = s->b.fin_keys;
+                out.view_cnt = s->b.fin_cnt;
+                out.nw = nw;
+            } else if ((rc = grow_store(out, st, out.n + n_out, nw))) {
+                return rc;
+            }
+            if (n_out && !view) {
                 HIPOK(hipMemcpyAsync(out.keys + out.n * nw, s->b.fin_keys, n_out * nw * 8ull,
                                      hipMemcpyDeviceToDevice, st));
                 HIPOK(hipMemcpyAsync(out.cnt + out.n, s->b.fin_cnt, n_out * 4ull,
@@ -746,8 +780,10 @@ int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uin
     BatchState* s = c->batch;
     c->n_out = s->one.len[0];
     c->nw = s->one.nw;
-    c->d_keys = s->one.keys + s->one.off[0] * s->one.nw;
-    c->d_cnt = s->one.cnt + s->one.off[0];
+    uint64_t* rk = s->one.view_keys ? s->one.view_keys : s->one.keys;
+    uint32_t* rc_ = s->one.view_cnt ? s->one.view_cnt : s->one.cnt;
+    c->d_keys = rk + s->one.off[0] * s->one.nw;
+    c->d_cnt = rc_ + s->one.off[0];
     collect_prof(c);
     if (st) {
         tg_stats o{};
