@@ -2189,6 +2189,7 @@ template <int SW>
 __global__ __launch_bounds__(256) void k_spill_jobs(CountArgs a, uint64_t n_slots,
                                                     unsigned long long* __restrict__ job_sp)
 {
+    if (a.counters[2] == 0) return;  // no k-mer reached the spill table: it is empty
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n_slots; s += stride) {
         const uint32_t c = a.sp_cnt[s];
@@ -2333,6 +2334,7 @@ __global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slo
                                                      uint64_t* __restrict__ dst_keys,
                                                      uint32_t* __restrict__ dst_cnt)
 {
+    if (a.counters[2] == 0) return;  // no k-mer reached the spill table: it is empty
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n_slots; s += stride) {
         const uint32_t c = a.sp_cnt[s];
