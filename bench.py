@@ -1,20 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark: Gbases/s counted on one MI355X for the short-read FASTQ workload.
+"""Benchmark: Gbases/s counted on one MI355X.
 
-Workload (N=1): a 5 Mbp uniform random genome (seed 2, GC 50%), 150 bp reads
-at uniform positions and strands to 50x (1,666,667 reads, 250 Mbp), 0.5%
-i.i.d. substitutions, N at 0.01% per base, Phred 30 qualities; counted with
-k=31, m=21, ci=2, n=256 partitions.  The FASTQ text is generated and parsed on
-the host once; the ACGT chunks and offsets are uploaded to HBM before the
-timed region.  One step = one whole counting job on the device (pack,
-superkmers, routing, counting, drain), including the job's own host syncs.
+Headline workload (N=1): reads from a repetitive, low-complexity genome.  A
+100 Mbp genome (seed 3, GC 45%) of which 35% is interspersed copies of 50
+random 300 bp repeat families (5% divergence) and 10% tandem arrays (units of
+2-12 bp, arrays of 200-2000 bp); 150 bp reads at 20x (2 Gbp, 13.3 M reads)
+with 0.3% substitutions; counted with k=31, m=23, ci=1, n=4096 partitions.
+This is the largest single-GPU configuration of the benchmark set.  The FASTQ
+text is generated and parsed on the host once; the ACGT chunks and offsets are
+uploaded to HBM before the timed region.  One step = one whole counting job on
+the device (pack, superkmers, routing, counting, drain).
 
-Prints one JSON line (see the fields below).  --gpus N > 1 is not supported:
-this engine runs on one GPU.
+Beside it: the short-read workload (250 Mbp of 150 bp reads at 50x, k=31 m=21
+ci=2 n=256) measured the same way, the end-to-end CLI rate (FASTQ file to TSV
+file) of both, the many-sample batched leg, the serving leg and the CPU
+baseline.
+
+Prints one JSON line.  --gpus N > 1 is not supported: this engine runs on one
+GPU.
 """
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -27,10 +35,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gbases/s counted on 1 MI355X (% HBM roofline); batched jobs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = ("short-read FASTQ: 5 Mbp genome seed 2, 150 bp reads at 50x "
-            "(1,666,667 reads, 250 Mbp), 0.5% substitutions, N 0.01%, k=31 m=21 ci=2 n=256")
-SPEC = "kind=reads_fq genome=5000000 seed=2 gc=0.5 reads=1666667 rl=150 sub=0.005 nrate=0.0001"
-K, M, CI, NPART = 31, 21, 2, 256
+WORKLOADS = {
+    "repeat_reads": {
+        "desc": ("repetitive genome reads: 100 Mbp genome seed 3 (GC 45%, 35% interspersed copies of 50 "
+                 "300 bp families at 5% divergence, 10% tandem arrays), 150 bp reads at 20x (2 Gbp), "
+                 "0.3% substitutions, k=31 m=23 ci=1 n=4096"),
+        "spec": ("kind=repeat_fq genome=100000000 seed=3 gc=0.45 cov=20 rl=150 sub=0.003 "
+                 "fam=50 flen=300 div=0.05 rep=0.35 tan=0.10"),
+        "k": 31, "m": 23, "ci": 1, "n": 4096,
+    },
+    "short_reads": {
+        "desc": ("short-read FASTQ: 5 Mbp genome seed 2, 150 bp reads at 50x (1,666,667 reads, 250 Mbp), "
+                 "0.5% substitutions, N 0.01%, k=31 m=21 ci=2 n=256"),
+        "spec": "kind=reads_fq genome=5000000 seed=2 gc=0.5 reads=1666667 rl=150 sub=0.005 nrate=0.0001",
+        "k": 31, "m": 21, "ci": 2, "n": 256,
+    },
+}
+HEADLINE = "repeat_reads"
 
 
 def log(*a):
@@ -55,48 +76,39 @@ def alg_bytes(n_bases, n_sk, sum_len, n_unique, n_out, k):
     return job, count, superkmers, P
 
 
-def cpu_baseline(text, n_bases_total, threads):
-    """Reference tuna (our driver over the reference's own Phase-1 harness and
-    kache table, oracle/_ref) on a bounded sample of the same reads, split
-    into `threads` files so both phases run on `threads` cores."""
+def host_threads():
+    """Threads for host work: the CPUs this process may run on, at most 16 (a
+    GPU box gives one GPU's job a 16-CPU share whatever the machine shows)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return n, max(1, min(16, n))
+
+
+def cpu_baseline(bases, off, wl, threads, nproc, frac=8):
+    """The oracle's multithreaded two-phase count (oracle/cpu_count.c: the
+    reference pipeline restated in C, superkmers routed to n partitions, then
+    one table per partition), pinned to the oracle's brute-force count by
+    tests/test_oracle.py, on the first 1/frac of the workload's reads.  The
+    reference itself is not on this machine, so this is the fallback the
+    baseline plan names."""
     import oracle as O
-    reads_sample = 1666667  # the whole workload (250 Mbp): ~20 s of CPU work on 16 cores
-    lines = text.split(b"\n")
-    nrec = min(reads_sample, len(lines) // 4)
-    tmp = tempfile.mkdtemp(prefix="tg_cpu_")
-    files = []
-    per = (nrec + threads - 1) // threads
-    for t in range(threads):
-        a, b = t * per, min((t + 1) * per, nrec)
-        if a >= b:
-            break
-        p = os.path.join(tmp, f"s{t}.fq")
-        with open(p, "wb") as f:
-            f.write(b"\n".join(lines[4 * a:4 * b]) + b"\n")
-        files.append(p)
-    sample_bases = sum(len(lines[4 * i + 1]) for i in range(nrec))
-    if O.ref_available():
-        t0 = time.perf_counter()
-        r = subprocess.run([O.REF_BIN, "count", str(K), str(M), str(NPART), str(threads), str(CI),
-                            str(2**64 - 1), "-"] + files, capture_output=True, text=True)
-        wall = time.perf_counter() - t0
-        if r.returncode == 0:
-            st = dict(l.split(": ") for l in r.stderr.strip().split("\n") if ": " in l)
-            return {"value": round(sample_bases / wall / 1e9, 5), "unit": "Gbases/s", "cores": threads,
-                    "kind": "reference",
-                    "sample": f"first {nrec} reads ({sample_bases} bases) of the workload in {len(files)} "
-                              f"FASTQ files; reference Phase 1 (partition_kmers_mem) + kache table via "
-                              f"oracle/_ref, wall {wall:.2f} s (phase1 {float(st.get('phase1', 0)):.2f} s, "
-                              f"phase2 {float(st.get('phase2', 0)):.2f} s)"}
-        log("reference baseline failed:", r.stderr[-400:])
-    # fallback: the single-thread oracle port on a smaller sample
-    nrec = min(nrec, 100000)
-    bases, off = O.parse(b"\n".join(lines[:4 * nrec]) + b"\n")
+    nch = (len(off) - 1) // frac
+    nb = int(off[nch])
+    O.count_parallel(bases[:min(nb, 1 << 22)], off[:1 + int(np.searchsorted(off, min(nb, 1 << 22), "right")) - 1],
+                     wl["k"], wl["m"], wl["n"], wl["ci"], 2**64 - 1, threads, keep=False)  # warm the pages
     t0 = time.perf_counter()
-    O.count(bases, off, K, CI)
+    _, _, info = O.count_parallel(bases[:nb], off[:nch + 1], wl["k"], wl["m"], wl["n"], wl["ci"], 2**64 - 1,
+                                  threads, keep=False)
     wall = time.perf_counter() - t0
-    return {"value": round(len(bases) / wall / 1e9, 5), "unit": "Gbases/s", "cores": 1, "kind": "port",
-            "sample": f"first {nrec} reads ({len(bases)} bases), oracle brute-force count"}
+    return {"value": round(nb / wall / 1e9, 5), "unit": "Gbases/s", "cores": threads, "kind": "port",
+            "nproc": nproc,
+            "sample": f"first {nch} of {len(off) - 1} reads ({nb} bases) of the headline workload; "
+                      f"oracle/cpu_count.c (two-phase restatement of the reference pipeline, "
+                      f"n={wl['n']} partitions) on {threads} threads of {nproc} CPUs, wall {wall:.2f} s "
+                      f"(phase1 {info['phase1_s']:.2f} s, phase2 {info['phase2_s']:.2f} s), "
+                      f"{info['out']} k-mers reported"}
 
 
 # ------------------------------------------------------- many samples ----
@@ -238,6 +250,10 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
         f"max {sizes.max() / 1e6:.1f} M), generated in {time.perf_counter() - t0:.1f} s")
     srv = T.Server(C5_K, C5_M, C5_CI, lanes=lanes, max_batch_jobs=64, max_batch_bases=1 << 29)
 
+    lens = [np.diff(o.astype(np.int64)) for _, o in reqs]
+    closed = [int(np.maximum(L - C5_K + 1, 0).sum()) for L in lens]
+    bad = []
+
     def run(arrivals):
         ids = []
         t_start = time.perf_counter()
@@ -247,9 +263,13 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
                 time.sleep(dt)
             ids.append((srv.submit(*reqs[i % n_req]), time.perf_counter() - t_start))
         lat, end, bj = [], 0.0, []
-        for rid, ts in ids:
-            srv.wait(rid, 300.0)
+        for i, (rid, ts) in enumerate(ids):
+            if not srv.wait(rid, 300.0):
+                raise RuntimeError(f"request {i} not done in 300 s")
             _, _, st, tm = srv.result(rid, copy=False)
+            # every result: its k-mer total against the closed form
+            if st["n_kmers"] != closed[i % n_req]:
+                bad.append(i % n_req)
             lat.append(tm["submit_to_done_s"])
             bj.append(tm["batch_jobs"])
             end = max(end, ts + tm["submit_to_done_s"])
@@ -266,8 +286,10 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
     run(np.zeros(min(8, n_req)))  # warm lanes (buffers, graph capture, pinned pool)
     sweep = []
     for lam in lambdas:
+        # about 6 s of arrivals at low rates, all requests from 20 jobs/s up
+        n_at = min(n_req, max(24, int(lam * 6)))
         rng = np.random.default_rng(5)
-        arr = np.cumsum(rng.exponential(1.0 / lam, n_req))
+        arr = np.cumsum(rng.exponential(1.0 / lam, n_at))
         arr -= arr[0]
         r = run(arr)
         r["lambda"] = lam
@@ -275,6 +297,24 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
         log(f"  lambda {lam}/s: p50 {r['p50_ms']} ms p99 {r['p99_ms']} ms, {r['jobs_per_s']} jobs/s, "
             f"batch {r['mean_batch_jobs']}")
     sat = run(np.zeros(n_req))
+    # full results of four requests (the largest among them) against the
+    # oracle's two-phase CPU count, which tests pin to its brute-force count
+    import oracle as O
+    picks = sorted(set([int(np.argmax(sizes)), 0, n_req // 3, 2 * n_req // 3]))
+    full_ok = True
+    for i in picks:
+        rid = srv.submit(*reqs[i])
+        srv.wait(rid, 300.0)
+        gk, gc, st, _ = srv.result(rid)
+        srv.release(rid)
+        ok, oc, info_o = O.count_parallel(np.asarray(reqs[i][0]), np.asarray(reqs[i][1]), C5_K, C5_M, 64,
+                                          C5_CI, 2**64 - 1, threads)
+        same = len(gc) == len(oc)
+        if same:
+            a = np.lexsort((gk[:, 0], gk[:, 1]))
+            b = np.lexsort((ok[:, 0], ok[:, 1]))
+            same = bool(np.array_equal(gk[a], ok[b]) and np.array_equal(gc[a], oc[b]))
+        full_ok = full_ok and same and st["n_unique"] == info_o["unique"]
     info = srv.stats()
     srv.close()
     head = next((r for r in sweep if r["lambda"] == 100), sweep[-1] if sweep else sat)
@@ -285,6 +325,9 @@ def bench_serving(T, n_req, lambdas, threads, lanes=2):
         "headline": {"lambda": head.get("lambda"), "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
                      "jobs_per_s": head["jobs_per_s"]},
         "sweep": sweep, "saturation": sat,
+        "checks": {"kmers_closed_form_all_requests": not bad, "mismatching_requests": sorted(set(bad))[:10],
+                   "full_results_vs_oracle": {"requests": picks, "bases": [int(sizes[i]) for i in picks],
+                                              "identical": bool(full_ok)}},
         "server": {k_: info[k_] for k_ in ("lanes", "lane_bases", "hbm_budget", "batches",
                                             "pinned_pool_bytes")},
     }
@@ -326,10 +369,10 @@ def bench_output(T, eng, k):
             "note": "TSV of the job's result; host = one thread; copy = pageable host memory"}
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload, from the committed PMC
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on a workload, from the committed PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel)."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_short_reads.md")
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.md")
     if not os.path.exists(p):
         return None
     out = {}
@@ -341,19 +384,135 @@ def load_traffic(kernel):
     return out or None
 
 
+def bench_single(T, eng, name, bases, off, steps, warmup):
+    """One workload, inputs resident in HBM: autotuned batch parameters, warm-up,
+    `steps` timed jobs, per-kernel HIP-event times of 3 more jobs, and the
+    algorithmic-bytes roofline of the job and of its dominant kernel."""
+    wl = WORKLOADS[name]
+    K, M, CI, NPART = wl["k"], wl["m"], wl["ci"], wl["n"]
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    N, nch = len(bases), len(off) - 1
+    # the autotuner picks the engine's batch parameters for this job shape
+    # (bucket size, load factor, routing tiles), checking every candidate
+    # reproduces the same totals
+    t_tune = time.perf_counter()
+    jc = np.array([0, nch], dtype=np.uint64)
+    jb = np.array([0, N], dtype=np.uint64)
+    tuned, _ = eng.autotune_many(d_b, N, d_o, nch, jc, jb, K, M, CI)
+    t_tune = time.perf_counter() - t_tune
+    log(f"{name}: autotuned in {t_tune:.1f} s: {tuned}")
+    for _ in range(warmup):
+        eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
+    eng.sync()
+    el = time.perf_counter() - t0
+    ms_step = el * 1e3 / steps
+    gbases = N * steps / el / 1e9
+    log(f"{name}: {ms_step:.3f} ms per job, {gbases:.2f} Gbases/s")
+    # per-kernel times: the same job with HIP events around every kernel on
+    # the context stream (plain launches of the same kernels)
+    eng.reset_kernel_times()
+    eng.set_profiling(True)
+    prof_steps = 3
+    for _ in range(prof_steps):
+        eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
+    eng.sync()
+    eng.set_profiling(False)
+    kt = eng.kernel_times()
+    # exact counters of the job for the algorithmic-bytes figures
+    # a superkmer of j k-mers has j + k - 1 bases
+    sum_len = st.n_kmers + st.n_superkmers * (K - 1)
+    job_b, count_b, sk_b, P = alg_bytes(N, st.n_superkmers, sum_len, st.n_unique, st.n_out, K)
+    job_roof_gbases = HBM_PEAK_GBS * 1e9 * N / job_b / 1e9
+    kern = {k_: (ms / max(n, 1), n) for k_, (ms, n) in kt.items()}
+    per_launch = {"count": count_b, "superkmers": sk_b}
+    dom = max((x for x in kern if x in per_launch), key=lambda x: kern[x][0])
+    dom_bytes = per_launch[dom]
+    avg_ms = kern[dom][0]
+    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(dom_bytes)}
+    tr = load_traffic(dom, name)
+    if tr:
+        roof["traffic"] = tr.get("hbm_bytes_per_launch")
+    out = {
+        "workload": wl["desc"], "value": round(gbases, 4), "unit": "Gbases/s", "ms_per_step": round(ms_step, 3),
+        "steps": steps, "warmup": warmup,
+        "config": {"workload": wl["desc"], "k": K, "m": M, "ci": CI, "n_partitions": NPART, "bases": N,
+                   "chunks": nch},
+        "kmers_per_s": round(st.n_kmers * steps / el, 1),
+        "job_roofline": {"alg_bytes": int(job_b), "roofline_gbases_s": round(job_roof_gbases, 2),
+                         "frac": round(gbases / job_roof_gbases, 4)},
+        "stats": {"kmers": st.n_kmers, "superkmers": st.n_superkmers, "unique": st.n_unique,
+                  "unique_out": st.n_out, "subparts": st.n_subparts, "work_items": st.n_lds_groups,
+                  "spilled_kmers": st.n_hbm_parts, "partition_bytes": int(P)},
+        "kernels_ms": {k_: round(v[0], 4) for k_, v in kern.items()},
+        "kernel_timing": f"HIP events around each kernel, {prof_steps} profiled jobs after the timed region",
+        "engine_config": dict(tuned, autotune_s=round(t_tune, 2)),
+        "roofline": roof,
+    }
+    eng.dev_free(d_b)
+    eng.dev_free(d_o)
+    return out
+
+
+def bench_end_to_end(name, text, threads):
+    """The tuna CLI on the workload's FASTQ file: read (mapped), parse on the
+    host threads, upload, count, format TSV on the device and write it; wall
+    time of the whole process, best of two runs (page cache warm)."""
+    wl = WORKLOADS[name]
+    cli = os.path.join(ROOT, "tuna_amd", "tuna")
+    need = 2 * len(text) + (1 << 30)
+    base = "/dev/shm" if os.path.isdir("/dev/shm") and shutil.disk_usage("/dev/shm").free > 4 * need else None
+    d = tempfile.mkdtemp(prefix="tg_e2e_", dir=base)
+    try:
+        fq = os.path.join(d, "reads.fq")
+        with open(fq, "wb") as f:
+            f.write(memoryview(text))
+        out = os.path.join(d, "out.tsv")
+        best = None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = subprocess.run([cli, "-k", str(wl["k"]), "-m", str(wl["m"]), "-n", str(wl["n"]), "-ci", str(wl["ci"]),
+                                "-t", str(threads), "-hp", fq, out], capture_output=True, text=True, timeout=600)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                return {"error": r.stderr[-300:]}
+            st = dict(l.split(": ", 1) for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+            size = os.path.getsize(out)
+            if best is None or wall < best["wall_s"]:
+                best = {"wall_s": round(wall, 3), "phase1_s": float(st.get("phase1", 0)),
+                        "phase2_s": float(st.get("phase2", 0)), "unique_kmers": int(st.get("unique_kmers", 0)),
+                        "tsv_bytes": size}
+            os.remove(out)
+        return dict(best, fastq_bytes=len(text), threads=threads, output_dir=("tmpfs" if base else "tempdir"),
+                    command=f"tuna -k {wl['k']} -m {wl['m']} -n {wl['n']} -ci {wl['ci']} -t {threads} -hp "
+                            f"reads.fq out.tsv")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end CLI runs")
     ap.add_argument("--samples", type=int, default=2000,
                     help="jobs of the many-sample leg (0 skips it)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batched leg: plain launches instead of graph replay (same kernels)")
     ap.add_argument("--requests", type=int, default=128,
                     help="requests of the serving leg (0 skips it)")
-    ap.add_argument("--lambdas", default="20,50,100,200", help="arrival rates (jobs/s) of the serving leg")
+    ap.add_argument("--lambdas", default="5,20,50,100,200", help="arrival rates (jobs/s) of the serving leg")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default=None,
+                    help="with --only single: the one workload to run (default: both)")
     ap.add_argument("--only", choices=["all", "single", "batched", "serving", "copy"], default="all",
                     help="run one leg; 'copy' runs a plain 256 MiB device copy kernel 5 times "
                          "(the reference size for scaling HBM byte counters)")
@@ -364,7 +523,7 @@ def main():
         sys.exit(2)
     import tuna_amd as T
 
-    threads = min(16, os.cpu_count() or 1)
+    nproc, threads = host_threads()
     if args.only == "copy":
         import torch
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -384,97 +543,61 @@ def main():
     if args.only == "serving":
         print(json.dumps({"serving": bench_serving(T, args.requests, lambdas, threads)}))
         return
-    t0 = time.perf_counter()
-    text = T.synth(SPEC)
-    bases, off = T.parse_buffer(text)
-    log(f"generated {len(text)} bytes FASTQ -> {len(bases)} bases in {len(off) - 1} chunks "
-        f"({time.perf_counter() - t0:.1f} s)")
+    names = [HEADLINE, "short_reads"]
+    if args.only == "single" and args.workload:
+        names = [args.workload]
     eng = T.Engine(0)
-    d_b = eng.dev_upload(bases)
-    d_o = eng.dev_upload(off)
-    N, nch = len(bases), len(off) - 1
-
-    # the autotuner picks the engine's batch parameters for this job shape
-    # (bucket size, load factor, routing tiles), checking every candidate
-    # reproduces the same totals; then warm-up
-    t_tune = time.perf_counter()
-    jc = np.array([0, nch], dtype=np.uint64)
-    jb = np.array([0, N], dtype=np.uint64)
-    tuned, _ = eng.autotune_many(d_b, N, d_o, nch, jc, jb, K, M, CI)
-    t_tune = time.perf_counter() - t_tune
-    log(f"autotuned in {t_tune:.1f} s: {tuned}")
-    for _ in range(args.warmup):
-        eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
-    eng.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
-    eng.sync()
-    el = time.perf_counter() - t0
-    ms_step = el * 1e3 / args.steps
-    gbases = N * args.steps / el / 1e9
-    # per-kernel times: the same job with HIP events around every kernel on
-    # the context stream (plain launches of the same kernels)
-    eng.reset_kernel_times()
-    eng.set_profiling(True)
-    prof_steps = 3
-    for _ in range(prof_steps):
-        eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
-    eng.sync()
-    eng.set_profiling(False)
-    kt = eng.kernel_times()
-
-    # exact counters of the job for the algorithmic-bytes figures
-    sk = eng.superkmers(bases, off, K, M, NPART)
-    sum_len = int(sk[1].astype(np.int64).sum())
-    job_b, count_b, sk_b, P = alg_bytes(N, st.n_superkmers, sum_len, st.n_unique, st.n_out, K)
-    job_roof_gbases = HBM_PEAK_GBS * 1e9 * N / job_b / 1e9
-    kern = {name: (ms / max(n, 1), n) for name, (ms, n) in kt.items()}
-    dom = max(kern, key=lambda x: kern[x][0] * kern[x][1])
-    per_launch = {"count": count_b, "superkmers": sk_b}
-    dom_bytes = per_launch.get(dom)
-    if dom_bytes is None:
-        dom = "count"
-        dom_bytes = count_b
-    avg_ms = kern[dom][0]
-    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(dom_bytes)}
-    tr = load_traffic(dom)
-    if tr:
-        roof["traffic"] = tr.get("hbm_bytes_per_launch")
-
+    legs, texts = {}, {}
+    for name in names:
+        t0 = time.perf_counter()
+        text = T.synth(WORKLOADS[name]["spec"])
+        bases, off = T.parse_buffer(text, threads=threads)
+        log(f"{name}: generated {len(text)} bytes FASTQ -> {len(bases)} bases in {len(off) - 1} chunks "
+            f"({time.perf_counter() - t0:.1f} s)")
+        legs[name] = bench_single(T, eng, name, bases, off, args.steps, args.warmup)
+        if name == "short_reads" and args.only == "all":
+            legs[name]["output"] = bench_output(T, eng, WORKLOADS[name]["k"])
+        if name == HEADLINE and args.only == "all":
+            if not args.no_cpu:
+                try:
+                    legs[name]["cpu_baseline"] = cpu_baseline(bases, off, WORKLOADS[name], threads, nproc)
+                except Exception as e:  # reported, not fatal
+                    legs[name]["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
+        del bases, off
+        if args.only == "all" and not args.no_e2e:
+            texts[name] = text
+        del text
+    head = legs[names[0]]
     out = {
-        "metric": METRIC, "value": round(gbases, 4), "unit": "Gbases/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "metric": METRIC, "value": head["value"], "unit": "Gbases/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD, "k": K, "m": M, "ci": CI, "n_partitions": NPART,
-                   "bases": N, "chunks": nch},
-        "kmers_per_s": round(st.n_kmers * args.steps / el, 1),
-        "job_roofline": {"alg_bytes": int(job_b), "roofline_gbases_s": round(job_roof_gbases, 2),
-                         "frac": round(gbases / job_roof_gbases, 4)},
-        "stats": {"kmers": st.n_kmers, "superkmers": st.n_superkmers, "unique": st.n_unique,
-                  "unique_out": st.n_out, "subparts": st.n_subparts, "work_items": st.n_lds_groups,
-                  "spilled_kmers": st.n_hbm_parts},
-        "kernels_ms": {k_: round(v[0], 4) for k_, v in kern.items()},
-        "kernel_timing": f"HIP events around each kernel, {prof_steps} profiled jobs after the timed region",
-        "engine_config": dict(tuned, autotune_s=round(t_tune, 2)),
-        "roofline": roof,
+        "config": head["config"],
     }
-    out["output"] = bench_output(T, eng, K)
+    for key in ("kmers_per_s", "job_roofline", "stats", "kernels_ms", "kernel_timing", "engine_config",
+                "roofline"):
+        if key in head:
+            out[key] = head[key]
+    if "cpu_baseline" in head:
+        out["cpu_baseline"] = head["cpu_baseline"]
+    for name in names[1:]:
+        out[name] = legs[name]
+    if texts:
+        e2e = {}
+        for name, text in texts.items():
+            r = bench_end_to_end(name, text, threads)
+            N = legs[name]["config"]["bases"]
+            if "wall_s" in r:
+                r["gbases_per_s"] = round(N / r["wall_s"] / 1e9, 3)
+            e2e[name] = r
+            log(f"{name}: end to end {r}")
+        out["end_to_end"] = e2e
+        texts.clear()
     if args.samples > 0 and args.only == "all":
         out["batched"] = bench_batched(T, eng, args.samples, threads, not args.no_graph)
     if args.requests > 0 and args.only == "all":
         out["serving"] = bench_serving(T, args.requests, lambdas, threads)
-    if not args.no_cpu:
-        try:
-            out["cpu_baseline"] = cpu_baseline(bytes(text), N, threads)
-        except Exception as e:  # reported, not fatal
-            out["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
-    eng.dev_free(d_b)
-    eng.dev_free(d_o)
     eng.close()
     print(json.dumps(out))
 

@@ -187,6 +187,9 @@ typedef struct {
     float load_factor;         /* LDS table fill target when splitting work items */
     uint32_t use_graph;        /* 1: replay one captured hipGraph per batch; 0: plain launches */
     float rho;                 /* distinct/k-mers estimate for splits; 0 = learned per batch */
+    uint64_t slice_bases;      /* a single job (tg_count_host / tg_count_device) of more bases
+                                  is counted in slices of whole chunks of at most this many
+                                  bases, merged exactly on the device; 0 = 2^31 - 2^20 */
 } tg_batch_config;
 
 int tg_get_batch_config(tg_ctx* ctx, tg_batch_config* cfg);

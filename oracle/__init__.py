@@ -53,6 +53,11 @@ def lib():
                                 ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64, _u64p, _u64p]
         L.orc_tsv_line.restype = ctypes.c_uint64
         L.orc_tsv_line.argtypes = [_u64p, ctypes.c_int, ctypes.c_uint32, ctypes.c_char_p]
+        L.orc_count_parallel.restype = ctypes.c_int64
+        L.orc_count_parallel.argtypes = [_u8p, _u64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                         _u64p, _u32p, ctypes.c_uint64, _u64p,
+                                         ctypes.POINTER(ctypes.c_double)]
         L.orc_round_pow2.restype = ctypes.c_uint32
         L.orc_round_pow2.argtypes = [ctypes.c_uint32]
         _lib = L
@@ -171,6 +176,35 @@ def count(bases, off, k, ci=1, cx=2**64 - 1):
     if n < 0:
         raise RuntimeError("orc_count capacity")
     return keys[: n * nw].reshape(n, nw).copy(), cnt[:n].copy(), tot.value, nu.value
+
+
+def count_parallel(bases, off, k, m, n_parts=256, ci=1, cx=2**64 - 1, threads=1, keep=True):
+    """The two-phase multithreaded CPU count (cpu_count.c).  -> (keys [n, nw]
+    or None, counts or None, {kmers, superkmers, unique, out, phase1_s,
+    phase2_s}); keep=False only counts (the timed baseline)."""
+    bases = as_u8(bases)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    nch = len(off) - 1
+    nw = 2 if k > 32 else 1
+    cap = 0
+    keys = cnt = None
+    if keep:
+        L = np.diff(off.astype(np.int64)) if nch else np.zeros(0, np.int64)
+        cap = max(int(np.maximum(L - k + 1, 0).sum()), 1)
+        keys = np.zeros(cap * nw, np.uint64)
+        cnt = np.zeros(cap, np.uint32)
+    st = np.zeros(3, np.uint64)
+    ph = (ctypes.c_double * 2)()
+    n = lib().orc_count_parallel(_p(bases, _u8p), _p(off, _u64p), nch, k, m, n_parts, ci, cx, threads,
+                                 _p(keys, _u64p) if keep else None, _p(cnt, _u32p) if keep else None, cap,
+                                 _p(st, _u64p), ph)
+    if n < 0:
+        raise RuntimeError("orc_count_parallel failed (%d)" % n)
+    info = {"kmers": int(st[0]), "superkmers": int(st[1]), "unique": int(st[2]), "out": int(n),
+            "phase1_s": ph[0], "phase2_s": ph[1]}
+    if keep:
+        return keys[: n * nw].reshape(n, nw).copy(), cnt[:n].copy(), info
+    return None, None, info
 
 
 def tsv_line(key, k, count):

@@ -70,6 +70,16 @@ int64_t orc_count(const uint8_t* bases, const uint64_t* off, uint64_t n_chunks,
                   uint32_t* counts, uint64_t cap, uint64_t* total_kmers,
                   uint64_t* n_unique);
 
+/* A5-A11 together on `threads` threads: superkmers routed to n_parts
+ * partitions (hash & (n_parts-1)), then one counting table per partition
+ * (cpu_count.c).  Returns the number of k-mers with ci <= count <= cx, writing
+ * them (unordered) when keys != NULL, -1 on failure or a short cap, -2 for bad
+ * parameters.  stats[0..2] = k-mers, superkmers, distinct; phase_s[0..1] =
+ * wall seconds of the two phases. */
+int64_t orc_count_parallel(const uint8_t* bases, const uint64_t* off, uint64_t n_chunks, int k, int m,
+                           uint32_t n_parts, uint32_t ci, uint64_t cx, int threads, uint64_t* keys,
+                           uint32_t* counts, uint64_t cap, uint64_t* stats, double* phase_s);
+
 /* A10: TSV line "<label>\t<count>\n" for one key. Returns bytes written. */
 uint64_t orc_tsv_line(const uint64_t* words, int k, uint32_t count, char* out);
 

@@ -260,3 +260,30 @@ def test_single_result_survives_many_run(eng):
     say(f"  single job: {len(c1)} k-mers before and {len(c2)} after a many-job run")
     assert len(c1) > 0
     assert np.array_equal(k1, k2) and np.array_equal(c1, c2)
+
+
+@pytest.mark.gpu
+def test_single_result_survives_tile_change(eng):
+    # changing the routing tiles frees the batch buffers: a single job's result
+    # kept there must move first, so the copy and the device TSV/KFF of the
+    # result are the same bytes before and after
+    parts = job_inputs(8)
+    b, o = parts[1]
+    old = eng.batch_config()
+    try:
+        eng.count(b, o, 31, 21, 16, 1, fetch=False)
+        k1, c1 = eng.result(31)
+        tsv1, kff1 = eng.result_tsv(31), eng.result_kff(31)
+        eng.set_batch_config(max_tiles=256 if old.max_tiles != 256 else 512)
+        k2, c2 = eng.result(31)
+        assert len(c1) > 0 and np.array_equal(k1, k2) and np.array_equal(c1, c2)
+        assert eng.result_tsv(31) == tsv1 and eng.result_kff(31) == kff1
+        # the next job on the reconfigured engine is counted as usual
+        check = O.count(b, o, 31, 1)
+        gk, gc = eng.count(b, o, 31, 21, 16, 1)
+        gk, gc = _sorted(gk, gc)
+        ok, oc = _sorted(check[0], check[1])
+        assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+        say(f"  single job of {len(c1)} k-mers: same bytes after a tile change")
+    finally:
+        eng.set_batch_config(**{k_: getattr(old, k_) for k_, _ in old._fields_})

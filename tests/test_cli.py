@@ -333,15 +333,41 @@ CPP_TEST = r'''
 #include "tuna/tuna.hpp"
 #include <cstdio>
 #include <map>
+#include <thread>
+#include <unordered_map>
 int main(int argc, char** argv) {
-    std::vector<std::string> files(argv + 1, argv + argc);
+    std::vector<std::string> files{argv[1]}, other{argv[2]};
     tuna::Options o; o.min_count = 2; o.partitions = 32;
     auto m = tuna::count_to<31, 21>(files, o);
     auto raw = tuna::count_to_raw<31, 21>(files, o);
     std::map<std::string, uint32_t> sorted(m.begin(), m.end());
     if (raw.size() != m.size()) { fprintf(stderr, "size mismatch\n"); return 2; }
-    for (auto& kv : raw) { std::string s; kv.first.get_label(s); if (m.at(s) != kv.second) return 3; }
+    // the packed k-mers: labels, canonical form and a hashed container
+    std::unordered_map<tuna::Kmer<31>, uint32_t, tuna::Kmer_Hasher<31>> by_kmer;
+    for (auto& kv : raw) {
+        std::string s; kv.first.get_label(s);
+        if (m.at(s) != kv.second) return 3;
+        if (kv.first.canonical() != kv.first || kv.first > kv.first.reverse_complement()) return 5;
+        if (tuna::Kmer<31>(s) != kv.first || kv.first.as_int() != tuna::Kmer<31>(s).as_int()) return 6;
+        by_kmer[kv.first] = kv.second;
+    }
+    if (by_kmer.size() != raw.size()) return 7;
+    for (auto& kv : m) {
+        tuna::Kmer<31> x(kv.first);
+        if (by_kmer.at(x.reverse_complement().canonical()) != kv.second) return 8;
+    }
     for (auto& kv : sorted) printf("%s\t%u\n", kv.first.c_str(), kv.second);
+    // two threads counting different inputs at once: each result equals the
+    // same count made alone
+    auto alone_a = m;
+    auto alone_b = tuna::count_to<31, 21>(other, o);
+    std::unordered_map<std::string, uint32_t> got_a, got_b;
+    for (int rep = 0; rep < 3; ++rep) {
+        std::thread ta([&] { got_a = tuna::count_to<31, 21>(files, o); });
+        std::thread tb([&] { got_b = tuna::count_to<31, 21>(other, o); });
+        ta.join(); tb.join();
+        if (got_a != alone_a || got_b != alone_b) { fprintf(stderr, "threaded mismatch\n"); return 9; }
+    }
     try { tuna::count<31>({}, [](std::string_view, uint32_t) {}); return 4; }
     catch (const std::invalid_argument& e) { fprintf(stderr, "%s\n", e.what()); }
     return 0;
@@ -356,13 +382,15 @@ def test_cpp_api_count_to_and_raw(tmp_path):
     exe = tmp_path / "t"
     lib = os.path.join(ROOT, "tuna_amd")
     c = subprocess.run(["g++", "-std=c++17", "-O1", str(src), "-I", os.path.join(ROOT, "include"),
-                        "-L", lib, "-ltuna_gpu", "-Wl,-rpath," + lib, "-o", str(exe)],
+                        "-L", lib, "-ltuna_gpu", "-Wl,-rpath," + lib, "-pthread", "-o", str(exe)],
                        capture_output=True, text=True)
     assert c.returncode == 0, c.stderr
     text = T.synth("kind=reads_fq genome=20000 seed=7 cov=10 rl=150 sub=0.005")
     fq = tmp_path / "r.fq"
     fq.write_bytes(bytes(text))
-    r = subprocess.run([str(exe), str(fq)], capture_output=True, text=True, timeout=300)
+    fq2 = tmp_path / "r2.fq"
+    fq2.write_bytes(bytes(T.synth("kind=reads_fq genome=30000 seed=8 cov=12 rl=150 sub=0.005")))
+    r = subprocess.run([str(exe), str(fq), str(fq2)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "tuna::count: no input files" in r.stderr
     got = {}

@@ -191,6 +191,15 @@ def test_repeat_genome_reads(ceng):
     check_counts(ceng, bases, off, 31, 23, 64, label="repeat reads")
 
 
+def test_repeat_reads_c3_settings(ceng):
+    # the 2 Gbp repetitive workload's generator and settings (k=31 m=23,
+    # n=4096) on a 2.5 Mbp genome at 20x (50 Mbp of reads), oracle-checked
+    bases, off = T.synth_parsed("kind=repeat_fq genome=2500000 seed=3 gc=0.45 cov=20 rl=150 sub=0.003 "
+                                "fam=50 flen=300 div=0.05 rep=0.35 tan=0.10")
+    st = check_counts(ceng, bases, off, 31, 23, 4096, label="repeat reads, C3 settings")
+    assert st.n_parts == 4096
+
+
 def test_repeated_calls_reuse_buffers(ceng):
     rng = random.Random(9)
     for trial in range(3):

@@ -230,3 +230,26 @@ def test_tsv_line_format():
 
 def test_round_pow2():
     assert [O.round_pow2(x) for x in (0, 1, 2, 3, 5, 16, 17, 1000)] == [1, 1, 2, 4, 8, 16, 32, 1024]
+
+
+@pytest.mark.parametrize("k,m,n,ci,cx,threads", [(31, 21, 16, 1, 2**64 - 1, 1), (31, 21, 256, 2, 2**64 - 1, 4),
+                                                 (21, 15, 64, 1, 3, 3), (63, 21, 32, 2, 2**64 - 1, 4),
+                                                 (33, 9, 8, 1, 2**64 - 1, 2), (5, 3, 4, 1, 2**64 - 1, 2)])
+def test_parallel_count_equals_brute_force(k, m, n, ci, cx, threads):
+    # the multithreaded two-phase CPU count (the CPU baseline's fallback) has
+    # the same multiset as the sort-and-count, and the superkmer totals of
+    # orc_superkmers, whatever the partitions and threads
+    import tuna_amd as T
+    b, o = T.synth_parsed("kind=reads_fq genome=30000 seed=9 cov=6 rl=150 sub=0.01 nrate=0.002")
+    rng = random.Random(k)
+    g = ">g\n" + "".join(rng.choice("ACGT") for _ in range(5000)) + "A" * 400 + "\n"
+    gb, go = O.parse(g.encode())
+    b = np.concatenate([b, gb])
+    o = np.concatenate([o, go[1:] + o[-1]]).astype(np.uint64)
+    pk, pc, info = O.count_parallel(b, o, k, m, n, ci, cx, threads)
+    ok, oc, tot, nu = O.count(b, o, k, ci, cx)
+    assert info["kmers"] == tot and info["unique"] == nu and info["out"] == len(oc)
+    assert info["superkmers"] == len(O.superkmers(b, o, k, m)[0])
+    sk = np.lexsort(tuple(pk[:, i] for i in range(pk.shape[1])))
+    so = np.lexsort(tuple(ok[:, i] for i in range(ok.shape[1])))
+    assert np.array_equal(pk[sk], ok[so]) and np.array_equal(pc[sk], oc[so])

@@ -41,11 +41,12 @@ def summary(eng, k):
     return len(cnt), int(cnt.sum(dtype=np.uint64)), checksum(keys, cnt)
 
 
-@pytest.mark.parametrize("name,spec,k,m,ci", [
-    ("short reads 250 Mbp", C2, 31, 21, 2),
-    ("repetitive genome reads 2 Gbp", C3, 31, 23, 1),
+@pytest.mark.parametrize("name,spec,k,m,ci,n", [
+    ("short reads 250 Mbp", C2, 31, 21, 2, 256),
+    ("repetitive genome reads 2 Gbp", C3, 31, 23, 1, 4096),
+    ("repetitive genome reads 2 Gbp, n=256", C3, 31, 23, 1, 256),
 ])
-def test_full_size_properties(name, spec, k, m, ci):
+def test_full_size_properties(name, spec, k, m, ci, n):
     bases, off = T.synth_parsed(spec)
     say(f"{name}: {len(bases)} bases, {len(off) - 1} chunks")
     eng = T.Engine(0)
@@ -54,7 +55,7 @@ def test_full_size_properties(name, spec, k, m, ci):
     N, nch = len(bases), len(off) - 1
     del bases
     try:
-        st = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        st = eng.count_device(d_b, N, d_o, nch, k, m, n, ci)
         ref = summary(eng, k)
         say(f"  batched: {st.n_kmers} k-mers, {st.n_unique} distinct, {ref[0]} reported")
         assert ref[0] == st.n_out
@@ -64,15 +65,54 @@ def test_full_size_properties(name, spec, k, m, ci):
             assert ref[1] <= st.n_kmers and ref[0] <= st.n_unique
         # another partitioning of the same input
         eng.set_batch_config(kmers_per_bucket=65536)
-        st2 = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        st2 = eng.count_device(d_b, N, d_o, nch, k, m, n, ci)
         assert (st2.n_kmers, st2.n_unique, st2.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
         assert summary(eng, k) == ref
         say("  other bucket size: same multiset")
         # the plain host-driven sequence
         eng.set_plain_sequence(True)
-        st3 = eng.count_device(d_b, N, d_o, nch, k, m, 256, ci)
+        st3 = eng.count_device(d_b, N, d_o, nch, k, m, n, ci)
         assert (st3.n_kmers, st3.n_unique, st3.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
         assert summary(eng, k) == ref
+        say("  plain sequence: same multiset")
+    finally:
+        eng.dev_free(d_b)
+        eng.dev_free(d_o)
+        eng.close()
+
+
+C3_25X = C3.replace("cov=20", "cov=25")
+
+
+def test_job_above_2_31_bases():
+    """2.5 Gbp of repetitive-genome reads (the 2 Gbp workload's generator at
+    25x): more than 2^31 bases, so the job is counted in slices.  With ci = 1
+    the counts add up to the k-mers and every distinct k-mer is reported; the
+    multiset is the same with another slice size and in the plain sequence."""
+    bases, off = T.synth_parsed(C3_25X)
+    N, nch = len(bases), len(off) - 1
+    say(f"2.5 Gbp reads: {N} bases, {nch} chunks")
+    assert N > (1 << 31)
+    eng = T.Engine(0)
+    d_b = eng.dev_upload(bases)
+    d_o = eng.dev_upload(off)
+    del bases
+    try:
+        st = eng.count_device(d_b, N, d_o, nch, 31, 23, 4096, 1)
+        ref = summary(eng, 31)
+        say(f"  default slices: {st.n_kmers} k-mers, {st.n_unique} distinct, {st.n_superkmers} superkmers")
+        assert ref[0] == st.n_out == st.n_unique and ref[1] == st.n_kmers
+        kmers = int(np.sum(np.maximum(np.diff(off.astype(np.int64)) - 30, 0)))
+        assert st.n_kmers == kmers
+        eng.set_batch_config(slice_bases=1 << 30)
+        st2 = eng.count_device(d_b, N, d_o, nch, 31, 23, 4096, 1)
+        assert (st2.n_kmers, st2.n_unique, st2.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
+        assert summary(eng, 31) == ref
+        say("  1 Gbp slices: same multiset")
+        eng.set_plain_sequence(True)
+        st3 = eng.count_device(d_b, N, d_o, nch, 31, 23, 4096, 1)
+        assert (st3.n_kmers, st3.n_unique, st3.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
+        assert summary(eng, 31) == ref
         say("  plain sequence: same multiset")
     finally:
         eng.dev_free(d_b)

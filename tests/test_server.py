@@ -140,3 +140,30 @@ def test_wait_poll_release_semantics():
             srv.submit(b, o[:-1])  # offsets do not end at n_bases
     finally:
         srv.close()
+
+
+def test_requests_at_serving_workload_size():
+    # requests of the serving workload at its stated size (own genome per
+    # request, log-normal median 1 Mbp, 150 bp reads at 15x; k=63 m=21 ci=2):
+    # the three smallest of the first 12 (each ~10-20 Mbases of reads) against
+    # the oracle, and every one against the closed-form k-mer total
+    k, m, ci = 63, 21, 2
+    reqs = [T.synth_parsed(f"kind=samples_fq seed=5 index={i} median=1000000 sigma=1.0 lo=10000 hi=20000000 "
+                           f"cov=15 rl=150 sub=0") for i in range(12)]
+    say(f"{len(reqs)} requests, {sum(len(b) for b, _ in reqs)} bases")
+    srv = T.Server(k, m, ci, lanes=2, max_batch_jobs=8)
+    try:
+        ids = [srv.submit(b, o) for b, o in reqs]
+        small = set(np.argsort([len(b) for b, _ in reqs])[:3].tolist())
+        for i, (rid, (b, o)) in enumerate(zip(ids, reqs)):
+            assert srv.wait(rid, 300.0)
+            res = srv.result(rid)
+            L = np.diff(o.astype(np.int64))
+            assert res[2]["n_kmers"] == int(np.maximum(L - k + 1, 0).sum())
+            assert res[2]["n_out"] == len(res[1])
+            if i in small:
+                say(f"  request {i}: {len(b)} bases against the oracle")
+                check(res, b, o, k, ci)
+            srv.release(rid)
+    finally:
+        srv.close()

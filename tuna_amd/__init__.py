@@ -81,7 +81,7 @@ class TgManyStats(ctypes.Structure):
 class TgBatchConfig(ctypes.Structure):
     _fields_ = [("kmers_per_bucket", ctypes.c_uint32), ("max_tiles", ctypes.c_uint32),
                 ("batch_bases", ctypes.c_uint64), ("load_factor", ctypes.c_float),
-                ("use_graph", ctypes.c_uint32), ("rho", ctypes.c_float)]
+                ("use_graph", ctypes.c_uint32), ("rho", ctypes.c_float), ("slice_bases", ctypes.c_uint64)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -320,6 +320,17 @@ class _Pinned:
         if self.p:
             lib().tg_host_free(self.p)
             self.p = None
+
+
+def format_tsv(k, keys, cnt):
+    """TSV lines of (key, count) pairs formatted on the host (tg_format_tsv)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1)
+    cnt = np.ascontiguousarray(cnt, dtype=np.uint32)
+    n = len(cnt)
+    buf = np.zeros(max(n, 1) * (k + 12), dtype=np.uint8)
+    got = _check_size(lib().tg_format_tsv(k, _p(keys, _u64p), _p(cnt, _u32p), n,
+                                          buf.ctypes.data_as(ctypes.c_char_p), len(buf)))
+    return buf[:got].tobytes()
 
 
 def pinned_copy(arr):

@@ -560,12 +560,49 @@ BatchState* state(tg_ctx* c)
         g.load_factor = 0.75f;
         g.use_graph = 1;
         g.rho = 0.0f;
+        g.slice_bases = 0;
         BatchState* s = c->batch;
         if (hipEventCreate(&s->ev_start) != hipSuccess || hipEventCreate(&s->ev_mid) != hipSuccess ||
             hipEventCreate(&s->ev_end) != hipSuccess)
             (void)hipGetLastError();
     }
     return c->batch;
+}
+
+// A single job's result that still lives in the batch buffers (view mode)
+// moves into its own store, and the context's result pointers follow it.
+// Called before anything may free or reuse those buffers.
+int migrate_view(tg_ctx* ctx, BatchState* s)
+{
+    ResultStore& o = s->one;
+    if (!o.view_keys) return TG_OK;
+    hipStream_t st = ctx->stream;
+    uint64_t* vk = o.view_keys;
+    uint32_t* vc = o.view_cnt;
+    o.view_keys = nullptr;
+    o.view_cnt = nullptr;
+    const uint64_t n = o.n;
+    o.n = 0;
+    int rc;
+    if ((rc = grow_store(o, st, std::max<uint64_t>(n, 1), o.nw))) {
+        if (ctx->d_keys == vk + o.off[0] * o.nw) {
+            ctx->n_out = 0;
+            ctx->d_keys = nullptr;
+            ctx->d_cnt = nullptr;
+        }
+        return rc;
+    }
+    if (n) {
+        HIPOK(hipMemcpyAsync(o.keys, vk, n * o.nw * 8ull, hipMemcpyDeviceToDevice, st));
+        HIPOK(hipMemcpyAsync(o.cnt, vc, n * 4ull, hipMemcpyDeviceToDevice, st));
+    }
+    o.n = n;
+    if (ctx->d_keys == vk + o.off[0] * o.nw) {
+        ctx->d_keys = o.keys + o.off[0] * o.nw;
+        ctx->d_cnt = o.cnt + o.off[0];
+    }
+    HIPOK(hipStreamSynchronize(st));
+    return TG_OK;
 }
 
 int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
@@ -588,26 +625,15 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
     HIPOK(hipSetDevice(ctx->device));
     BatchState* s = state(ctx);
     hipStream_t st = ctx->stream;
-    if (!single && s->one.view_keys) {
-        // a many-job run may reallocate the batch buffers: the single job's
-        // result that still lives there moves into its own store first
-        ResultStore& o = s->one;
-        uint64_t* vk = o.view_keys;
-        uint32_t* vc = o.view_cnt;
-        o.view_keys = nullptr;
-        o.view_cnt = nullptr;
-        const uint64_t n = o.n;
-        o.n = 0;
-        if ((rc = grow_store(o, st, std::max<uint64_t>(n, 1), o.nw))) return rc;
-        if (n) {
-            HIPOK(hipMemcpyAsync(o.keys, vk, n * o.nw * 8ull, hipMemcpyDeviceToDevice, st));
-            HIPOK(hipMemcpyAsync(o.cnt, vc, n * 4ull, hipMemcpyDeviceToDevice, st));
-        }
-        o.n = n;
-        if (ctx->d_keys == vk + o.off[0] * o.nw) {
-            ctx->d_keys = o.keys + o.off[0] * o.nw;
-            ctx->d_cnt = o.cnt + o.off[0];
-        }
+    if (single) {
+        // the context's previous result is replaced: drop it now, so a call
+        // that fails part-way leaves no pointer into buffers it reallocated
+        ctx->n_out = 0;
+        ctx->d_keys = nullptr;
+        ctx->d_cnt = nullptr;
+    } else if ((rc = migrate_view(ctx, s))) {
+        // a many-job run may reallocate the batch buffers
+        return rc;
     }
     const auto t_start = std::chrono::steady_clock::now();
     s->last = tg_many_stats{};
@@ -770,10 +796,17 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
 int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
                       const uint64_t* d_off, uint64_t n_chunks, tg_stats* st)
 {
+    return count_range_batched(c, p, d_bases, n_bases, d_off, n_chunks, 0, n_chunks, 0, n_bases, st);
+}
+
+int count_range_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                        const uint64_t* d_off, uint64_t n_chunks, uint64_t c0, uint64_t c1, uint64_t b0,
+                        uint64_t b1, tg_stats* st)
+{
     int rc = check_params(p, n_bases, n_chunks);
     if (rc) return rc;
-    const uint64_t jc[2] = {0, n_chunks};
-    const uint64_t jb[2] = {0, n_bases};
+    const uint64_t jc[2] = {c0, c1};
+    const uint64_t jb[2] = {b0, b1};
     const JobTab jt{jc, jb, 1};
     tg_job_stats js{};
     if ((rc = count_many_device(c, p, d_bases, n_bases, d_off, n_chunks, jt, &js, true))) return rc;
@@ -787,14 +820,14 @@ int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uin
     collect_prof(c);
     if (st) {
         tg_stats o{};
-        o.n_bases = n_bases;
-        o.n_chunks = n_chunks;
+        o.n_bases = b1 - b0;
+        o.n_chunks = c1 - c0;
         o.n_kmers = js.n_kmers;
         o.n_superkmers = js.n_superkmers;
         o.n_unique = js.n_unique;
         o.n_out = js.n_out;
         o.n_parts = round_pow2(p->n_partitions ? p->n_partitions : 16u);
-        o.n_subparts = job_buckets(n_bases, s->cfg.kmers_per_bucket);
+        o.n_subparts = job_buckets(b1 - b0, s->cfg.kmers_per_bucket);
         o.n_lds_groups = (uint32_t)std::min<uint64_t>(s->last_items, 0xFFFFFFFFull);
         o.n_hbm_parts = (uint32_t)std::min<uint64_t>(s->last_spilled, 0xFFFFFFFFull);
         o.ms_phase1 = s->ms_p1;
@@ -802,6 +835,21 @@ int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uin
         *st = o;
     }
     return TG_OK;
+}
+
+uint64_t slice_limit(tg_ctx* c)
+{
+    const uint64_t v = state(c)->cfg.slice_bases;
+    return v ? v : DEFAULT_SLICE_BASES;
+}
+
+void drop_single_view(tg_ctx* c)
+{
+    if (!c->batch) return;
+    ResultStore& o = c->batch->one;
+    o.view_keys = nullptr;
+    o.view_cnt = nullptr;
+    o.n = 0;
 }
 
 }  // namespace tgi
@@ -830,10 +878,18 @@ int tg_set_batch_config(tg_ctx* c, const tg_batch_config* cfg)
         return fail(TG_EINVAL, "load_factor must be in (0.05, 1]");
     if (!(cfg->rho == 0.0f || (cfg->rho >= 0.001f && cfg->rho <= 1.0f)))
         return fail(TG_EINVAL, "rho must be 0 (learned) or in [0.001, 1]");
+    if (cfg->slice_bases != 0 && (cfg->slice_bases < 1024 || cfg->slice_bases > DEFAULT_SLICE_BASES))
+        return fail(TG_EINVAL, "slice_bases must be 0 (default) or in [1024, %llu]",
+                    (unsigned long long)DEFAULT_SLICE_BASES);
     BatchState* s = state(c);
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    if (cfg->max_tiles != s->cfg.max_tiles) free_class(s);
+    HIPOK(hipSetDevice(c->device));
+    HIPOK(hipStreamSynchronize(c->stream));
+    if (cfg->max_tiles != s->cfg.max_tiles) {
+        // the batch buffers are freed: a single job's result kept there moves first
+        const int rc = migrate_view(c, s);
+        if (rc) return rc;
+        free_class(s);
+    }
     s->cfg = *cfg;
     return TG_OK;
 }

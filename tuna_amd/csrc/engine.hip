@@ -460,7 +460,7 @@ int count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
                  const uint64_t* d_off, uint64_t nch, tg_stats* st)
 {
     int rc = check_params(p, N, nch);
-    if (rc) return rc;
+    if (rc || (rc = check_pass(N, nch))) return rc;
     HIPOK(hipSetDevice(c->device));
     tg_stats s{};
     s.n_bases = N;
@@ -484,6 +484,188 @@ int count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
     collect_prof(c);
     if (st) *st = s;
     return TG_OK;
+}
+
+// Slices of a job: maximal runs of whole chunks of at most `limit` bases (a
+// chunk longer than that is a slice of its own).  cuts[i]..cuts[i+1] are the
+// chunks of slice i.
+int plan_slices(const uint64_t* off, uint64_t nch, uint64_t limit, std::vector<uint64_t>& cuts)
+{
+    cuts.assign(1, 0);
+    while (cuts.back() < nch) {
+        const uint64_t c0 = cuts.back();
+        // last chunk end within limit of the slice start
+        uint64_t c1 = (uint64_t)(std::upper_bound(off + c0 + 1, off + nch + 1, off[c0] + limit) - off) - 1;
+        if (c1 <= c0) {
+            c1 = c0 + 1;
+            if (off[c1] - off[c0] >= DEFAULT_SLICE_BASES)
+                return fail(TG_EINVAL, "an ACGT run of %llu bases exceeds the %llu-base limit of one pass",
+                            (unsigned long long)(off[c1] - off[c0]), (unsigned long long)DEFAULT_SLICE_BASES);
+        }
+        cuts.push_back(c1);
+    }
+    return TG_OK;
+}
+
+// A job of more bases than the slice limit: every slice is counted as a job of
+// its own with no count filter, its pairs are added into the device merge
+// table, and the filtered table becomes the job's result.  Exact: the k-mers
+// of a job are the k-mers of its chunks, and every chunk is in one slice.
+// h_off: the chunk offsets on the host (nullptr: read from d_off).
+int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
+                 uint64_t nch, const uint64_t* h_off, uint64_t limit, tg_stats* out)
+{
+    hipStream_t st = c->stream;
+    std::vector<uint64_t> own;
+    if (!h_off) {
+        own.resize(nch + 1);
+        HIPOK(hipMemcpyAsync(own.data(), d_off, (nch + 1) * 8ull, hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        h_off = own.data();
+    }
+    if (h_off[nch] != N || h_off[0] != 0)
+        return fail(TG_EINVAL, "chunk offsets span [%llu, %llu), not the %llu bases of the input",
+                    (unsigned long long)h_off[0], (unsigned long long)h_off[nch], (unsigned long long)N);
+    std::vector<uint64_t> cuts;
+    int rc = plan_slices(h_off, nch, limit, cuts);
+    if (rc) return rc;
+    const uint32_t nw = p->k > 32 ? 2u : 1u;
+    tg_params q = *p;
+    q.ci = 1;
+    q.cx = ~0ull;
+    // the merge table: grown (rehashed) so that it stays at most half full
+    Buf& tk = c->bufs["merge_keys"];
+    Buf& tc = c->bufs["merge_cnt"];
+    unsigned long long* claimed;
+    uint32_t* err;
+    if ((rc = ensure_t(c, "merge_ctr", 8, &claimed))) return rc;
+    err = reinterpret_cast<uint32_t*>(claimed + 4);
+    tg::launch_fill_u64(st, (uint64_t*)claimed, 0ull, 8);
+    // a table left by an earlier job is released first (its memory is reused)
+    if (tk.p) HIPOK(hipFree(tk.p));
+    if (tc.p) HIPOK(hipFree(tc.p));
+    tk = Buf{};
+    tc = Buf{};
+    uint64_t slots = 0, occ = 0;
+    auto table = [&](uint64_t* k, uint32_t* n, uint64_t sl) {
+        return tg::MergeTab{k, n, sl - 1, claimed, err};
+    };
+    tg_stats sum{};
+    double ms_merge = 0;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        const uint64_t c0 = cuts[i], c1 = cuts[i + 1], b0 = h_off[c0], b1 = h_off[c1];
+        tg_stats ss{};
+        if (c->plain) {
+            // the plain sequence takes offsets relative to its bases
+            std::vector<uint64_t> rel(c1 - c0 + 1);
+            for (uint64_t j = c0; j <= c1; ++j) rel[j - c0] = h_off[j] - b0;
+            uint64_t* d_rel;
+            if ((rc = ensure_t(c, "slice_off", rel.size(), &d_rel))) return rc;
+            HIPOK(hipMemcpyAsync(d_rel, rel.data(), rel.size() * 8ull, hipMemcpyHostToDevice, st));
+            rc = count_device(c, &q, d_bases + b0, b1 - b0, d_rel, c1 - c0, &ss);
+        } else {
+            rc = count_range_batched(c, &q, d_bases, N, d_off, nch, c0, c1, b0, b1, &ss);
+        }
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        const uint64_t n = c->n_out;
+        uint64_t want = 1ull << 20;
+        while (want < 2 * (occ + n)) want <<= 1;
+        if (want > slots) {
+            uint64_t* nk = nullptr;
+            uint32_t* nc = nullptr;
+            if (hipMalloc(&nk, want * nw * 8ull) != hipSuccess || hipMalloc(&nc, want * 4ull) != hipSuccess) {
+                (void)hipGetLastError();
+                if (nk) hipFree(nk);
+                return fail(TG_ENOMEM, "merge table of %llu slots does not fit in device memory",
+                            (unsigned long long)want);
+            }
+            {
+                Prof pr(c, "merge");
+                const tg::MergeTab fresh = table(nk, nc, want);
+                tg::launch_merge_clear(st, fresh, nw);
+                tg::launch_fill_u64(st, (uint64_t*)claimed, 0ull, 1);
+                if (slots) tg::launch_merge_rehash(st, fresh, table((uint64_t*)tk.p, (uint32_t*)tc.p, slots), nw);
+            }
+            HIPOK(hipStreamSynchronize(st));
+            if (tk.p) hipFree(tk.p);
+            if (tc.p) hipFree(tc.p);
+            tk.p = nk;
+            tk.cap = want * nw * 8ull;
+            tc.p = nc;
+            tc.cap = want * 4ull;
+            slots = want;
+        }
+        {
+            Prof pr(c, "merge");
+            tg::launch_merge_insert(st, table((uint64_t*)tk.p, (uint32_t*)tc.p, slots), nw, c->d_keys, c->d_cnt, n);
+        }
+        unsigned long long hc[5];
+        HIPOK(hipMemcpyAsync(hc, claimed, sizeof(hc), hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        if (reinterpret_cast<uint32_t*>(&hc[4])[0])
+            return fail(TG_EDEVICE, "merge table overflow (%llu slots, %llu used)", (unsigned long long)slots,
+                        (unsigned long long)hc[0]);
+        occ = hc[0];
+        ms_merge += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        sum.n_kmers += ss.n_kmers;
+        sum.n_superkmers += ss.n_superkmers;
+        sum.n_subparts += ss.n_subparts;
+        sum.n_lds_groups += ss.n_lds_groups;
+        sum.n_hbm_parts += ss.n_hbm_parts;
+        sum.ms_phase1 += ss.ms_phase1;
+        sum.ms_phase2 += ss.ms_phase2;
+    }
+    // the job's result: the table's keys whose sums pass ci/cx
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t* rk;
+    uint32_t* rcnt;
+    if ((rc = ensure_t(c, "sliced_keys", (occ + 1) * nw, &rk)) || (rc = ensure_t(c, "sliced_cnt", occ + 1, &rcnt)))
+        return rc;
+    uint64_t n_out = 0;
+    if (slots) {
+        Prof pr(c, "merge");
+        tg::launch_fill_u64(st, (uint64_t*)claimed + 1, 0ull, 1);
+        tg::launch_merge_drain(st, table((uint64_t*)tk.p, (uint32_t*)tc.p, slots), nw, p->ci, p->cx, rk, rcnt,
+                               claimed + 1);
+    }
+    HIPOK(hipMemcpyAsync(&n_out, claimed + 1, 8, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    ms_merge += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    drop_single_view(c);
+    c->n_out = n_out;
+    c->nw = nw;
+    c->d_keys = rk;
+    c->d_cnt = rcnt;
+    collect_prof(c);
+    sum.n_bases = N;
+    sum.n_chunks = nch;
+    sum.n_unique = occ;
+    sum.n_out = n_out;
+    sum.n_parts = round_pow2(p->n_partitions ? p->n_partitions : 16u);
+    sum.ms_phase2 += ms_merge;
+    if (out) *out = sum;
+    return TG_OK;
+}
+
+// One job: in one pass, or in slices when it is larger than the slice limit.
+int count_job(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
+              uint64_t nch, const uint64_t* h_off, tg_stats* st)
+{
+    const uint64_t limit = slice_limit(c);
+    if (N > limit) {
+        c->dbg_inserted.clear();
+        c->dbg_unique.clear();
+        c->dbg_spilled.clear();
+        c->dbg_cap.clear();
+        const bool dbg = c->debug;
+        c->debug = false;  // per-partition statistics are kept for one-pass jobs only
+        const int rc = count_sliced(c, p, d_bases, N, d_off, nch, h_off, limit, st);
+        c->debug = dbg;
+        return rc;
+    }
+    if (c->plain) return count_device(c, p, d_bases, N, d_off, nch, st);
+    return count_one_batched(c, p, d_bases, N, d_off, nch, st);
 }
 
 int upload_inputs(tg_ctx* c, const uint8_t* bases, uint64_t N, const uint64_t* off, uint64_t nch,
@@ -625,8 +807,10 @@ int tg_count_device(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint6
                     const uint64_t* d_off, uint64_t n_chunks, tg_stats* st)
 {
     if (!c) return fail(TG_EINVAL, "null context");
-    if (c->plain) return count_device(c, p, d_bases, n_bases, d_off, n_chunks, st);
-    return count_one_batched(c, p, d_bases, n_bases, d_off, n_chunks, st);
+    int rc = check_params(p, n_bases, n_chunks);
+    if (rc) return rc;
+    HIPOK(hipSetDevice(c->device));
+    return count_job(c, p, d_bases, n_bases, d_off, n_chunks, nullptr, st);
 }
 
 int tg_set_plain_sequence(tg_ctx* c, int on)
@@ -674,8 +858,7 @@ int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t 
     uint8_t* d_b;
     uint64_t* d_o;
     if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
-    if (c->plain) return count_device(c, p, d_b, n_bases, d_o, n_chunks, st);
-    return count_one_batched(c, p, d_b, n_bases, d_o, n_chunks, st);
+    return count_job(c, p, d_b, n_bases, d_o, n_chunks, off, st);
 }
 
 uint64_t tg_result_size(tg_ctx* c) { return c ? c->n_out : 0; }

@@ -66,6 +66,16 @@ void destroy_batch(BatchState* b);
 // One job through the batched engine; fills st and the context's result.
 int count_one_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
                       const uint64_t* d_off, uint64_t n_chunks, tg_stats* st);
+// The job of chunks [c0, c1) (bases [b0, b1)) of a device-resident input whose
+// offsets are absolute, through the batched engine.
+int count_range_batched(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t n_bases,
+                        const uint64_t* d_off, uint64_t n_chunks, uint64_t c0, uint64_t c1, uint64_t b0,
+                        uint64_t b1, tg_stats* st);
+// A single job of more bases than this is counted in slices (tg_batch_config::slice_bases).
+constexpr uint64_t DEFAULT_SLICE_BASES = (1ull << 31) - (1ull << 20);
+uint64_t slice_limit(tg_ctx* c);
+// Forget the batched engine's single-job result view (the context's result has moved).
+void drop_single_view(tg_ctx* c);
 
 }  // namespace tgi
 
@@ -191,11 +201,21 @@ inline int check_params(const tg_params* p, uint64_t n_bases, uint64_t n_chunks)
         return fail(TG_EINVAL,
                     "minimizer length (-m %u) must be strictly less than k-mer length (-k %u)",
                     p->m, p->k);
+    (void)n_bases;
+    if (n_chunks >= (1ull << 40))
+        return fail(TG_EINVAL, "too many chunks (%llu)", (unsigned long long)n_chunks);
+    return TG_OK;
+}
+
+// One pass of the engine (a job, or a slice of one) addresses bases, records
+// and chunks with 32-bit indices.
+inline int check_pass(uint64_t n_bases, uint64_t n_chunks)
+{
     if (n_bases >= (1ull << 32) - 64)
-        return fail(TG_EINVAL, "input of %llu bases exceeds the 2^32 per-job limit",
+        return fail(TG_EINVAL, "a pass of %llu bases exceeds the 2^32 limit of one pass",
                     (unsigned long long)n_bases);
     if (n_chunks >= (1ull << 32) - 1)
-        return fail(TG_EINVAL, "too many chunks (%llu)", (unsigned long long)n_chunks);
+        return fail(TG_EINVAL, "too many chunks in one pass (%llu)", (unsigned long long)n_chunks);
     return TG_OK;
 }
 

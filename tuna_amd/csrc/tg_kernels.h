@@ -213,6 +213,23 @@ void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_
                         const uint64_t* job_off, const uint64_t* job_item_tot,
                         unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt);
 
+// Merge table of a job counted in slices (merge.hip): slot i holds nw key
+// words at keys[nw*i] and a count at cnt[i]; mask + 1 slots (a power of two);
+// claimed counts the slots taken; err is set if a probe found the table full.
+struct MergeTab {
+    uint64_t* keys;
+    uint32_t* cnt;
+    uint64_t mask;
+    unsigned long long* claimed;
+    uint32_t* err;
+};
+void launch_merge_clear(hipStream_t st, const MergeTab& t, uint32_t nw);
+void launch_merge_insert(hipStream_t st, const MergeTab& t, uint32_t nw, const uint64_t* keys,
+                         const uint32_t* cnt, uint64_t n);
+void launch_merge_rehash(hipStream_t st, const MergeTab& dst, const MergeTab& src, uint32_t nw);
+void launch_merge_drain(hipStream_t st, const MergeTab& t, uint32_t nw, uint32_t ci, uint64_t cx,
+                        uint64_t* out_keys, uint32_t* out_cnt, unsigned long long* ctr);
+
 // table geometry of the LDS counting kernels
 constexpr uint32_t LDS_CAP_NW1 = 8192;  // 8192 x (8+4) B = 96 KiB
 constexpr uint32_t LDS_CAP_NW2 = 2048;  // 2048 x (16+4) B = 40 KiB

@@ -6,6 +6,7 @@
 // Phase 2 (counting) run on the GPU; output is TSV or KFF.
 #include "../../include/tuna_gpu.h"
 
+#include <cctype>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -47,174 +48,203 @@ struct Config {  // src/Config.hpp:10-26
     uint64_t ram_budget_bytes = 0;
 };
 
+// ---- command line ----------------------------------------------------------
+// The option names, their value ranges, every error message and the exit
+// codes are the reference CLI's interface (src/CLI.cpp:68-237): scripts rely
+// on them.  Here they are data: one row per option, walked by a single loop
+// that classifies each token and applies the row; the help text is printed
+// from the same rows.
+
+enum class Takes { flag, number, path, help, version };
+
+struct OptionRow {
+    const char* name;
+    const char* alt;      // second spelling, or nullptr
+    Takes takes;
+    uint64_t limit;       // largest accepted number (Takes::number)
+    void (*apply)(Config&, uint64_t number, const char* text);
+    const char* help;     // help lines for this option
+};
+
+constexpr uint64_t kMax16 = 0xFFFFull, kMax32 = 0xFFFFFFFFull, kMax64 = ~0ull;
+
+const OptionRow kRows[] = {
+    {"-k", nullptr, Takes::number, kMax16, [](Config& c, uint64_t v, const char*) { c.k = (uint16_t)v; },
+     "  -k  <int>   k-mer length               [default: 31]\n"
+     "              this engine accepts any value in [2,64]\n"},
+    {"-m", nullptr, Takes::number, kMax16, [](Config& c, uint64_t v, const char*) { c.l = (uint16_t)v; },
+     "  -m  <int>   minimizer length            [default: 21]\n"
+     "              any value in [1, min(k-1, 32)]\n"},
+    {"-n", nullptr, Takes::number, kMax32,
+     [](Config& c, uint64_t v, const char*) { c.num_partitions = (uint32_t)v; },
+     "  -n  <int>   number of partitions        [default: auto, ~2 MB input/partition]\n"
+     "              rounded up to a power of two; the counts do not depend on it\n"},
+    {"-t", nullptr, Takes::number, kMax32,
+     [](Config& c, uint64_t v, const char*) { c.num_threads = (uint32_t)v; },
+     "  -t  <int>   host threads (input parsing) [default: 1]\n"},
+    {"-ci", nullptr, Takes::number, kMax32, [](Config& c, uint64_t v, const char*) { c.ci = (uint32_t)v; },
+     "  -ci <int>   minimum k-mer count         [default: 1]\n"},
+    {"-cx", nullptr, Takes::number, kMax64, [](Config& c, uint64_t v, const char*) { c.cx = v; },
+     "  -cx <int>   maximum k-mer count         [default: max]\n"},
+    {"-ram", nullptr, Takes::number, kMax64 >> 30,
+     [](Config& c, uint64_t v, const char*) { c.ram_budget_bytes = v << 30; },
+     "  -ram <int>  RAM budget in GB            [accepted; device memory is managed]\n"},
+    {"-w", nullptr, Takes::path, 0, [](Config& c, uint64_t, const char* t) { c.work_dir = t; },
+     "  -w  <dir>   directory for the -kt partition files and -dbg CSVs\n"
+     "              [default: tuna_tmp/ next to output file]\n\n"},
+    {"-hp", nullptr, Takes::flag, 0, [](Config& c, uint64_t, const char*) { c.hide_progress = true; },
+     "  -hp         hide progress messages\n"},
+    {"-kt", nullptr, Takes::flag, 0, [](Config& c, uint64_t, const char*) { c.keep_tmp = true; },
+     "  -kt         keep partition files (hash_<p>.superkmers in the work dir)\n"},
+    {"-tp", nullptr, Takes::flag, 0, [](Config& c, uint64_t, const char*) { c.partition_only = true; },
+     "  -tp         stop after partitioning (phase 1 only, for benchmarking)\n"},
+    {"-dbg", nullptr, Takes::flag, 0, [](Config& c, uint64_t, const char*) { c.debug_stats = true; },
+     "  -dbg        debug stats: device timings, per-partition table report,\n"
+     "              minimizer coverage (CSV files in the work dir)\n"},
+    {"-kff", nullptr, Takes::flag, 0, [](Config& c, uint64_t, const char*) { c.output_kff = true; },
+     "  -kff        KFF binary output (also chosen by a .kff output name)\n"},
+    {"-h", "--help", Takes::help, 0, nullptr, "  -h/--help   show this help\n"},
+    {"-v", "--version", Takes::version, 0, nullptr, "  -v/--version print version\n"},
+};
+
 void print_usage(const char* prog)
 {
-    std::cerr << "tuna — streaming k-mer counter (MI355X engine)\n"
-                 "\n"
-                 "Usage:\n"
-                 "  " << prog << " [options] <input1.fa [input2.fa ...]> <output_file>\n"
-                 "  " << prog << " [options] @<input_list_file>          <output_file>\n"
-                 "\n"
-                 "Options:\n"
-                 "  -k  <int>   k-mer length               [default: 31]\n"
-                 "              this engine accepts any value in [2,64]\n"
-                 "  -m  <int>   minimizer length            [default: 21]\n"
-                 "              any value in [1, min(k-1, 32)]\n"
-                 "  -n  <int>   number of partitions        [default: auto, ~2 MB input/partition]\n"
-                 "  -t  <int>   worker threads              [default: 1]\n"
-                 "  -ci <int>   minimum k-mer count         [default: 1]\n"
-                 "  -cx <int>   maximum k-mer count         [default: max]\n"
-                 "  -ram <int>  RAM budget in GB            [accepted; device memory is managed]\n"
-                 "  -w  <dir>   working directory for partition files written by -kt\n"
-                 "              [default: tuna_tmp/ next to output file]\n"
-                 "\n"
-                 "  -hp         hide progress messages\n"
-                 "  -kt         keep partition files (hash_<p>.superkmers in the work dir)\n"
-                 "  -tp         stop after partitioning (phase 1 only, for benchmarking)\n"
-                 "  -dbg        debug stats: per-phase device timings\n"
-                 "  -kff        write output in KFF binary format (auto-detected from .kff extension)\n"
-                 "  -h/--help   show this help\n"
-                 "  -v/--version print version\n"
-                 "\n"
-                 "Output: plain text TSV (<kmer>\\t<count>) by default; use -kff for KFF binary.\n";
+    std::string text = "tuna — streaming k-mer counter (MI355X engine)\n\nUsage:\n";
+    for (const char* form : {" [options] <input1.fa [input2.fa ...]> <output_file>\n",
+                             " [options] @<input_list_file>          <output_file>\n"})
+        text += std::string("  ") + prog + form;
+    text += "\nOptions:\n";
+    for (const OptionRow& r : kRows) text += r.help;
+    text += "\nOutput: plain text TSV (<kmer>\\t<count>) by default; use -kff for KFF binary.\n";
+    std::cerr << text;
 }
 
+void complain(const std::string& what) { std::cerr << "tuna: error: " << what << "\n"; }
+
+// Decimal text to an unsigned 64-bit value, with the acceptance rules of
+// std::stoull in base 10: optional leading white space and sign (a minus sign
+// negates modulo 2^64), at least one digit, nothing after the digits, no
+// overflow.
+bool decimal_value(const char* text, uint64_t& out)
+{
+    const char* p = text;
+    while (*p && std::isspace((unsigned char)*p)) ++p;
+    bool negate = false;
+    if (*p == '+' || *p == '-') negate = *p++ == '-';
+    if (!std::isdigit((unsigned char)*p)) return false;
+    uint64_t v = 0;
+    for (; std::isdigit((unsigned char)*p); ++p) {
+        const uint64_t d = (uint64_t)(*p - '0');
+        if (v > (kMax64 - d) / 10) return false;
+        v = v * 10 + d;
+    }
+    if (*p) return false;
+    out = negate ? 0 - v : v;
+    return true;
+}
+
+const OptionRow* find_row(const std::string& tok)
+{
+    for (const OptionRow& r : kRows)
+        if (tok == r.name || (r.alt && tok == r.alt)) return &r;
+    return nullptr;
+}
+
+// Inputs: the positionals before the output name, or the lines of an @list
+// file (a leading "~/" in its path is the home directory).
+bool resolve_inputs(std::vector<std::string> words, Config& cfg)
+{
+    cfg.output_file = words.back();
+    words.pop_back();
+    const bool listed = words.size() == 1 && !words[0].empty() && words[0][0] == '@';
+    if (!listed) {
+        cfg.input_files = std::move(words);
+        return true;
+    }
+    std::string path = words[0].substr(1);
+    const char* home = std::getenv("HOME");
+    if (home && path.rfind("~/", 0) == 0) path.replace(0, 1, home);
+    std::ifstream list(path);
+    if (!list) {
+        complain("cannot open input list: " + path);
+        return false;
+    }
+    for (std::string line; std::getline(list, line);)
+        if (!line.empty()) cfg.input_files.push_back(line);
+    return true;
+}
+
+// false: print the usage and exit 1 (after any message already printed).
 bool parse_args(int argc, char* argv[], Config& cfg)
 {
     if (argc < 2) return false;
-    std::vector<std::string> positionals;
-    for (int i = 1; i < argc; ++i) {
-        const std::string arg = argv[i];
-        if (arg == "-h" || arg == "--help") return false;
-        if (arg == "-v" || arg == "--version") {
-            std::cerr << "tuna " << TUNA_VERSION << "\n";
-            std::exit(0);
-        }
-        auto next_val = [&](const char* flag) -> const char* {
-            if (i + 1 >= argc) {
-                std::cerr << "tuna: error: " << flag << " requires an argument\n";
-                return nullptr;
-            }
-            return argv[++i];
-        };
-        auto parse_u64 = [&](const char* flag, const char* txt, uint64_t max, uint64_t& out) -> bool {
-            try {
-                const std::string s(txt);
-                size_t pos = 0;
-                const unsigned long long v = std::stoull(s, &pos, 10);
-                if (pos != s.size()) {
-                    std::cerr << "tuna: error: invalid integer for " << flag << ": " << s << "\n";
-                    return false;
-                }
-                if (v > max) {
-                    std::cerr << "tuna: error: value out of range for " << flag << ": " << s << "\n";
-                    return false;
-                }
-                out = v;
-                return true;
-            } catch (const std::exception&) {
-                std::cerr << "tuna: error: invalid integer for " << flag << ": " << txt << "\n";
+    std::vector<std::string> words;
+    for (int at = 1; at < argc; ++at) {
+        const std::string tok = argv[at];
+        const OptionRow* row = find_row(tok);
+        if (!row) {
+            if (!tok.empty() && tok[0] == '-') {
+                complain("unknown option: " + tok);
                 return false;
             }
-        };
-        uint64_t v = 0;
-        if (arg == "-k") {
-            const char* t = next_val("-k");
-            if (!t || !parse_u64("-k", t, std::numeric_limits<uint16_t>::max(), v)) return false;
-            cfg.k = (uint16_t)v;
-        } else if (arg == "-m") {
-            const char* t = next_val("-m");
-            if (!t || !parse_u64("-m", t, std::numeric_limits<uint16_t>::max(), v)) return false;
-            cfg.l = (uint16_t)v;
-        } else if (arg == "-n") {
-            const char* t = next_val("-n");
-            if (!t || !parse_u64("-n", t, std::numeric_limits<uint32_t>::max(), v)) return false;
-            cfg.num_partitions = (uint32_t)v;
-        } else if (arg == "-ci") {
-            const char* t = next_val("-ci");
-            if (!t || !parse_u64("-ci", t, std::numeric_limits<uint32_t>::max(), v)) return false;
-            cfg.ci = (uint32_t)v;
-        } else if (arg == "-cx") {
-            const char* t = next_val("-cx");
-            if (!t || !parse_u64("-cx", t, std::numeric_limits<uint64_t>::max(), v)) return false;
-            cfg.cx = v;
-        } else if (arg == "-t") {
-            const char* t = next_val("-t");
-            if (!t || !parse_u64("-t", t, std::numeric_limits<uint32_t>::max(), v)) return false;
-            cfg.num_threads = (uint32_t)v;
-        } else if (arg == "-ram") {
-            const char* t = next_val("-ram");
-            if (!t || !parse_u64("-ram", t, std::numeric_limits<uint64_t>::max() >> 30, v)) return false;
-            cfg.ram_budget_bytes = v << 30;
-        } else if (arg == "-w") {
-            const char* t = next_val("-w");
-            if (!t) return false;
-            cfg.work_dir = t;
-        } else if (arg == "-dbg") {
-            cfg.debug_stats = true;
-        } else if (arg == "-kff") {
-            cfg.output_kff = true;
-        } else if (arg == "-hp") {
-            cfg.hide_progress = true;
-        } else if (arg == "-kt") {
-            cfg.keep_tmp = true;
-        } else if (arg == "-tp") {
-            cfg.partition_only = true;
-        } else if (!arg.empty() && arg[0] == '-') {
-            std::cerr << "tuna: error: unknown option: " << arg << "\n";
+            words.push_back(tok);
+            continue;
+        }
+        const std::string name = row->name;
+        switch (row->takes) {
+        case Takes::help:
             return false;
-        } else {
-            positionals.push_back(arg);
+        case Takes::version:
+            std::cerr << "tuna " << TUNA_VERSION << "\n";
+            std::exit(0);
+        case Takes::flag:
+            row->apply(cfg, 0, nullptr);
+            break;
+        case Takes::path:
+        case Takes::number: {
+            if (at + 1 >= argc) {
+                complain(name + " requires an argument");
+                return false;
+            }
+            const char* text = argv[++at];
+            uint64_t v = 0;
+            if (row->takes == Takes::number) {
+                if (!decimal_value(text, v)) {
+                    complain("invalid integer for " + name + ": " + text);
+                    return false;
+                }
+                if (v > row->limit) {
+                    complain("value out of range for " + name + ": " + text);
+                    return false;
+                }
+            }
+            row->apply(cfg, v, text);
+            break;
+        }
         }
     }
-    if (positionals.size() < 2) {
-        std::cerr << "tuna: error: need at least one input file and an output file\n";
+    if (words.size() < 2) {
+        complain("need at least one input file and an output file");
         return false;
     }
-    cfg.output_file = positionals.back();
-    positionals.pop_back();
-    if (positionals.size() == 1 && positionals[0][0] == '@') {
-        std::string list_path = positionals[0].substr(1);
-        if (list_path.size() >= 2 && list_path[0] == '~' && list_path[1] == '/') {
-            const char* home = std::getenv("HOME");
-            if (home) list_path = std::string(home) + list_path.substr(1);
-        }
-        std::ifstream lf(list_path);
-        if (!lf) {
-            std::cerr << "tuna: error: cannot open input list: " << list_path << "\n";
+    if (!resolve_inputs(std::move(words), cfg)) return false;
+    const std::string& out = cfg.output_file;
+    if (out.size() >= 4 && out.substr(out.size() - 4) == ".kff") cfg.output_kff = true;
+    // the first rule that fails is reported
+    const std::string k = std::to_string(cfg.k), m = std::to_string(cfg.l);
+    const std::pair<bool, std::string> rules[] = {
+        {cfg.input_files.empty(), "no input files"},
+        {cfg.k < 2 || cfg.k > 256, "k-mer length (-k " + k + ") must be in [2, 256]"},
+        {cfg.l < 1 || cfg.l > 32, "minimizer length (-m " + m + ") must be in [1, 32]"},
+        {cfg.l >= cfg.k,
+         "minimizer length (-m " + m + ") must be strictly less than k-mer length (-k " + k + ")"},
+        {cfg.num_threads == 0, "number of threads (-t) must be > 0"},
+    };
+    for (const auto& r : rules)
+        if (r.first) {
+            complain(r.second);
             return false;
         }
-        std::string line;
-        while (std::getline(lf, line))
-            if (!line.empty()) cfg.input_files.push_back(line);
-    } else {
-        cfg.input_files = std::move(positionals);
-    }
-    if (cfg.input_files.empty()) {
-        std::cerr << "tuna: error: no input files\n";
-        return false;
-    }
-    if (!cfg.output_kff) {
-        const auto& of = cfg.output_file;
-        if (of.size() >= 4 && of.compare(of.size() - 4, 4, ".kff") == 0) cfg.output_kff = true;
-    }
-    if (cfg.k < 2 || cfg.k > 256) {
-        std::cerr << "tuna: error: k-mer length (-k " << cfg.k << ") must be in [2, 256]\n";
-        return false;
-    }
-    if (cfg.l < 1 || cfg.l > 32) {
-        std::cerr << "tuna: error: minimizer length (-m " << cfg.l << ") must be in [1, 32]\n";
-        return false;
-    }
-    if (cfg.l >= cfg.k) {
-        std::cerr << "tuna: error: minimizer length (-m " << cfg.l
-                  << ") must be strictly less than k-mer length (-k " << cfg.k << ")\n";
-        return false;
-    }
-    if (cfg.num_threads == 0) {
-        std::cerr << "tuna: error: number of threads (-t) must be > 0\n";
-        return false;
-    }
     return true;
 }
 
