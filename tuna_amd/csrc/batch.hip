@@ -9,7 +9,8 @@
 // order on the device.  The sequence needs no host round trip: every size the
 // host does not know (strips, flagged strips, records, work items) is read by
 // the kernels from device memory, grids cover a capacity class, and the whole
-// sequence is captured once per class as a hipGraph and replayed per batch.
+// sequence is captured once per class as two hipGraphs (the two phases) and
+// replayed per batch.
 #include "engine_impl.h"
 
 #include <chrono>
@@ -113,8 +114,11 @@ struct BatchState {
     bool have_cls = false;
     BatchBufs b{};
     std::vector<void*> allocs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // the launch sequence as two captured graphs, phase 1 (pack .. routing)
+    // and phase 2 (work items .. readback); the phase-split event is
+    // recorded on the stream between their launches, never inside a graph
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint8_t* h_tab = nullptr;  // pinned host copy of the per-batch tables
     uint8_t* h_back = nullptr; // pinned readback
     size_t h_tab_bytes = 0, h_back_bytes = 0;
@@ -127,7 +131,6 @@ struct BatchState {
     // device-time split of the last call (phase 1: pack..routing; phase 2:
     // counting and gathering), from events around and inside the sequence
     hipEvent_t ev_start = nullptr, ev_mid = nullptr, ev_end = nullptr;
-    bool mid_in_graph = true;   // the captured graph records ev_mid
     double ms_p1 = 0, ms_p2 = 0;
     uint64_t last_items = 0, last_spilled = 0;
 };
@@ -138,10 +141,12 @@ namespace {
 
 void free_class(BatchState* s)
 {
-    if (s->exec) hipGraphExecDestroy(s->exec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    s->exec = nullptr;
-    s->graph = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (s->exec[i]) hipGraphExecDestroy(s->exec[i]);
+        if (s->graph[i]) hipGraphDestroy(s->graph[i]);
+        s->exec[i] = nullptr;
+        s->graph[i] = nullptr;
+    }
     for (void* p : s->allocs) hipFree(p);
     s->allocs.clear();
     s->b = BatchBufs{};
@@ -233,13 +238,15 @@ int alloc_class(BatchState* s, const BatchClass& c)
 
 // The launch sequence of one batch (no host synchronisation inside).
 // prof: the context whose per-kernel timing is on (nullptr while capturing);
-// mid: record ev_mid between the phases.
-void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
+// parts: bit 0 phase 1 (pack .. routing), bit 1 phase 2 (work items ..
+// readback); with both, ev_mid is recorded between them.
+void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
 {
     const BatchClass& c = s->cls;
     BatchBufs& b = s->b;
     const tg::BatchJobs jobs{b.job_chunk, b.job_route};
     const uint32_t J = c.cap_jobs;
+    if (parts & 1) {
     {
         // clear state (kernels only: the sequence may be captured) and inputs
         Prof pr(prof, "batch_init");
@@ -349,7 +356,9 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
         Prof pr(prof, "route_scatter");
         tg::launch_route_scatter(st, r, c.max_tiles, b.route_tmp, c.rw);
     }
-    if (mid) (void)hipEventRecord(s->ev_mid, st);
+    }
+    if (parts == 3) (void)hipEventRecord(s->ev_mid, st);
+    if (!(parts & 2)) return;
     // work items and counting
     {
         Prof pr(prof, "count_items");
@@ -402,27 +411,24 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, bool mid)
 
 int ensure_graph(BatchState* s, hipStream_t st)
 {
-    if (s->exec) return TG_OK;
-    // kernel attributes are set outside the capture; the phase-split event is
-    // recorded inside the graph when the runtime supports capturing it
-    for (int attempt = 0;; ++attempt) {
+    // kernel attributes are set before any capture (set_kernel_attributes)
+    for (int part = 0; part < 2; ++part) {
+        if (s->exec[part]) continue;
         HIPOK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        enqueue_batch(s, st, nullptr, s->mid_in_graph);
+        enqueue_batch(s, st, nullptr, part + 1);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
-        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0);
-        if (e == hipSuccess) {
-            s->graph = g;
-            s->last.n_captures += 1;
-            return TG_OK;
-        }
-        (void)hipGetLastError();
-        if (g) hipGraphDestroy(g);
-        s->exec = nullptr;
-        if (!s->mid_in_graph || attempt)
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->exec[part], g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (g) hipGraphDestroy(g);
+            s->exec[part] = nullptr;
             return fail(TG_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-        s->mid_in_graph = false;
+        }
+        s->graph[part] = g;
+        s->last.n_captures += part == 0;
     }
+    return TG_OK;
 }
 
 int grow_store(ResultStore& r, hipStream_t st, uint64_t need, uint32_t nw)
@@ -688,15 +694,15 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
             const uint64_t tab_bytes = 256 + 4ull * (c.cap_jobs + 1) + 4ull * c.cap_jobs;
             HIPOK(hipMemcpyAsync(s->b.tab, s->h_tab, tab_bytes, hipMemcpyHostToDevice, st));
             HIPOK(hipEventRecord(s->ev_start, st));
-            bool have_mid = true;
             if (s->cfg.use_graph && !ctx->profiling) {
                 if ((rc = ensure_graph(s, st))) return rc;
-                HIPOK(hipGraphLaunch(s->exec, st));
+                HIPOK(hipGraphLaunch(s->exec[0], st));
+                HIPOK(hipEventRecord(s->ev_mid, st));
+                HIPOK(hipGraphLaunch(s->exec[1], st));
                 s->last.n_graph_launches += 1;
-                have_mid = s->mid_in_graph;
             } else {
                 // plain launches (also whenever per-kernel timing is on)
-                enqueue_batch(s, st, ctx, true);
+                enqueue_batch(s, st, ctx, 3);
                 HIPOK(hipGetLastError());
             }
             HIPOK(hipEventRecord(s->ev_end, st));
@@ -712,12 +718,10 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
             HIPOK(hipStreamSynchronize(st));
             {
                 float a1 = 0, a2 = 0;
-                if (have_mid && hipEventElapsedTime(&a1, s->ev_start, s->ev_mid) == hipSuccess &&
+                if (hipEventElapsedTime(&a1, s->ev_start, s->ev_mid) == hipSuccess &&
                     hipEventElapsedTime(&a2, s->ev_mid, s->ev_end) == hipSuccess) {
                     s->ms_p1 += a1;
                     s->ms_p2 += a2;
-                } else if (hipEventElapsedTime(&a1, s->ev_start, s->ev_end) == hipSuccess) {
-                    s->ms_p1 += a1;  // no split available: all of it as phase 1
                 }
                 (void)hipGetLastError();
             }

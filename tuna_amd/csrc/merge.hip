@@ -10,7 +10,8 @@
 // (src/pipeline.hpp:186-263); here the unit of a pass is a slice of the input
 // and the per-key sums make the passes exact.
 //
-// Layout: slot i holds nw key words at keys[nw*i] and a u32 count at cnt[i].
+// Layout: slot i holds nw key words at keys[nw*i] (word 0, the low 64 bits of
+// the key, first, as in the result arrays) and a u32 count at cnt[i].
 // k <= 32: a slot is claimed by a CAS of its key word; EMPTY64 (~0) is never a
 // canonical key (the reverse complement of ~0 is 0, the smaller).  k > 32: the
 // count word is the slot state (0 empty, LOCKED being written, else the
@@ -56,16 +57,16 @@ __device__ uint32_t merge_add(const MergeTab& t, uint64_t hi, uint64_t lo, uint3
             const uint32_t st = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             if (st == 0u) {
                 if (atomicCAS(cp, 0u, LOCKED) == 0u) {
-                    __hip_atomic_store(&t.keys[2 * s], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&t.keys[2 * s + 1], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&t.keys[2 * s], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&t.keys[2 * s + 1], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_store(cp, c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                     return 1u;
                 }
                 continue;  // lost the claim: read the slot again
             }
             if (st == LOCKED) continue;  // being written by its claimer
-            if (__hip_atomic_load(&t.keys[2 * s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == hi &&
-                __hip_atomic_load(&t.keys[2 * s + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == lo) {
+            if (__hip_atomic_load(&t.keys[2 * s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == lo &&
+                __hip_atomic_load(&t.keys[2 * s + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == hi) {
                 atomicAdd(cp, c);
                 return 0u;
             }
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(MT) void k_merge_rehash(MergeTab dst, MergeTab src)
             const uint64_t key = src.keys[s];
             if (key != EMPTY64) claimed += merge_add<1>(dst, 0ull, key, c);
         } else if (c) {
-            claimed += merge_add<2>(dst, src.keys[2 * s], src.keys[2 * s + 1], c);
+            claimed += merge_add<2>(dst, src.keys[2 * s + 1], src.keys[2 * s], c);
         }
     }
     const uint32_t tot = block_sum(claimed, s_red);
