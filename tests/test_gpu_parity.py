@@ -260,3 +260,24 @@ def test_debug_min_coverage_matches_oracle(eng, k, m):
         exp = sorted(hist.items())
         assert list(zip(cov.tolist(), cnt.tolist())) == exp
         assert int((cov.astype(np.int64) * cnt.astype(np.int64)).sum()) == km
+
+
+@pytest.mark.parametrize("n", [32768, 1 << 18])
+def test_partition_export_beyond_sub_partitions(eng, n):
+    # more reference partitions than device sub-partitions (2^14): records are
+    # grouped by H & (n-1) on the host; every partition's record bytes equal
+    # the oracle's records of that partition (orc_pack_record, pinned to the
+    # reference's writer by the golden vectors)
+    rng = random.Random(n)
+    strings = [rs(rng, rng.randint(40, 3000)) for _ in range(60)] + ["ACG" * 900]
+    bases, off = O.chunks_of(strings)
+    k, m = 31, 21
+    say(f"partitions n={n}")
+    parts = eng.partitions(bases, off, k, m, n)
+    st, ln, mp, hs, _ = O.superkmers(bases, off, k, m)
+    want = {}
+    for s0, l0, m0, h0 in zip(st.tolist(), ln.tolist(), mp.tolist(), hs.tolist()):
+        rec = O.pack_record(bytes(bases[s0:s0 + l0]), m0, k, m)
+        want.setdefault(h0 & (n - 1), []).append(rec)
+    got = {P: sorted(r) for P, r in enumerate(parts) if r}
+    assert got == {P: sorted(r) for P, r in want.items()}

@@ -91,3 +91,29 @@ def test_slice_config_validation(seng):
         seng.set_batch_config(slice_bases=100)
     with pytest.raises(T.TunaGpuError, match="slice_bases"):
         seng.set_batch_config(slice_bases=1 << 31)
+
+
+def test_sliced_phase1_exports(seng):
+    # the superkmer and partition exports of an input larger than the slice
+    # limit: the same superkmers (global starts), sub-partitions and reference
+    # partition bytes as in one pass
+    b, o = inputs()[1][1]
+    r2, r2o = inputs()[0][1]
+    bases = np.concatenate([b, r2])
+    off = np.concatenate([o, r2o[1:] + o[-1]]).astype(np.uint64)
+    seng.set_batch_config(slice_bases=0)
+    one = seng.superkmers(bases, off, 31, 21, 64)
+    parts_one = seng.partitions(bases, off, 31, 21, 64)
+    seng.set_batch_config(slice_bases=3000)
+    sl = seng.superkmers(bases, off, 31, 21, 64)
+    parts_sl = seng.partitions(bases, off, 31, 21, 64)
+    assert sl[4] == one[4]
+    a = sorted(zip(*(x.tolist() for x in one[:4])))
+    c = sorted(zip(*(x.tolist() for x in sl[:4])))
+    assert a == c and len(a) == len(O.superkmers(bases, off, 31, 21)[0])
+    assert [sorted(p) for p in parts_one] == [sorted(p) for p in parts_sl]
+    seng.set_batch_config(slice_bases=0)
+    cov_one = seng.debug_min_coverage(bases, off, 31, 21)
+    seng.set_batch_config(slice_bases=3000)
+    cov_sl = seng.debug_min_coverage(bases, off, 31, 21)
+    assert all(np.array_equal(x, y) for x, y in zip(cov_one, cov_sl))

@@ -37,7 +37,7 @@ struct Phase1 {
 // slim = true keeps (start, meta) records for the export entry points; the
 // counting path uses fat records that carry their bases.
 int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N,
-               const uint64_t* d_off, uint64_t nch, Phase1& o, bool slim = false)
+               const uint64_t* d_off, uint64_t nch, Phase1& o, bool slim = false, uint32_t nsp_min = 0)
 {
     hipStream_t st = c->stream;
     o.k = p->k;
@@ -45,7 +45,9 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     o.W = p->k - p->m + 1;
     o.rw = slim ? 2u : tg::fat_rw(p->k);
     o.n = round_pow2(p->n_partitions ? p->n_partitions : 16u);
-    if (o.n > (1u << 14)) return fail(TG_EINVAL, "n_partitions %u exceeds 16384 on the GPU engine", o.n);
+    if (o.n > (1u << tg::HSEL_BITS))
+        return fail(TG_EINVAL, "n_partitions %u exceeds the %u partitions of the GPU engine", o.n,
+                    1u << tg::HSEL_BITS);
     o.log2n = ilog2(o.n);
     // strip length: short chunks are one strip; long chunks are cut so that
     // the look-back (256-k k-mers) stays a small fraction
@@ -183,14 +185,16 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         uint64_t tile = 16384;
         while ((o.n_rec + tile - 1) / tile > tg::ROUTE_MAX_TILES) tile <<= 1;
         o.ntiles = (uint32_t)((o.n_rec + tile - 1) / tile);
-        // sub-partitions: ~16K k-mers each (N/16384), at least the reference n,
-        // at least 2, at most 2^14 (LDS histograms)
+        // sub-partitions: ~16K k-mers each (N/16384), at least the reference n
+        // and nsp_min, at least 2, at most 2^14 (LDS histograms).  With more
+        // than 2^14 reference partitions a sub-partition holds several of them
+        // (its top bits); the exports split them on the host.
         {
             uint64_t want = 2;
             while (want < o.n_kmers / 16384 && want < (1u << 14)) want <<= 1;
-            if (want < o.n) want = o.n;
-            o.nsp = (uint32_t)want;
-            o.log2s = ilog2(o.nsp) - o.log2n;
+            want = std::max<uint64_t>({want, o.n, nsp_min});
+            o.nsp = (uint32_t)std::min<uint64_t>(want, 1u << 14);
+            o.log2s = o.nsp >= o.n ? ilog2(o.nsp) - o.log2n : 0;
             o.sb = tg::HSEL_BITS - ilog2(o.nsp);
         }
         uint32_t *hsk, *hkm, *route_tmp;
@@ -373,6 +377,8 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
         const uint32_t stride = o.nsp >= 256 ? o.nsp / 64 : 0;
         // debug statistics per reference partition (src/count.hpp:42-60)
         const bool dbg = c->debug;
+        if (dbg && o.n > o.nsp)
+            return fail(TG_EINVAL, "debug statistics support at most %u partitions (-n %u)", o.nsp, o.n);
         uint32_t pshift = 0;
         while ((o.n << pshift) < o.nsp) ++pshift;
         std::vector<uint64_t> d_ins(dbg ? o.n : 0), d_uni(dbg ? o.n : 0), d_sp(dbg ? o.n : 0),
@@ -679,6 +685,61 @@ int upload_inputs(tg_ctx* c, const uint8_t* bases, uint64_t N, const uint64_t* o
     return TG_OK;
 }
 
+// A superkmer of a host input, as the exports need it: the global index of its
+// first base and its slim meta word (length, minimizer offset, routing field).
+struct SlimRec {
+    uint64_t start;
+    uint32_t meta;
+};
+
+// Phase 1 (slim records) over a host input, in slices of whole chunks when it
+// is larger than the slice limit.  Every slice uses the same sub-partition
+// count (that of the whole input), so routing fields mean the same in all.
+int superkmers_all(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t N, const uint64_t* off,
+                   uint64_t nch, std::vector<SlimRec>& out, Phase1& info)
+{
+    int rc;
+    if (off[nch] != N || off[0] != 0)
+        return fail(TG_EINVAL, "chunk offsets span [%llu, %llu), not the %llu bases of the input",
+                    (unsigned long long)off[0], (unsigned long long)off[nch], (unsigned long long)N);
+    std::vector<uint64_t> cuts;
+    if ((rc = plan_slices(off, nch, slice_limit(c), cuts))) return rc;
+    uint64_t kmers = 0;
+    for (uint64_t i = 0; i < nch; ++i)
+        if (off[i + 1] - off[i] >= p->k) kmers += off[i + 1] - off[i] - p->k + 1;
+    uint64_t want = 2;
+    while (want < kmers / 16384 && want < (1u << 14)) want <<= 1;
+    out.clear();
+    std::vector<uint64_t> rel;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i) {
+        const uint64_t c0 = cuts[i], c1 = cuts[i + 1], b0 = off[c0];
+        rel.resize(c1 - c0 + 1);
+        for (uint64_t j = c0; j <= c1; ++j) rel[j - c0] = off[j] - b0;
+        uint8_t* d_b;
+        uint64_t* d_o;
+        if ((rc = check_pass(off[c1] - b0, c1 - c0)) ||
+            (rc = upload_inputs(c, bases + b0, off[c1] - b0, rel.data(), c1 - c0, &d_b, &d_o)))
+            return rc;
+        Phase1 o;
+        if ((rc = run_phase1(c, p, d_b, off[c1] - b0, d_o, c1 - c0, o, true, (uint32_t)want))) return rc;
+        std::vector<uint2> recs(o.n_sk);
+        if (o.n_sk) HIPOK(hipMemcpy(recs.data(), o.sorted, o.n_sk * sizeof(uint2), hipMemcpyDeviceToHost));
+        out.reserve(out.size() + recs.size());
+        for (const uint2& r : recs) out.push_back(SlimRec{b0 + r.x, r.y});
+        info = o;
+    }
+    info.n = round_pow2(p->n_partitions ? p->n_partitions : 16u);
+    info.log2n = ilog2(info.n);
+    if (cuts.size() < 2) {  // no chunks: no pass ran
+        info.nsp = (uint32_t)std::max<uint64_t>(want, std::min<uint64_t>(info.n, 1u << 14));
+        info.sb = tg::HSEL_BITS - ilog2(info.nsp);
+    }
+    info.n_sk = out.size();
+    info.n_kmers = kmers;
+    collect_prof(c);
+    return TG_OK;
+}
+
 }  // namespace
 
 // =================================================================== C ABI ==
@@ -900,27 +961,22 @@ int tg_superkmers_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     int rc = check_params(p, n_bases, n_chunks);
     if (rc) return rc;
     HIPOK(hipSetDevice(c->device));
-    uint8_t* d_b;
-    uint64_t* d_o;
-    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    std::vector<SlimRec> recs;
     Phase1 o;
-    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
-    collect_prof(c);
-    if (n_sk) *n_sk = o.n_sk;
+    if ((rc = superkmers_all(c, p, bases, n_bases, off, n_chunks, recs, o))) return rc;
+    if (n_sk) *n_sk = recs.size();
     if (n_subparts) *n_subparts = o.nsp;
-    const uint64_t n = std::min<uint64_t>(o.n_sk, cap);
-    std::vector<uint2> recs(n);
-    if (n) HIPOK(hipMemcpy(recs.data(), o.sorted, n * sizeof(uint2), hipMemcpyDeviceToHost));
+    const uint64_t n = std::min<uint64_t>(recs.size(), cap);
     const uint32_t shift = tg::REC_HSEL_SHIFT + o.sb;
     for (uint64_t i = 0; i < n; ++i) {
-        if (start) start[i] = recs[i].x;
-        if (len) len[i] = recs[i].y & tg::REC_LEN_MASK;
-        if (minpos) minpos[i] = (recs[i].y >> tg::REC_MINPOS_SHIFT) & 63u;
-        if (pid) pid[i] = recs[i].y >> shift;
+        if (start) start[i] = recs[i].start;
+        if (len) len[i] = recs[i].meta & tg::REC_LEN_MASK;
+        if (minpos) minpos[i] = (recs[i].meta >> tg::REC_MINPOS_SHIFT) & 63u;
+        if (pid) pid[i] = recs[i].meta >> shift;
     }
-    return o.n_sk > cap ? fail(TG_ECAP, "superkmer buffer too small (%llu needed)",
-                               (unsigned long long)o.n_sk)
-                        : TG_OK;
+    return recs.size() > cap ? fail(TG_ECAP, "superkmer buffer too small (%llu needed)",
+                                    (unsigned long long)recs.size())
+                             : TG_OK;
 }
 
 int64_t tg_debug_min_coverage(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
@@ -931,14 +987,9 @@ int64_t tg_debug_min_coverage(tg_ctx* c, const tg_params* p, const uint8_t* base
     int rc = check_params(p, n_bases, n_chunks);
     if (rc) return rc;
     HIPOK(hipSetDevice(c->device));
-    uint8_t* d_b;
-    uint64_t* d_o;
-    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    std::vector<SlimRec> recs;
     Phase1 o;
-    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
-    collect_prof(c);
-    std::vector<uint2> recs(o.n_sk);
-    if (o.n_sk) HIPOK(hipMemcpy(recs.data(), o.sorted, o.n_sk * sizeof(uint2), hipMemcpyDeviceToHost));
+    if ((rc = superkmers_all(c, p, bases, n_bases, off, n_chunks, recs, o))) return rc;
     // k-mers per minimizer, keyed by the canonical ntHash of the m-mer at the
     // superkmer's minimizer position, as count_partition does under -dbg
     // (src/count.hpp:99-103,141-144 with Kmer_Window::init_packed_with_min,
@@ -946,11 +997,11 @@ int64_t tg_debug_min_coverage(tg_ctx* c, const tg_params* p, const uint8_t* base
     const uint32_t m = p->m;
     std::unordered_map<uint64_t, uint32_t> per_min;
     per_min.reserve(recs.size() / 2 + 16);
-    for (const uint2& r : recs) {
-        const uint32_t len = r.y & tg::REC_LEN_MASK, mp = (r.y >> tg::REC_MINPOS_SHIFT) & 63u;
+    for (const SlimRec& r : recs) {
+        const uint32_t len = r.meta & tg::REC_LEN_MASK, mp = (r.meta >> tg::REC_MINPOS_SHIFT) & 63u;
         uint64_t f = 0, rv = 0;
         for (uint32_t i = 0; i < m; ++i) {
-            const uint32_t ch = bases[(uint64_t)r.x + mp + i];
+            const uint32_t ch = bases[r.start + mp + i];
             const uint32_t kc = ((ch >> 2) ^ (ch >> 1)) & 3u, nt = kc ^ (kc >> 1);
             f ^= tg::rotl64(tg::nt_seed(nt), m - 1u - i);
             rv ^= tg::rotl64(tg::nt_seed(nt ^ 2u), i);
@@ -981,27 +1032,28 @@ int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     int rc = check_params(p, n_bases, n_chunks);
     if (rc) return rc;
     HIPOK(hipSetDevice(c->device));
-    uint8_t* d_b;
-    uint64_t* d_o;
-    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    std::vector<SlimRec> recs;
     Phase1 o;
-    if ((rc = run_phase1(c, p, d_b, n_bases, d_o, n_chunks, o, true))) return rc;
-    collect_prof(c);
-    std::vector<uint2> recs(o.n_sk);
-    std::vector<uint64_t> poff(o.nsp + 1);
-    if (o.n_sk) HIPOK(hipMemcpy(recs.data(), o.sorted, o.n_sk * sizeof(uint2), hipMemcpyDeviceToHost));
-    HIPOK(hipMemcpy(poff.data(), o.part_off, (o.nsp + 1) * 8ull, hipMemcpyDeviceToHost));
-    if (n_sk) *n_sk = o.n_sk;
-    // reference partition P = sub-partitions [P*s, (P+1)*s)
-    const uint32_t s = 1u << o.log2s;
+    if ((rc = superkmers_all(c, p, bases, n_bases, off, n_chunks, recs, o))) return rc;
+    if (n_sk) *n_sk = recs.size();
+    // group by reference partition P = H & (n-1), the top log2(n) bits of the
+    // routing field (counting sort; order inside a partition is unspecified,
+    // as it is in the reference's partition files)
+    const uint32_t n = o.n, pshift = tg::REC_HSEL_SHIFT + (tg::HSEL_BITS - o.log2n);
+    std::vector<uint64_t> cnt(n + 1, 0);
+    for (const SlimRec& r : recs) ++cnt[(r.meta >> pshift) + 1];
+    for (uint32_t q = 0; q < n; ++q) cnt[q + 1] += cnt[q];
     if (part_off)
-        for (uint32_t P = 0; P <= o.n; ++P) part_off[P] = poff[(uint64_t)P * s];
+        for (uint32_t q = 0; q <= n; ++q) part_off[q] = cnt[q];
+    std::vector<uint64_t> order(recs.size());
+    for (uint64_t i = 0; i < recs.size(); ++i) order[cnt[recs[i].meta >> pshift]++] = i;
     // serialise in the reference record format (src/superkmer_io.hpp:154-172);
     // hdr_t is one byte for every k <= 64 (2k-m <= 127)
     uint64_t used = 0;
-    for (uint64_t i = 0; i < o.n_sk; ++i) {
-        const uint32_t L = recs[i].y & tg::REC_LEN_MASK;
-        const uint32_t mp = (recs[i].y >> tg::REC_MINPOS_SHIFT) & 63u;
+    for (uint64_t i = 0; i < recs.size(); ++i) {
+        const SlimRec& r = recs[order[i]];
+        const uint32_t L = r.meta & tg::REC_LEN_MASK;
+        const uint32_t mp = (r.meta >> tg::REC_MINPOS_SHIFT) & 63u;
         const uint64_t need = 2 + (L + 3) / 4;
         if (rec_off && i < rec_cap) rec_off[i] = used;
         if (rec_bytes && used + need <= bytes_cap) {
@@ -1010,17 +1062,17 @@ int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
             d[1] = (uint8_t)mp;
             memset(d + 2, 0, (L + 3) / 4);
             for (uint32_t j = 0; j < L; ++j) {
-                const uint8_t ch = bases[recs[i].x + j];
+                const uint8_t ch = bases[r.start + j];
                 d[2 + j / 4] |= (uint8_t)((((ch >> 2) ^ (ch >> 1)) & 3u) << (6u - 2u * (j % 4u)));
             }
         }
         used += need;
     }
-    if (rec_off && o.n_sk < rec_cap) rec_off[o.n_sk] = used;
+    if (rec_off && recs.size() < rec_cap) rec_off[recs.size()] = used;
     if (n_bytes) *n_bytes = used;
-    if (used > bytes_cap || o.n_sk + 1 > rec_cap)
+    if (used > bytes_cap || recs.size() + 1 > rec_cap)
         return fail(TG_ECAP, "record buffers too small (%llu bytes, %llu records)",
-                    (unsigned long long)used, (unsigned long long)o.n_sk);
+                    (unsigned long long)used, (unsigned long long)recs.size());
     return TG_OK;
 }
 

@@ -619,11 +619,6 @@ int main(int argc, char* argv[])
         cfg.num_partitions = tg_auto_partitions(fp.data(), (uint32_t)fp.size(), cfg.k, cfg.num_threads);
     }
     cfg.num_partitions = round_pow2(cfg.num_partitions);
-    if (cfg.num_partitions > (1u << 14)) {
-        std::cerr << "tuna: error: -n " << cfg.num_partitions
-                  << " exceeds the 16384 partitions of the gfx950 engine\n";
-        return 1;
-    }
     if (cfg.work_dir.empty()) {
         std::filesystem::path parent = std::filesystem::path(cfg.output_file).parent_path();
         if (parent.empty()) parent = std::filesystem::current_path();
