@@ -384,7 +384,7 @@ def load_traffic(kernel, workload):
     return out or None
 
 
-def bench_single(T, eng, name, bases, off, steps, warmup):
+def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
     """One workload, inputs resident in HBM: autotuned batch parameters, warm-up,
     `steps` timed jobs, per-kernel HIP-event times of 3 more jobs, and the
     algorithmic-bytes roofline of the job and of its dominant kernel."""
@@ -399,9 +399,16 @@ def bench_single(T, eng, name, bases, off, steps, warmup):
     t_tune = time.perf_counter()
     jc = np.array([0, nch], dtype=np.uint64)
     jb = np.array([0, N], dtype=np.uint64)
-    tuned, _ = eng.autotune_many(d_b, N, d_o, nch, jc, jb, K, M, CI)
+    if config:
+        kw = {}
+        for item in config.split(","):
+            key, v = item.split("=")
+            kw[key] = float(v) if key in ("load_factor", "rho") else int(v)
+        tuned = eng.set_batch_config(**kw).as_dict()
+    else:
+        tuned, _ = eng.autotune_many(d_b, N, d_o, nch, jc, jb, K, M, CI)
     t_tune = time.perf_counter() - t_tune
-    log(f"{name}: autotuned in {t_tune:.1f} s: {tuned}")
+    log(f"{name}: {'configured' if config else 'autotuned'} in {t_tune:.1f} s: {tuned}")
     for _ in range(warmup):
         eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
     eng.sync()
@@ -511,6 +518,9 @@ def main():
     ap.add_argument("--requests", type=int, default=128,
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--lambdas", default="5,20,50,100,200", help="arrival rates (jobs/s) of the serving leg")
+    ap.add_argument("--config", default=None,
+                    help="single legs: batch parameters as key=value,... instead of autotuning them "
+                         "(e.g. kmers_per_bucket=4096,max_tiles=1024,load_factor=0.9)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default=None,
                     help="with --only single: the one workload to run (default: both)")
     ap.add_argument("--only", choices=["all", "single", "batched", "serving", "copy"], default="all",
@@ -554,7 +564,7 @@ def main():
         bases, off = T.parse_buffer(text, threads=threads)
         log(f"{name}: generated {len(text)} bytes FASTQ -> {len(bases)} bases in {len(off) - 1} chunks "
             f"({time.perf_counter() - t0:.1f} s)")
-        legs[name] = bench_single(T, eng, name, bases, off, args.steps, args.warmup)
+        legs[name] = bench_single(T, eng, name, bases, off, args.steps, args.warmup, args.config)
         if name == "short_reads" and args.only == "all":
             legs[name]["output"] = bench_output(T, eng, WORKLOADS[name]["k"])
         if name == HEADLINE and args.only == "all":
