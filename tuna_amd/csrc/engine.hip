@@ -1039,14 +1039,16 @@ int tg_partitions_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint
     // group by reference partition P = H & (n-1), the top log2(n) bits of the
     // routing field (counting sort; order inside a partition is unspecified,
     // as it is in the reference's partition files)
+    // (the shift is 32 when n = 1: taken on 64 bits, where it gives 0)
     const uint32_t n = o.n, pshift = tg::REC_HSEL_SHIFT + (tg::HSEL_BITS - o.log2n);
+    auto part = [&](const SlimRec& r) { return (uint64_t)r.meta >> pshift; };
     std::vector<uint64_t> cnt(n + 1, 0);
-    for (const SlimRec& r : recs) ++cnt[(r.meta >> pshift) + 1];
+    for (const SlimRec& r : recs) ++cnt[part(r) + 1];
     for (uint32_t q = 0; q < n; ++q) cnt[q + 1] += cnt[q];
     if (part_off)
         for (uint32_t q = 0; q <= n; ++q) part_off[q] = cnt[q];
     std::vector<uint64_t> order(recs.size());
-    for (uint64_t i = 0; i < recs.size(); ++i) order[cnt[recs[i].meta >> pshift]++] = i;
+    for (uint64_t i = 0; i < recs.size(); ++i) order[cnt[part(recs[i])]++] = i;
     // serialise in the reference record format (src/superkmer_io.hpp:154-172);
     // hdr_t is one byte for every k <= 64 (2k-m <= 127)
     uint64_t used = 0;
