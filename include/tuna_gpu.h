@@ -336,6 +336,14 @@ int64_t tg_result_tsv(tg_ctx* ctx, uint32_t k, uint64_t first, uint64_t n, char*
 int64_t tg_result_kff(tg_ctx* ctx, uint32_t k, uint64_t first, uint64_t n, uint8_t* out,
                       uint64_t cap);
 int tg_result_write(tg_ctx* ctx, uint32_t k, const char* path, int kff);
+/* tg_result_tsv for callers that cannot allocate device memory (serving loops
+ * that stream a result in chunks, code that must not call hipMalloc): the same
+ * arguments, bytes and errors, for at most 262144 lines per call (TG_EINVAL
+ * beyond).  Its device scratch is one fixed array of about 20 MB per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race.  No counterpart in the reference, which has no
+ * device memory. */
+int64_t tg_result_tsv_noalloc(tg_ctx* ctx, uint32_t k, uint64_t first, uint64_t n, char* out, uint64_t cap);
 
 /* ---- synthetic inputs (seeded, deterministic) ------------------------- */
 /* Generate the FASTA/FASTQ text of a named workload into a library-owned

@@ -36,7 +36,7 @@ EXPORTS = [
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
     "tg_set_debug_stats", "tg_debug_partitions", "tg_debug_min_coverage",
-    "tg_result_tsv", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
+    "tg_result_tsv", "tg_result_tsv_noalloc", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
     "tg_unmap_file",
 ]
 
@@ -171,6 +171,9 @@ def lib():
     L.tg_format_tsv.argtypes = [ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, ctypes.c_char_p,
                                 ctypes.c_uint64]
     L.tg_write_kff.argtypes = [ctypes.c_char_p, ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64]
+    L.tg_result_tsv_noalloc.restype = ctypes.c_int64
+    L.tg_result_tsv_noalloc.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p,
+                                        ctypes.c_uint64]
     L.tg_result_tsv.restype = ctypes.c_int64
     L.tg_result_tsv.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p,
                                 ctypes.c_uint64]

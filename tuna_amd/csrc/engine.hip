@@ -764,6 +764,7 @@ int tg_ctx_create(int device, tg_ctx** out)
         delete c;
         return fail(TG_EDEVICE, "HIP context setup failed");
     }
+    noalloc_setup(device);
     *out = c;
     return TG_OK;
 }

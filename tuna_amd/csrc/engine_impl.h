@@ -76,6 +76,8 @@ constexpr uint64_t DEFAULT_SLICE_BASES = (1ull << 31) - (1ull << 20);
 uint64_t slice_limit(tg_ctx* c);
 // Forget the batched engine's single-job result view (the context's result has moved).
 void drop_single_view(tg_ctx* c);
+// Resolve the device's copy of the no-allocation scratch array (format.hip).
+void noalloc_setup(int device);
 
 }  // namespace tgi
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <mutex>
 
 namespace tg {
 
@@ -241,6 +242,53 @@ int kff_device(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n, uint8_t** d_ou
     return TG_OK;
 }
 
+// ---- the TSV entry point that allocates nothing -------------------------
+// tg_result_tsv_noalloc formats up to NOALLOC_TSV_MAX lines per call with all
+// of its device scratch (tile byte totals, their scan, the formatted bytes) in
+// one file-scope device array.  Every GPU holds its own copy of the array; its
+// address on a device is taken once (hipGetSymbolAddress, at the first context
+// created on that device) and only that address is handed to kernels and
+// copies.  Calls on one GPU share the array: calls that overlap on one GPU race.
+constexpr uint64_t NOALLOC_TSV_MAX = 1ull << 18;                   // lines per call
+constexpr uint64_t NOALLOC_TILES = NOALLOC_TSV_MAX / tg::TSV_TILE;  // 512
+constexpr uint64_t NOALLOC_TILE_BYTES = 8ull * (NOALLOC_TILES + 1);
+constexpr uint64_t NOALLOC_SCAN_BYTES = 8ull * 1024;                // >= scan_tmp_words(512) + 2 words
+constexpr uint64_t NOALLOC_OUT_OFF = 2 * NOALLOC_TILE_BYTES + NOALLOC_SCAN_BYTES;
+constexpr uint64_t NOALLOC_OUT_BYTES = NOALLOC_TSV_MAX * tg::TSV_LINE_MAX + 256;
+constexpr uint64_t SCRATCH_BYTES = (NOALLOC_OUT_OFF + NOALLOC_OUT_BYTES + 255) / 256 * 256;
+
+__device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[MAX_DEVICES] = {};
+hipError_t g_scratch_err[MAX_DEVICES] = {};
+
+}  // namespace
+
+namespace tgi {
+
+// Called with `device` current, once per context creation; resolves the
+// device's copy of the scratch array once per device.
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = e;
+        return;
+    }
+    g_scratch_ptr[device] = static_cast<uint8_t*>(ptr);
+}
+
+}  // namespace tgi
+
+namespace {
+
 }  // namespace
 
 extern "C" {
@@ -357,6 +405,47 @@ int tg_result_write(tg_ctx* c, uint32_t k, const char* path, int kff)
     }
     if (fclose(f) != 0 && rc == TG_OK) io_fail("failed while writing");
     return rc;
+}
+
+int64_t tg_result_tsv_noalloc(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n, char* out, uint64_t cap)
+{
+    int rc;
+    if ((rc = check_fmt(c, k, first, n))) return rc;
+    if (n > NOALLOC_TSV_MAX)
+        return fail(TG_EINVAL, "tg_result_tsv_noalloc formats at most %llu lines per call (%llu asked)",
+                    (unsigned long long)NOALLOC_TSV_MAX, (unsigned long long)n);
+    if (c->device < 0 || c->device >= MAX_DEVICES) return fail(TG_EINVAL, "device %d out of range", c->device);
+    uint8_t* base;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        base = g_scratch_ptr[c->device];
+        if (!base)
+            return fail(TG_EDEVICE, "scratch array of device %d unavailable: %s", c->device,
+                        hipGetErrorString(g_scratch_err[c->device]));
+    }
+    HIPOK(hipSetDevice(c->device));
+    const uint64_t tiles = (n + tg::TSV_TILE - 1) / tg::TSV_TILE;
+    uint64_t* tb = reinterpret_cast<uint64_t*>(base);
+    uint64_t* toff = reinterpret_cast<uint64_t*>(base + NOALLOC_TILE_BYTES);
+    uint64_t* tmp = reinterpret_cast<uint64_t*>(base + 2 * NOALLOC_TILE_BYTES);
+    uint8_t* d_out = base + NOALLOC_OUT_OFF;
+    uint64_t total = 0;
+    if (tiles) {
+        tg::k_tsv_len<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_cnt + first, n, k, tb);
+        tg::launch_scan_u64(c->stream, tb, tiles, toff, tmp);
+        tg::k_tsv_write<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_keys + first * c->nw,
+                                                                           c->d_cnt + first, n, k, c->nw,
+                                                                           toff, d_out);
+        HIPOK(hipGetLastError());
+        HIPOK(hipMemcpyAsync(&total, toff + tiles, 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPOK(hipStreamSynchronize(c->stream));
+    if (!out) return (int64_t)total;
+    if (total > cap) return fail(TG_ECAP, "TSV needs %llu bytes, buffer holds %llu", (unsigned long long)total,
+                                 (unsigned long long)cap);
+    if (total) HIPOK(hipMemcpyAsync(out, d_out, total, hipMemcpyDeviceToHost, c->stream));
+    HIPOK(hipStreamSynchronize(c->stream));
+    return (int64_t)total;
 }
 
 }  // extern "C"
