@@ -281,3 +281,26 @@ def test_partition_export_beyond_sub_partitions(eng, n):
         want.setdefault(h0 & (n - 1), []).append(rec)
     got = {P: sorted(r) for P, r in enumerate(parts) if r}
     assert got == {P: sorted(r) for P, r in want.items()}
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (31, 23), (21, 15), (17, 9), (25, 5), (32, 16)])
+def test_tandem_repeats_and_homopolymers(ceng, k, m):
+    # records with a short period (tandem units of 1-12 bases, homopolymers),
+    # alone and as reads with substitutions, between random flanks: their
+    # k-mers recur every period and are counted with weights
+    rng = random.Random(7000 + k)
+    strings = []
+    for unit_len in range(1, 13):
+        unit = rs(rng, unit_len)
+        arr = unit * (600 // unit_len + 3)
+        strings.append(rs(rng, 40) + arr + rs(rng, 40))
+        for _ in range(30):
+            a = rng.randint(0, len(arr) - 150)
+            r = list(arr[a:a + 150])
+            if rng.random() < 0.5:
+                r[rng.randint(0, 149)] = rng.choice("ACGT")
+            strings.append("".join(r))
+    strings += ["A" * 500, "AC" * 300, "ACGTTGCA" * 80, rs(rng, 3000)]
+    bases, off = O.chunks_of(strings)
+    check_counts(ceng, bases, off, k, m, 16, label="tandem repeats")
+    check_counts(ceng, bases, off, k, m, 16, ci=3, cx=40, label="tandem repeats, ci=3 cx=40")

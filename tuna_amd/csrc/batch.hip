@@ -67,6 +67,9 @@ struct BatchBufs {
     uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_start, *run_cnt, *flags;
     uint64_t *strip_slot, *run_cnt64, *run_off;
     uint32_t *recs, *sorted, *route_tmp;  // rw words per record
+    uint32_t* route_meta;                 // meta word of every coarse-ordered record
+    uint64_t* split_off;                  // fine scatter's split-group starts
+    uint32_t* split_flags;                // coarse buckets it orders by split group
     uint32_t* metas;                      // meta word per record slot
     uint32_t *hsk, *hkm, *bucket_job, *item_job;
     uint64_t *part_sk, *part_km, *part_off, *bucket_item;
@@ -215,6 +218,9 @@ int alloc_class(BatchState* s, const BatchClass& c)
         (rc = balloc(s, &b.recs, (slots + 1) * c.rw)) || (rc = balloc(s, &b.metas, slots + 1)) ||
         (rc = balloc(s, &b.sorted, (c.rec_cap + 1) * c.rw)) ||
         (rc = balloc(s, &b.route_tmp, (c.rec_cap + 1) * c.rw)) ||
+        (rc = balloc(s, &b.route_meta, c.rec_cap + 1)) ||
+        (rc = balloc(s, &b.split_off, (1ull << tg::HSEL_BITS) + 2)) ||
+        (rc = balloc(s, &b.split_flags, 256)) ||
         (rc = balloc(s, &b.hsk, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.hkm, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.part_sk, c.nsp + 1)) || (rc = balloc(s, &b.part_km, c.nsp + 1)) ||
@@ -338,6 +344,12 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
     r.max_tiles = c.max_tiles;
     r.rec_cap = c.rec_cap;
     r.err = b.flags + F_SKERR;
+    r.tmp_meta = b.route_meta;
+    r.split_off = b.split_off;
+    r.split_bits = tg::route_split_bits(c.nsp);
+    r.split_km = b.part_km;
+    r.split_scale = &b.desc->scale;
+    r.split_flags = b.split_flags;
     {
         Prof pr(prof, "route_scan");
         tg::launch_u32_to_u64(st, b.run_cnt, b.n_runs, b.run_cnt64);
@@ -364,7 +376,7 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
         Prof pr(prof, "count_items");
         tg::launch_build_items(st, b.part_km, b.part_off, c.nsp, c.xb, b.bucket_job, &b.desc->scale,
                                b.items, b.item_job, b.bucket_item, b.flags + F_NITEMS, b.part_sk,
-                               b.scan_tmp);
+                               b.scan_tmp, b.split_off, tg::route_split_bits(c.nsp));
     }
     tg::CountArgs ca{};
     ca.rec = b.sorted;
@@ -373,6 +385,7 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
     ca.ci = c.ci;
     ca.cx = c.cx;
     ca.probe_max = 64;
+    ca.per_len = c.k + (c.k - c.m + 1);
     ca.sp_keys = b.sp_keys;
     ca.sp_cnt = b.sp_cnt;
     ca.sp_mask = c.sp_slots - 1;

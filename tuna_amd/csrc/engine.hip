@@ -224,6 +224,12 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         r.hkm = hkm;
         r.part_off = o.part_off;
         r.dst = o.sorted;
+        r.tmp_meta = nullptr;  // (the plain sequence's items filter split buckets)
+        r.split_off = nullptr;
+        r.split_bits = tg::route_split_bits(o.nsp);
+        r.split_km = nullptr;
+        r.split_scale = nullptr;
+        r.split_flags = nullptr;
         {
             Prof pr(c, "route_hist");
             tg::launch_route_hist(st, r, o.ntiles, o.rw);
@@ -353,6 +359,7 @@ int run_phase2(tg_ctx* c, const tg_params* p, const Phase1& o, tg_stats* st_out)
         a.ci = p->ci;
         a.cx = p->cx;
         a.probe_max = 64;
+        a.per_len = p->k + (p->k - p->m + 1);
         a.sp_keys = sp_keys;
         a.sp_cnt = sp_cnt;
         a.sp_mask = sp_slots - 1;

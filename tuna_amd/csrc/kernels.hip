@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 namespace tg {
 
@@ -1058,7 +1059,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     constexpr uint32_t CH = 16384u / RW;  // records per chunk (64 KiB)
     constexpr uint32_t PER = CH / ROUTE_T;
     __shared__ Rec<RW> s_rec[CH];
-    __shared__ uint32_t s_hist[256], s_start[256], s_cur[256];
+    __shared__ uint32_t s_hist[256], s_start[256], s_cur[256], s_ord[256];
     __shared__ uint64_t s_off[COARSE_RUNS + 1];
     __shared__ int64_t s_base[COARSE_RUNS];
     __shared__ uint32_t s_meta[4];
@@ -1072,6 +1073,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
         uint32_t o = (uint32_t)a.part_off[c * F];
         for (uint32_t f = 0; f < F; ++f) o += hs[c * F + f];
         s_cur[c] = o;
+        s_ord[c] = a.split_flags ? a.split_flags[c] : 0u;
     }
     const uint64_t lo = (uint64_t)blockIdx.x * g.tile;
     const uint64_t hi = min(lo + g.tile, g.n_rec);
@@ -1125,8 +1127,11 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
         const uint32_t nch = (uint32_t)min((uint64_t)CH, hi - cb);
         for (uint32_t i = tid; i < nch; i += ROUTE_T) {
             const Rec<RW> r = s_rec[i];
-            const uint32_t b = rec_meta<RW>(r) >> shift;
-            store_rec<RW>(tmp, (uint64_t)s_cur[b] + (i - s_start[b]), r);
+            const uint32_t meta = rec_meta<RW>(r);
+            const uint32_t b = meta >> shift;
+            const uint64_t at = (uint64_t)s_cur[b] + (i - s_start[b]);
+            store_rec<RW>(tmp, at, r);
+            if (s_ord[b]) a.tmp_meta[at] = meta;  // the fine pass sizes these buckets' groups
         }
         __syncthreads();
         if (tid < nc) s_cur[tid] += s_hist[tid];
@@ -1134,28 +1139,72 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     }
 }
 
-// Scatter, level 2: one workgroup per coarse bucket reads its region
-// (contiguous) in chunks that fit in LDS (64 KiB), groups each chunk by
-// sub-partition there (histogram, scan, place) and writes the groups out as
-// runs, so consecutive threads write consecutive addresses.
+// Scatter, level 2: one workgroup per coarse bucket.  Its records are put in
+// groups: the sub-partition, then the top split_bits of the split field (G =
+// F << split_bits <= 512 groups).  A first pass over the region's meta words
+// sizes the groups, so every group is written contiguously at its final place;
+// the second pass reads the region in chunks that fit in LDS (128 KiB), groups
+// each chunk there (histogram, scan, place) and writes the groups out as runs,
+// so consecutive threads write consecutive addresses.  A bucket's records are
+// thus ordered by split group, and a work item that counts 1/2^i of a bucket
+// reads one contiguous range.
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp)
 {
-    constexpr uint32_t CH = 16384u / RW;  // records per chunk (64 KiB)
+    constexpr uint32_t CH = 32768u / RW;  // records per chunk (128 KiB): >= 16 per group
     constexpr uint32_t PER = CH / ROUTE_T;
     __shared__ Rec<RW> s_rec[CH];
     __shared__ uint32_t s_red[ROUTE_T / 64];
-    extern __shared__ uint32_t s_dyn[];  // F cursors, F counts, F group starts
+    extern __shared__ uint32_t s_dyn[];  // G cursors, G counts, G group starts
     const uint32_t F = 1u << log2f, c = blockIdx.x, tid = threadIdx.x;
+    // split groups only in coarse buckets holding a bucket the work items
+    // will split (split_flags); elsewhere the groups are the sub-partitions,
+    // whose starts are known
+    const uint32_t sbits = a.split_flags && a.split_flags[c] ? a.split_bits : 0u;
+    const uint32_t gbits = log2f + sbits, G = 1u << gbits;
     uint32_t* cur = s_dyn;
-    uint32_t* hist = s_dyn + F;
-    uint32_t* gst = s_dyn + 2 * F;
-    for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[c * F + f];
-    const uint64_t b = a.part_off[c * F], e = a.part_off[(c + 1) * F];
-    const uint32_t shift = REC_HSEL_SHIFT + a.sb;
-    const uint32_t fm = F - 1u;
+    uint32_t* hist = s_dyn + G;
+    uint32_t* gst = s_dyn + 2 * G;
+    const uint64_t b = a.part_off[c << log2f], e = a.part_off[(c + 1) << log2f];
+    const uint32_t shift = REC_HSEL_SHIFT + a.sb - sbits;
+    const uint32_t gm = G - 1u;
+    if (!sbits) {
+        for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[(c << log2f) + f];
+    } else {
+        // group sizes of the whole region (from the meta words), then their
+        // absolute starts; the end of the region closes its last group
+        for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
+        __syncthreads();
+        constexpr uint32_t U = 8;  // loads in flight per thread
+        for (uint64_t i0 = b + tid; i0 < e; i0 += U * ROUTE_T) {
+            uint32_t mv[U];
+#pragma unroll
+            for (uint32_t j = 0; j < U; ++j) {
+                const uint64_t i = i0 + (uint64_t)j * ROUTE_T;
+                mv[j] = i < e ? a.tmp_meta[i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < U; ++j)
+                if (i0 + (uint64_t)j * ROUTE_T < e) atomicAdd(&hist[(mv[j] >> shift) & gm], 1u);
+        }
+        __syncthreads();
+        uint32_t carry = 0;
+        for (uint32_t g0 = 0; g0 < G; g0 += blockDim.x) {
+            const uint32_t g = g0 + tid;
+            uint32_t tot;
+            const uint32_t ex = block_excl_u32(g < G ? hist[g] : 0u, s_red, tot);
+            if (g < G) {
+                cur[g] = (uint32_t)b + carry + ex;
+                a.split_off[((uint64_t)c << gbits) + g] = b + carry + ex;
+            }
+            carry += tot;
+            __syncthreads();
+        }
+        if (tid == 0) a.split_off[(uint64_t)(c + 1) << gbits] = e;
+    }
+    __syncthreads();
     for (uint64_t cb = b; cb < e; cb += CH) {
-        for (uint32_t f = tid; f < F; f += blockDim.x) hist[f] = 0;
+        for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
         __syncthreads();
         Rec<RW> v[PER];
         uint32_t fk[PER], pos[PER];
@@ -1165,18 +1214,18 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
             fk[j] = 0xFFFFFFFFu;
             if (i < e) {
                 v[j] = load_rec<RW>(tmp, i);
-                fk[j] = (rec_meta<RW>(v[j]) >> shift) & fm;
+                fk[j] = (rec_meta<RW>(v[j]) >> shift) & gm;
                 pos[j] = atomicAdd(&hist[fk[j]], 1u);
             }
         }
         __syncthreads();
-        // exclusive scan of the F counts (F <= blockDim.x * 4 handled in a loop)
+        // exclusive scan of the G counts (G <= 1024: in steps of blockDim.x)
         uint32_t carry = 0;
-        for (uint32_t f0 = 0; f0 < F; f0 += blockDim.x) {
-            const uint32_t f = f0 + tid;
+        for (uint32_t g0 = 0; g0 < G; g0 += blockDim.x) {
+            const uint32_t g = g0 + tid;
             uint32_t tot;
-            const uint32_t ex = block_excl_u32(f < F ? hist[f] : 0u, s_red, tot);
-            if (f < F) gst[f] = carry + ex;
+            const uint32_t ex = block_excl_u32(g < G ? hist[g] : 0u, s_red, tot);
+            if (g < G) gst[g] = carry + ex;
             carry += tot;
             __syncthreads();
         }
@@ -1187,13 +1236,40 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
         const uint32_t nch = (uint32_t)min((uint64_t)CH, e - cb);
         for (uint32_t i = tid; i < nch; i += ROUTE_T) {
             const Rec<RW> r = s_rec[i];
-            const uint32_t f = (rec_meta<RW>(r) >> shift) & fm;
-            store_rec<RW>(a.dst, (uint64_t)cur[f] + (i - gst[f]), r);
+            const uint32_t g = (rec_meta<RW>(r) >> shift) & gm;
+            store_rec<RW>(a.dst, (uint64_t)cur[g] + (i - gst[g]), r);
         }
         __syncthreads();
-        for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] += hist[f];
+        for (uint32_t g = tid; g < G; g += blockDim.x) cur[g] += hist[g];
         __syncthreads();
     }
+}
+
+uint32_t route_split_bits(uint32_t nsp)
+{
+    uint32_t log2f = 0;
+    while ((nsp >> log2f) > 256u) ++log2f;
+    uint32_t lnsp = 0;
+    while ((1u << lnsp) < nsp) ++lnsp;
+    const uint32_t xb = HSEL_BITS - lnsp;
+    // at most 8 split groups per bucket (items of up to 8 splits read one
+    // range; more splits filter) and 512 groups: a 128 KiB chunk then writes
+    // runs of 16 records or more per group on average (coalesced stores)
+    return min(min(xb, 3u), 9u - min(log2f, 9u));
+}
+
+// Coarse buckets to split-order: those holding a bucket whose estimated
+// distinct k-mers exceed one table (k-mers x scale > 1, the rule of
+// bucket_splits), so the fine pass lays out its split groups.
+__global__ __launch_bounds__(256) void k_route_flags(RouteArgs a, uint32_t log2f)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= (a.nsp >> log2f)) return;
+    const float scale = *a.split_scale;
+    uint32_t any = 0;
+    for (uint32_t f = 0; f < (1u << log2f); ++f)
+        any |= (float)a.split_km[(c << log2f) + f] * scale > 1.0f ? 1u : 0u;
+    a.split_flags[c] = any;
 }
 
 void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t* tmp, uint32_t rw)
@@ -1203,9 +1279,11 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     uint32_t log2f = 0;
     while ((a.nsp >> log2f) > 256u) ++log2f;
     const uint32_t nc = a.nsp >> log2f;
-    // nsp <= 2^HSEL_BITS, so F = nsp / nc <= FINE_F_MAX (the fine scatter's LDS)
+    // groups of the fine scatter: F = nsp / nc sub-partitions of 2^split_bits
+    // split groups each, G <= FINE_F_MAX (its LDS)
     set_kernel_attributes();
-    const size_t lc = 0, lf = 3ull * (1u << log2f) * sizeof(uint32_t);
+    if (a.split_flags) k_route_flags<<<1, 256, 0, st>>>(a, log2f);
+    const size_t lc = 0, lf = 3ull * (1u << (log2f + a.split_bits)) * sizeof(uint32_t);
     if (rw == 2) {
         k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
         k_route_fine<2><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
@@ -1296,7 +1374,7 @@ __device__ __forceinline__ SpillRef spill_ref(const CountArgs& a)
 }
 
 // Global spill table, one 64-bit word per key: claim by CAS on the key.
-__device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x)
+__device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x, uint32_t w = 1u)
 {
     if (__hip_atomic_load(t.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t slot = mix64(x ^ 0x9e3779b97f4a7c15ull) & t.mask;
@@ -1304,13 +1382,13 @@ __device__ __noinline__ bool spill_insert1(SpillRef t, uint64_t x)
         unsigned long long* kp = (unsigned long long*)&t.keys[slot];
         const uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == x) {
-            atomicAdd(&t.cnt[slot], 1u);
+            atomicAdd(&t.cnt[slot], w);
             return true;
         }
         if (cur == EMPTY64) {
             const uint64_t old = atomicCAS(kp, (unsigned long long)EMPTY64, (unsigned long long)x);
             if (old == EMPTY64 || old == x) {
-                atomicAdd(&t.cnt[slot], 1u);
+                atomicAdd(&t.cnt[slot], w);
                 return true;
             }
         }
@@ -1359,7 +1437,7 @@ struct SpKey {
 };
 
 template <int SW>
-__device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
+__device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key, uint32_t w = 1u)
 {
     if (__hip_atomic_load(t.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t h = 0x9e3779b97f4a7c15ull;
@@ -1375,7 +1453,7 @@ __device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
                 for (int i = 0; i < SW; ++i)
                     __hip_atomic_store(&t.keys[SW * slot + i], key.w[i], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(cp, w, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 return true;
             }
             continue;
@@ -1387,7 +1465,7 @@ __device__ __noinline__ bool spill_insert_w(SpillRef t, SpKey key)
             eq = eq && __hip_atomic_load(&t.keys[SW * slot + i], __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT) == key.w[i];
         if (eq) {
-            atomicAdd(cp, 1u);
+            atomicAdd(cp, w);
             return true;
         }
         slot = (slot + 1) & t.mask;
@@ -1538,6 +1616,34 @@ __device__ __forceinline__ uint32_t slot_count(const uint64_t* k0, const uint32_
 // per entry, the misses pushed back one step further along their sequence.
 constexpr uint32_t DEFER_CAP = 128;  // entries per wave (>= 64 + 64)
 constexpr uint32_t DEFER_PSH = 13;   // entry state: slot | probes << 13 (CAP <= 8192)
+constexpr uint32_t DEFER_WSH = 20;   //   | weight << 20 (occurrences the entry stands for)
+
+// A fat record whose bases repeat with a period P (tandem repeats,
+// homopolymers) has only P distinct k-mer positions: k-mer j equals k-mer
+// j + P.  Its k-mers are counted as positions 0..P-1 with weights: position j
+// stands for the floor((nk - 1 - j) / P) + 1 k-mers j, j + P, ... of the record,
+// i.e. nk / P, plus one for the first nk % P positions.  Periods tried: 12, 10
+// and 8, which cover every period dividing one of them (1-6, 8, 10, 12); the
+// smallest of those three that holds is used.  bases: the record's base words
+// (first base in bits 31:30 of b[0]), L its length (<= 48).
+__device__ __forceinline__ uint32_t record_period(const uint32_t* b, uint32_t L)
+{
+    const uint64_t hi = ((uint64_t)b[0] << 32) | b[1];  // bases 0..31
+    const uint64_t lo = (uint64_t)b[2] << 32;            // bases 32..47 on top
+    uint32_t best = 0;
+#pragma unroll
+    for (uint32_t P = 12; P >= 8; P -= 2) {
+        if (P >= L) continue;
+        const uint32_t sh = 2u * P;
+        const uint64_t xh = hi ^ ((hi << sh) | (lo >> (64u - sh)));
+        const uint64_t xl = lo ^ (lo << sh);
+        const uint32_t n = 2u * (L - P);  // bits that must agree (from the top)
+        const bool ok = n <= 64u ? (xh >> (64u - n)) == 0ull
+                                 : xh == 0ull && (n == 64u || (xl >> (128u - n)) == 0ull);
+        if (ok) best = P;
+    }
+    return best;
+}
 
 template <uint32_t CAP, bool BATCH>
 __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, uint64_t* qk,
@@ -1558,17 +1664,20 @@ __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        uint32_t slot = e & ((1u << DEFER_PSH) - 1u), p = e >> DEFER_PSH;
+        uint32_t slot = e & ((1u << DEFER_PSH) - 1u), p = (e >> DEFER_PSH) & ((1u << (DEFER_WSH - DEFER_PSH)) - 1u);
+        const uint32_t w = e >> DEFER_WSH;
         bool again = false;
         if (has) {
             const uint64_t old = atomicCAS((unsigned long long*)&keys[slot], (unsigned long long)EMPTY64,
                                            (unsigned long long)x);
             if (old == EMPTY64 || old == x) {
-                if (old == x) atomicAdd(&cnt[slot], 1u);  // a claim is counted by the drain
+                // a claim is one occurrence, counted by the drain
+                const uint32_t add = old == x ? w : w - 1u;
+                if (add) atomicAdd(&cnt[slot], add);
             } else if (++p == a.probe_max) {
-                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}});
-                else spill_insert1(spill_ref(a), x);
-                ++spilled;
+                if (BATCH) spill_insert_w<2>(spill_ref(a), SpKey{{(uint64_t)tag, x, 0ull}}, w);
+                else spill_insert1(spill_ref(a), x, w);
+                spilled += w;
             } else {
                 again = true;
                 slot = (slot + p) & (CAP - 1u);  // triangular steps
@@ -1578,7 +1687,7 @@ __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, 
         if (again) {
             const uint32_t at = base + lane_rank(am);
             qk[at] = x;
-            qs[at] = slot | (p << DEFER_PSH);
+            qs[at] = slot | (p << DEFER_PSH) | (w << DEFER_WSH);
         }
         qn = base + (uint32_t)__popcll(am);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1626,6 +1735,7 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
     __shared__ uint32_t s_cnt[CAP];
     __shared__ uint32_t s_stage[NWAVE][count_stage<NW>()];
     __shared__ uint32_t s_incl[NWAVE][64];
+    __shared__ uint32_t s_wgt[NW == 1 ? NWAVE : 1][64];  // k <= 32: weights of a record's positions
     __shared__ uint64_t s_seed[NWAVE][64][2 * NW];  // first k-mer of each record: fwd, rc
     __shared__ uint64_t s_qk[NWAVE][NW == 1 ? DEFER_CAP : 1];  // deferred keys (k <= 32)
     __shared__ uint32_t s_qs[NWAVE][NW == 1 ? DEFER_CAP : 1];  // their probe state
@@ -1705,7 +1815,19 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
             // the item's records, compacted to the front of the wave's stage
             const uint32_t len = rr.w[0] & REC_FAT_LEN_MASK;
             const bool live = i < r1 && len >= k && ((rr.w[0] >> REC_HSEL_SHIFT) & smask) == sval;
-            const uint32_t nk = live ? len - k + 1u : 0u;
+            uint32_t nk = live ? len - k + 1u : 0u;
+            // k <= 32: a periodic record (longer than a superkmer of one
+            // minimizer occurrence can be, a.per_len) visits only its first P
+            // positions, weighted: wb = nk / P each, one more for the first
+            // nk % P (packed wb | rem << 16)
+            uint32_t wgt = 1u;
+            if (NW == 1 && live && len >= a.per_len) {
+                const uint32_t P = record_period(rr.w + 1, len);
+                if (P && P < nk) {
+                    wgt = (nk / P) | ((nk % P) << 16);
+                    nk = P;
+                }
+            }
             const uint64_t lm = __ballot(live);
             const uint32_t ci = lane_rank(lm), nlive = (uint32_t)__popcll(lm);
             const uint32_t incl = wave_incl_u32(nk, lane);
@@ -1714,6 +1836,7 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
 #pragma unroll
                 for (uint32_t q = 0; q < PW; ++q) s_stage[wv][ci * PW + q] = rr.w[1 + q];
                 s_incl[wv][ci] = incl;
+                if (NW == 1) s_wgt[NW == 1 ? wv : 0][ci] = wgt;
                 uint64_t fh, fl, rh, rl;  // the record's first k-mer, from registers
                 kmer_at<NW>(rr.w + 1, 0u, 0u, k, fh, fl, rh, rl);
                 s_seed[wv][ci][0] = fl;
@@ -1724,6 +1847,7 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                 }
             }
             if (lane >= nlive) s_incl[wv][lane] = total;  // keeps the search in range
+            const bool weighted = NW == 1 && __ballot(wgt != 1u) != 0ull;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1734,6 +1858,7 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                 // misses moves the key to the wave's stack
                 const uint32_t nmine = j1 > j0 ? j1 - j0 : 0u;
                 uint32_t s = 0, end = 0, pc = 1u;
+                uint32_t rpos = 0, wcur = 1u;  // position in the record, its weights (wb | rem << 16)
                 uint64_t fh = 0, fl = 0, rh = 0, rl = 0;
                 if (nmine) {
 #pragma unroll
@@ -1741,7 +1866,9 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                         if (s_incl[wv][s + step - 1u] <= j0) s += step;
                     const uint32_t ex = s ? s_incl[wv][s - 1u] : 0u;
                     end = s_incl[wv][s];
+                    wcur = s_wgt[NW == 1 ? wv : 0][s];
                     const uint32_t gb = j0 - ex;
+                    rpos = gb;
                     kmer_at<1>(stage, s * PW + (gb >> 4), gb & 15u, k, fh, fl, rh, rl);
                     pc = 2u * (s * (16u * PW) + gb + k) + 1u;
                 }
@@ -1749,17 +1876,28 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                 // A probe result is used one iteration after the probe is issued,
                 // so two probes per wave are in flight.  hit: count; miss: push
                 // the key with its next slot.
-                auto settle = [&](bool pa, uint64_t px, uint32_t ps, uint64_t po) {
+                // WT: the batch holds a periodic record, so a probe stands for a
+                // weight pw of occurrences (the loop without weights is kept for
+                // the common case: pw == 1, no weight bookkeeping)
+                auto settle = [&](auto WT, bool pa, uint64_t px, uint32_t ps, uint64_t po, uint32_t pw) {
                     const bool hit = pa && (po == EMPTY64 || po == px);
                     const bool miss = pa && !hit;
                     // the occurrence that claimed the slot is counted by the drain
-                    if (pa && po == px) atomicAdd(&s_cnt[ps], 1u);
+                    if constexpr (decltype(WT)::value) {
+                        if (hit) {
+                            const uint32_t add = po == px ? pw : pw - 1u;
+                            if (add) atomicAdd(&s_cnt[ps], add);
+                        }
+                    } else {
+                        if (pa && po == px) atomicAdd(&s_cnt[ps], 1u);
+                    }
                     const uint64_t mm = __ballot(miss);
                     if (mm) {
                         if (miss) {
                             const uint32_t at = qn + lane_rank(mm);
                             s_qk[wv][at] = px;
-                            s_qs[wv][at] = ((ps + 1u) & (CAP - 1u)) | (1u << DEFER_PSH);
+                            s_qs[wv][at] = ((ps + 1u) & (CAP - 1u)) | (1u << DEFER_PSH) |
+                                           ((decltype(WT)::value ? pw : 1u) << DEFER_WSH);
                         }
                         qn += (uint32_t)__popcll(mm);
                     }
@@ -1768,20 +1906,24 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                                                       spilled, lane);
                 };
                 // k-mer i: issue its probe, then roll the lane on to k-mer i+1
-                auto step = [&](uint32_t i, uint64_t& x, uint32_t& slot, uint64_t& old, bool& act) {
+                auto step = [&](auto WT, uint32_t i, uint64_t& x, uint32_t& slot, uint64_t& old, bool& act,
+                                uint32_t& w) {
                     act = i < nmine;
                     const uint32_t wn = stage[pc >> 5];
                     const bool xing = act && j0 + i + 1u == end && i + 1u < nmine;
-                    uint32_t nend;
+                    uint32_t nend, nw;
                     uint64_t nfl, nrl;  // read only when xing
                     if (xing) {
                         const uint32_t sn = min(s + 1u, 63u);
                         nend = s_incl[wv][sn];
                         nfl = s_seed[wv][sn][0];
                         nrl = s_seed[wv][sn][1];
+                        if constexpr (decltype(WT)::value) nw = s_wgt[NW == 1 ? wv : 0][sn];
                     }
                     x = fl < rl ? fl : rl;
                     slot = slot_of<1, CAP>(0ull, x);
+                    if constexpr (decltype(WT)::value) w = (wcur & 0xFFFFu) + (rpos < (wcur >> 16) ? 1u : 0u);
+                    else w = 1u;
                     // every lane probes; a lane without a k-mer compares against
                     // a value no slot holds and writes nothing
                     old = atomicCAS((unsigned long long*)&s_k0[slot],
@@ -1793,29 +1935,38 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                             end = nend;
                             fl = nfl;
                             rl = nrl;
+                            if constexpr (decltype(WT)::value) {
+                                wcur = nw;
+                                rpos = 0;
+                            }
                             pc = 2u * (s * (16u * PW) + k) + 1u;
                         } else {
                             pc += 2u;
+                            if constexpr (decltype(WT)::value) ++rpos;
                             roll_base<1>(fh, fl, rh, rl, b, kmask, rsh);
                         }
                     }
                 };
                 // two k-mers per trip with their own result registers, so a probe
                 // result crosses the loop edge without a copy (which would wait)
-                uint64_t xB = 0, oB = 0;
-                uint32_t sB = 0;
-                bool aB = false;
-                for (uint32_t i = 0; i < q; i += 2u) {
-                    uint64_t xA, oA;
-                    uint32_t sA;
-                    bool aA;
-                    step(i, xA, sA, oA, aA);
-                    settle(aB, xB, sB, oB);
-                    aB = false;
-                    if (i + 1u < q) step(i + 1u, xB, sB, oB, aB);
-                    settle(aA, xA, sA, oA);
-                }
-                settle(aB, xB, sB, oB);
+                auto run = [&](auto WT) {
+                    uint64_t xB = 0, oB = 0;
+                    uint32_t sB = 0, wB = 1;
+                    bool aB = false;
+                    for (uint32_t i = 0; i < q; i += 2u) {
+                        uint64_t xA, oA;
+                        uint32_t sA, wA;
+                        bool aA;
+                        step(WT, i, xA, sA, oA, aA, wA);
+                        settle(WT, aB, xB, sB, oB, wB);
+                        aB = false;
+                        if (i + 1u < q) step(WT, i + 1u, xB, sB, oB, aB, wB);
+                        settle(WT, aA, xA, sA, oA, wA);
+                    }
+                    settle(WT, aB, xB, sB, oB, wB);
+                };
+                if (weighted) run(std::true_type{});
+                else run(std::false_type{});
             } else if (j0 < j1) {
                 uint32_t s = 0;
 #pragma unroll
@@ -2151,13 +2302,19 @@ __global__ __launch_bounds__(256) void k_item_counts(const uint64_t* __restrict_
     if (b < nsp) nitem[b] = bucket_splits(part_km, part_off, b, 1u << xb, *scale_p);
 }
 
+// A bucket split into r <= 2^split_bits items: item q counts the records of
+// split groups [q, q+1) * 2^split_bits / r, one contiguous range (the fine
+// scatter orders a bucket's records by split group), with no filter.  More
+// splits than that: every item reads the whole bucket and keeps the records
+// whose low split bits equal q.
 __global__ __launch_bounds__(256) void k_items(const uint64_t* __restrict__ part_km,
                                                const uint64_t* __restrict__ part_off, uint32_t nsp,
                                                uint32_t xb, const uint32_t* __restrict__ bucket_job,
                                                const float* __restrict__ scale_p,
                                                const uint64_t* __restrict__ bucket_item,
                                                uint4* __restrict__ items, uint32_t* __restrict__ item_job,
-                                               uint32_t* __restrict__ n_items)
+                                               uint32_t* __restrict__ n_items,
+                                               const uint64_t* __restrict__ split_off, uint32_t split_bits)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) *n_items = (uint32_t)bucket_item[nsp];
@@ -2166,8 +2323,17 @@ __global__ __launch_bounds__(256) void k_items(const uint64_t* __restrict__ part
     const uint32_t lo = (uint32_t)part_off[b], hi = (uint32_t)part_off[b + 1];
     const uint32_t job = bucket_job[b];
     const uint64_t cur = bucket_item[b];
+    // (the fine scatter orders only coarse buckets holding a split bucket, so
+    // an unsplit bucket keeps its whole range)
+    const bool ranges = split_off && r > 1u && r <= (1u << split_bits);
+    const uint32_t per = ranges ? (1u << split_bits) / r : 0u;
     for (uint32_t q = 0; q < r; ++q) {
-        items[cur + q] = make_uint4(lo, hi, r - 1u, q);
+        if (ranges) {
+            const uint64_t g = ((uint64_t)b << split_bits) + (uint64_t)q * per;
+            items[cur + q] = make_uint4((uint32_t)split_off[g], (uint32_t)split_off[g + per], 0u, 0u);
+        } else {
+            items[cur + q] = make_uint4(lo, hi, r - 1u, q);
+        }
         item_job[cur + q] = job;
     }
 }
@@ -2175,13 +2341,14 @@ __global__ __launch_bounds__(256) void k_items(const uint64_t* __restrict__ part
 void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
                         uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
                         uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items,
-                        uint64_t* nitem, uint64_t* scan_tmp)
+                        uint64_t* nitem, uint64_t* scan_tmp, const uint64_t* split_off,
+                        uint32_t split_bits)
 {
     const unsigned blocks = (nsp + 255) / 256;
     k_item_counts<<<blocks, 256, 0, st>>>(part_km, part_off, nsp, xb, scale, nitem);
     launch_scan_u64(st, nitem, nsp, bucket_item, scan_tmp);
     k_items<<<blocks, 256, 0, st>>>(part_km, part_off, nsp, xb, bucket_job, scale, bucket_item, items,
-                                    item_job, n_items);
+                                    item_job, n_items, split_off, split_bits);
 }
 
 // Spill entries per job: distinct keys, and reported keys (ci/cx filter).

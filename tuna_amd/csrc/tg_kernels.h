@@ -92,7 +92,24 @@ struct RouteArgs {
     uint32_t max_tiles;
     uint64_t rec_cap;
     uint32_t* err;
+    // scatter: the meta word of every coarse-ordered record (read by the fine
+    // pass to size its groups before moving records); the fine pass orders a
+    // sub-partition's records by the top split_bits of their split field and,
+    // when split_off is set, writes the start of every (sub-partition, split
+    // group): nsp << split_bits + 1 entries (see route_split_bits)
+    uint32_t* tmp_meta;
+    uint64_t* split_off;
+    uint32_t split_bits;
+    // which coarse buckets are split-ordered: those holding a bucket whose
+    // k-mers x *split_scale exceed 1 (the work items' split rule); nullptr: none
+    const uint64_t* split_km;
+    const float* split_scale;
+    uint32_t* split_flags;  // per coarse bucket (256 words): 1 = split-ordered
 };
+
+// Split-field bits the fine scatter orders records by (at most 512 groups
+// per coarse bucket).
+uint32_t route_split_bits(uint32_t nsp);
 
 // Phase-2 work item: records [x, y) of consecutive sub-partitions; records with
 // (split bits & z) == w are counted by this item.
@@ -104,6 +121,10 @@ struct CountArgs {
     uint32_t ci;
     uint64_t cx;
     uint32_t probe_max;
+    // k <= 32: records of at least per_len bases (longer than a superkmer of
+    // one minimizer occurrence, k + W; 0 = never) are tested for a short period
+    // and then counted with weights (a tandem repeat's k-mers recur every P)
+    uint32_t per_len;
     // spill table (global, shared by all items; keys are disjoint by partition)
     uint64_t* sp_keys;         // NW words per slot
     uint32_t* sp_cnt;
@@ -185,10 +206,13 @@ void launch_batch_setup(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_
                         uint32_t nsp, uint32_t* err);
 void set_kernel_attributes();
 // nitem: nsp per-bucket item counts; scan_tmp >= scan_tmp_words(nsp)
+// split_off / split_bits: the fine scatter's group starts; a bucket split into
+// at most 2^split_bits items gets contiguous record ranges (no filter).
 void launch_build_items(hipStream_t st, const uint64_t* part_km, const uint64_t* part_off,
                         uint32_t nsp, uint32_t xb, const uint32_t* bucket_job, const float* scale,
                         uint4* items, uint32_t* item_job, uint64_t* bucket_item, uint32_t* n_items,
-                        uint64_t* nitem, uint64_t* scan_tmp);
+                        uint64_t* nitem, uint64_t* scan_tmp, const uint64_t* split_off,
+                        uint32_t split_bits);
 void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks);
 // spill entries of a batch: per-job distinct and reported counts, then placement
 void launch_spill_jobs(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
