@@ -413,9 +413,9 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
         tg::launch_job_totals(st, b.desc, jobs, J, b.bucket_item, b.seg_x, b.job_acc + 3ull * J,
                               b.job_acc + J, b.nkm_x, b.job_tot, b.job_item_tot, b.job_nsk, b.job_nkm);
         tg::launch_scan_u64(st, b.job_tot, J, b.job_off, b.scan_tmp);
-        tg::launch_gather_items(st, ca, c.nw, jobs, b.bucket_item, b.seg_x, b.job_off, b.fin_keys,
+        tg::launch_gather_items(st, ca, c.nw, jobs, b.desc, b.bucket_item, b.seg_x, b.job_off, b.fin_keys,
                                 b.fin_cnt, GATHER_BLOCKS);
-        tg::launch_spill_place(st, ca, c.nw, c.sp_slots, b.job_off, b.job_item_tot,
+        tg::launch_spill_place(st, ca, b.desc, c.nw, c.sp_slots, b.job_off, b.job_item_tot,
                                b.job_acc + 2ull * J, b.fin_keys, b.fin_cnt);
         tg::launch_pack_back(st, b.desc, J, b.job_off, b.job_nsk, b.job_nkm, b.job_acc, b.flags, F_WORDS,
                              b.counters, b.back);
@@ -760,17 +760,20 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
                 continue;
             }
             const uint64_t n_out = h_off[nj];
+            // a batch of one job is not gathered: its result is the items' output
+            uint64_t* res_keys = nj == 1 ? s->b.out_keys : s->b.fin_keys;
+            uint32_t* res_cnt = nj == 1 ? s->b.out_cnt : s->b.fin_cnt;
             if (view) {
-                out.view_keys = s->b.fin_keys;
-                out.view_cnt = s->b.fin_cnt;
+                out.view_keys = res_keys;
+                out.view_cnt = res_cnt;
                 out.nw = nw;
             } else if ((rc = grow_store(out, st, out.n + n_out, nw))) {
                 return rc;
             }
             if (n_out && !view) {
-                HIPOK(hipMemcpyAsync(out.keys + out.n * nw, s->b.fin_keys, n_out * nw * 8ull,
+                HIPOK(hipMemcpyAsync(out.keys + out.n * nw, res_keys, n_out * nw * 8ull,
                                      hipMemcpyDeviceToDevice, st));
-                HIPOK(hipMemcpyAsync(out.cnt + out.n, s->b.fin_cnt, n_out * 4ull,
+                HIPOK(hipMemcpyAsync(out.cnt + out.n, res_cnt, n_out * 4ull,
                                      hipMemcpyDeviceToDevice, st));
             }
             uint64_t bk = 0, bu = 0;

@@ -2420,13 +2420,16 @@ void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t
 // Item segments into job order: job j's items first (item order), its spill
 // entries after them.
 template <int NW>
-__global__ __launch_bounds__(256) void k_gather_items(CountArgs a, BatchJobs j,
+__global__ __launch_bounds__(256) void k_gather_items(CountArgs a, BatchJobs j, const BatchDesc* __restrict__ d,
                                                       const uint64_t* __restrict__ bucket_item,
                                                       const uint64_t* __restrict__ seg_x,
                                                       const uint64_t* __restrict__ job_off,
                                                       uint64_t* __restrict__ dst_keys,
                                                       uint32_t* __restrict__ dst_cnt)
 {
+    // one job: its items' reservations already tile out_keys from 0, which
+    // is then the result (the spill entries follow them)
+    if (d->n_jobs == 1) return;
     const uint32_t n = *a.n_items_dev;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t job = a.item_job[i];
@@ -2445,15 +2448,15 @@ __global__ __launch_bounds__(256) void k_gather_items(CountArgs a, BatchJobs j,
     }
 }
 
-void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j,
+void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j, const BatchDesc* d,
                          const uint64_t* bucket_item, const uint64_t* seg_x,
                          const uint64_t* job_off, uint64_t* dst_keys, uint32_t* dst_cnt,
                          uint32_t blocks)
 {
     if (nw == 1)
-        k_gather_items<1><<<blocks, 256, 0, st>>>(a, j, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
+        k_gather_items<1><<<blocks, 256, 0, st>>>(a, j, d, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
     else
-        k_gather_items<2><<<blocks, 256, 0, st>>>(a, j, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
+        k_gather_items<2><<<blocks, 256, 0, st>>>(a, j, d, bucket_item, seg_x, job_off, dst_keys, dst_cnt);
 }
 
 __global__ __launch_bounds__(256) void k_pack_back(const BatchDesc* __restrict__ d, uint32_t cap_jobs,
@@ -2494,7 +2497,8 @@ void launch_pack_back(hipStream_t st, const BatchDesc* d, uint32_t cap_jobs, con
 }
 
 template <int SW>
-__global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slots,
+__global__ __launch_bounds__(256) void k_spill_place(CountArgs a, const BatchDesc* __restrict__ d,
+                                                     uint64_t n_slots,
                                                      const uint64_t* __restrict__ job_off,
                                                      const uint64_t* __restrict__ job_item_tot,
                                                      unsigned long long* __restrict__ job_cur,
@@ -2502,6 +2506,10 @@ __global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slo
                                                      uint32_t* __restrict__ dst_cnt)
 {
     if (a.counters[2] == 0) return;  // no k-mer reached the spill table: it is empty
+    if (d->n_jobs == 1) {  // one job: the result is out_keys (no gather)
+        dst_keys = a.out_keys;
+        dst_cnt = a.out_cnt;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < n_slots; s += stride) {
         const uint32_t c = a.sp_cnt[s];
@@ -2518,7 +2526,7 @@ __global__ __launch_bounds__(256) void k_spill_place(CountArgs a, uint64_t n_slo
     }
 }
 
-void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+void launch_spill_place(hipStream_t st, const CountArgs& a, const BatchDesc* d, uint32_t nw, uint64_t n_slots,
                         const uint64_t* job_off, const uint64_t* job_item_tot,
                         unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt)
 {
@@ -2526,10 +2534,10 @@ void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_
     if (blocks > 4096) blocks = 4096;
     if (!blocks) return;
     if (nw == 1)
-        k_spill_place<2><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_off, job_item_tot, job_cur,
+        k_spill_place<2><<<(unsigned)blocks, 256, 0, st>>>(a, d, n_slots, job_off, job_item_tot, job_cur,
                                                            dst_keys, dst_cnt);
     else
-        k_spill_place<3><<<(unsigned)blocks, 256, 0, st>>>(a, n_slots, job_off, job_item_tot, job_cur,
+        k_spill_place<3><<<(unsigned)blocks, 256, 0, st>>>(a, d, n_slots, job_off, job_item_tot, job_cur,
                                                            dst_keys, dst_cnt);
 }
 

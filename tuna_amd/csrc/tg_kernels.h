@@ -229,11 +229,14 @@ void launch_job_totals(hipStream_t st, const BatchDesc* d, BatchJobs j, uint32_t
                        const unsigned long long* job_sk, const unsigned long long* job_sp,
                        const uint64_t* nkm_x, uint64_t* job_tot, uint64_t* job_item_tot,
                        uint64_t* job_nsk, uint64_t* job_nkm);
-void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j,
+// A batch of one job (d->n_jobs == 1) is not gathered: its result is a.out_keys
+// (the items' reservations tile it from 0) with the spill entries placed after
+// them.
+void launch_gather_items(hipStream_t st, const CountArgs& a, uint32_t nw, BatchJobs j, const BatchDesc* d,
                          const uint64_t* bucket_item, const uint64_t* seg_x,
                          const uint64_t* job_off, uint64_t* dst_keys, uint32_t* dst_cnt,
                          uint32_t blocks);
-void launch_spill_place(hipStream_t st, const CountArgs& a, uint32_t nw, uint64_t n_slots,
+void launch_spill_place(hipStream_t st, const CountArgs& a, const BatchDesc* d, uint32_t nw, uint64_t n_slots,
                         const uint64_t* job_off, const uint64_t* job_item_tot,
                         unsigned long long* job_cur, uint64_t* dst_keys, uint32_t* dst_cnt);
 
