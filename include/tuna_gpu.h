@@ -308,6 +308,14 @@ void tg_unmap_file(const uint8_t* data, uint64_t len);
  * rounded to a power of two (src/main.cpp:47-52 does that). */
 uint32_t tg_auto_partitions(const char* const* files, uint32_t n_files, uint32_t k,
                             uint32_t threads);
+/* The same formula with every host-dependent input given: the input files'
+ * sizes in bytes and gzip flags (a gzip file counts 6x its size), the L2 cache
+ * of one core and the L3 cache in bytes (the reference reads them from sysfs
+ * cpu0/cache, src/setup.hpp:14-51) and the open-file limit (RLIMIT_NOFILE soft
+ * limit; the reference uses 1024 when it is infinite or unknown). */
+uint32_t tg_auto_partitions_with(const uint64_t* sizes, const uint8_t* gz, uint32_t n_files, uint32_t k,
+                                 uint32_t threads, uint64_t l2_bytes, uint64_t l3_bytes,
+                                 uint64_t fd_limit);
 
 /* ---- output formats (A10) ---------------------------------------------- */
 /* TSV lines "<k-mer>\t<count>\n" (src/count.hpp:180-216) for n pairs; returns

@@ -398,3 +398,26 @@ def test_cpp_api_count_to_and_raw(tmp_path):
         a, b = line.split("\t")
         got[a] = int(b)
     assert got == oracle_tsv([bytes(text)], 31, 2)
+
+
+@pytest.mark.gpu
+def test_cli_partition_count_rounding_and_auto(tmp_path):
+    # -n is rounded up to a power of two (src/main.cpp:47-52); without -n the
+    # auto-tuned count of this host is used; counts are the same either way
+    text = T.synth("kind=reads_fq genome=20000 seed=9 cov=8 rl=150 sub=0.005")
+    fq = tmp_path / "r.fq"
+    fq.write_bytes(bytes(text))
+    exp = oracle_tsv([bytes(text)], 31, 1)
+    r = run_cli("-n", "100", str(fq), str(tmp_path / "a.tsv"))
+    assert r.returncode == 0, r.stderr
+    stats = dict(l.split(": ") for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+    assert int(stats["n_parts"]) == 128 and "n=128" in r.stderr
+    assert read_tsv(tmp_path / "a.tsv") == exp
+    arr = (T.ctypes.c_char_p * 1)(str(fq).encode())
+    T.lib().tg_auto_partitions.restype = T.ctypes.c_uint32
+    auto = O.round_pow2(T.lib().tg_auto_partitions(arr, 1, 31, 1))
+    r = run_cli(str(fq), str(tmp_path / "b.tsv"))
+    assert r.returncode == 0, r.stderr
+    stats = dict(l.split(": ") for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+    assert int(stats["n_parts"]) == auto
+    assert read_tsv(tmp_path / "b.tsv") == exp

@@ -144,3 +144,53 @@ def test_synth_is_deterministic_and_parses():
     g = T.synth("kind=genome_fa len=10000 seed=1")
     bases, off = T.parse_buffer(g)
     assert len(off) == 2 and int(off[-1]) == 10000
+
+
+def auto_n(sizes, gz, k, threads, l2, l3, fd):
+    L = T.lib()
+    L.tg_auto_partitions_with.restype = T.ctypes.c_uint32
+    L.tg_auto_partitions_with.argtypes = [T._u64p, T._u8p, T.ctypes.c_uint32, T.ctypes.c_uint32,
+                                          T.ctypes.c_uint32, T.ctypes.c_uint64, T.ctypes.c_uint64,
+                                          T.ctypes.c_uint64]
+    s = np.array(sizes, dtype=np.uint64)
+    g = np.array(gz, dtype=np.uint8)
+    return L.tg_auto_partitions_with(s.ctypes.data_as(T._u64p), g.ctypes.data_as(T._u8p), len(sizes), k,
+                                     threads, l2, l3, fd)
+
+
+# Host shapes with the reference formula (src/setup.hpp:82-196) worked by hand:
+#   max_parts = fd - 32;  writer_cap = largest 4096 * 2^i <= l2 / 32, at most 32768
+#   est = 76% of the bytes (the largest file's when several files average < 50 MB;
+#         gzip counts 6x);  per_kmer = (8 ceil(k/32) + 6) * 5 / 4 rounded up
+#   n_l3 = pow2 >= ceil(t * est * per_kmer / (3/4 l3)), at least 16
+#   n_tp = pow2 >= max(1, total >> 21), at least 16;  n = min(max(n_l3, n_tp), caps), >= 16
+@pytest.mark.parametrize("case,expect", [
+    # 4,660,000 B FASTA, k=31, t=1, L2 1 MiB, L3 32 MiB, fd 1024:
+    #   est = 3,541,600; per_kmer = (14*5+3)/4 = 18; 3,541,600*18 = 63,748,800
+    #   / 25,165,824 -> 3 -> n_l3 16; 4,660,000 >> 21 = 2 -> n_tp 16; n = 16
+    ((([4660000], [0]), 31, 1, 1 << 20, 32 << 20, 1024), 16),
+    # 4,188,888,785 B FASTQ, k=31, t=16, L2 512 KiB, L3 256 MiB, fd 1024:
+    #   est = 3,183,555,476; 16*est*18 = 916,863,977,088 / 201,326,592 = 4554.1 -> 4555
+    #   -> n_l3 8192; total >> 21 = 1997 -> n_tp 2048; n = 8192; caps: fd 992,
+    #   writer 16384 (524288/32 = 16384) -> 992
+    ((([4188888785], [0]), 31, 16, 512 << 10, 256 << 20, 1024), 992),
+    # the same with fd 65536: caps 65504 and 16384 -> 8192
+    ((([4188888785], [0]), 31, 16, 512 << 10, 256 << 20, 65536), 8192),
+    # the same with L2 2 MiB (writer cap 32768 at most) and t = 64:
+    #   64*est*18 = 3,667,455,908,352 / 201,326,592 = 18216.4 -> 32768; caps 32768 -> 32768
+    ((([4188888785], [0]), 31, 64, 2 << 20, 256 << 20, 1 << 20), 32768),
+    # 200 files of 5,000,000 B (average < 50 MB: the largest counts), k=31, t=8,
+    #   L2 1 MiB, L3 32 MiB, fd 4096: est = 3,800,000; 8*3,800,000*18 = 547,200,000
+    #   / 25,165,824 = 21.7 -> 22 -> n_l3 32; total 1e9 >> 21 = 476 -> n_tp 512; n = 512
+    ((([5000000] * 200, [0] * 200), 31, 8, 1 << 20, 32 << 20, 4096), 512),
+    # one 100,000,000 B gzip file (600,000,000 B counted), k=63, t=4, L2 256 KiB,
+    #   L3 64 MiB, fd 2048: per_kmer = (22*5+3)/4 = 28; est = 456,000,000;
+    #   4*456,000,000*28 = 51,072,000,000 / 50,331,648 = 1014.7 -> 1015 -> n_l3 1024;
+    #   600,000,000 >> 21 = 286 -> n_tp 512; n = 1024; caps 2016, writer 8192 -> 1024
+    ((([100000000], [1]), 63, 4, 256 << 10, 64 << 20, 2048), 1024),
+    # tiny input, fd 40 (max_parts 8) -> min(16, 8) = 8, floor 16 -> 16
+    ((([1000], [0]), 21, 1, 1 << 20, 32 << 20, 40), 16),
+])
+def test_auto_partitions_formula_pinned(case, expect):
+    (sizes, gz), k, t, l2, l3, fd = case
+    assert auto_n(sizes, gz, k, t, l2, l3, fd) == expect

@@ -29,7 +29,7 @@ EXPORTS = [
     "tg_reset_kernel_times", "tg_dev_alloc", "tg_dev_free", "tg_h2d", "tg_d2h", "tg_sync",
     "tg_count_host", "tg_count_device", "tg_result_size", "tg_result_copy", "tg_pack_host",
     "tg_superkmers_host", "tg_partitions_host", "tg_parse_buffer", "tg_read_file", "tg_free",
-    "tg_auto_partitions", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
+    "tg_auto_partitions", "tg_auto_partitions_with", "tg_format_tsv", "tg_write_kff", "tg_synth", "tg_synth_copy",
     "tg_synth_free", "tg_get_batch_config", "tg_set_batch_config", "tg_count_many",
     "tg_count_many_device", "tg_many_result_size", "tg_many_result_copy", "tg_many_result_device",
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",

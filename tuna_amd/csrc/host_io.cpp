@@ -772,46 +772,58 @@ int tg_read_file(const char* path, uint8_t** data, uint64_t* len)
 void tg_free(void* p) { free(p); }
 
 // src/setup.hpp:82-196, restated: n = max(n_L3, n_throughput) capped by the
-// fd limit and the writer-cache cap, floor 16.
-uint32_t tg_auto_partitions(const char* const* files, uint32_t n_files, uint32_t k, uint32_t threads)
+// fd limit and the writer-cache cap, floor 16.  Every host-dependent input is
+// an argument here; tg_auto_partitions reads them from this host.
+uint32_t tg_auto_partitions_with(const uint64_t* sizes, const uint8_t* gz, uint32_t n_files, uint32_t k,
+                                 uint32_t threads, uint64_t l2_bytes, uint64_t l3_bytes, uint64_t fd_limit)
 {
-    struct rlimit rl {};
-    size_t fd_limit = 1024;
-    if (getrlimit(RLIMIT_NOFILE, &rl) == 0 && rl.rlim_cur != RLIM_INFINITY) fd_limit = rl.rlim_cur;
-    const size_t max_parts = fd_limit > 32 ? fd_limit - 32 : 16;
-    const size_t writer_bytes = 32;  // sizeof the reference's SuperkmerWriter
-    const size_t ratio = cache_size(2, 256u << 10) / writer_bytes;
-    size_t writer_cap = 4096;
+    const uint64_t max_parts = fd_limit > 32 ? fd_limit - 32 : 16;
+    const uint64_t writer_bytes = 32;  // sizeof the reference's SuperkmerWriter
+    const uint64_t ratio = l2_bytes / writer_bytes;
+    uint64_t writer_cap = 4096;
     while ((writer_cap << 1) <= ratio) writer_cap <<= 1;
-    writer_cap = std::min(writer_cap, (size_t)32768);
+    writer_cap = std::min<uint64_t>(writer_cap, 32768);
     uint64_t total = 0, max_single = 0;
     for (uint32_t i = 0; i < n_files; ++i) {
-        struct stat st {};
-        if (stat(files[i], &st) != 0) continue;
-        uint64_t sz = (uint64_t)st.st_size;
-        const std::string f(files[i]);
-        if (f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0) sz *= 6;
+        const uint64_t sz = sizes[i] * (gz && gz[i] ? 6u : 1u);
         total += sz;
         max_single = std::max(max_single, sz);
     }
     uint64_t est_bytes = total;
     if (n_files > 1 && total / n_files < (50ull << 20)) est_bytes = max_single;
     const uint64_t est_unique = est_bytes * 76 / 100;
-    const size_t words = (k + 31) / 32;
-    const size_t per_slot = words * 8 + 4 + 2;
-    const size_t per_kmer = (per_slot * 5 + 3) / 4;
-    const size_t l3_usable = cache_size(3, 30u << 20) * 3 / 4;
+    const uint64_t words = (k + 31) / 32;
+    const uint64_t per_slot = words * 8 + 4 + 2;
+    const uint64_t per_kmer = (per_slot * 5 + 3) / 4;
+    const uint64_t l3_usable = std::max<uint64_t>(l3_bytes * 3 / 4, 1);
     const uint64_t num = (uint64_t)std::max<uint32_t>(threads, 1) * est_unique * per_kmer;
-    const size_t n_l3_raw = (size_t)((num + l3_usable - 1) / l3_usable);
-    size_t n_l3 = 16;
+    const uint64_t n_l3_raw = (num + l3_usable - 1) / l3_usable;
+    uint64_t n_l3 = 16;
     while (n_l3 < n_l3_raw) n_l3 <<= 1;
-    const size_t n_tp_raw = (size_t)std::max<uint64_t>(1, total >> 21);
-    size_t n_tp = 16;
+    const uint64_t n_tp_raw = std::max<uint64_t>(1, total >> 21);
+    uint64_t n_tp = 16;
     while (n_tp < n_tp_raw) n_tp <<= 1;
-    size_t n = std::max(n_l3, n_tp);
+    uint64_t n = std::max(n_l3, n_tp);
     n = std::min(n, std::min(max_parts, writer_cap));
-    n = std::max(n, (size_t)16);
+    n = std::max<uint64_t>(n, 16);
     return (uint32_t)n;
+}
+
+uint32_t tg_auto_partitions(const char* const* files, uint32_t n_files, uint32_t k, uint32_t threads)
+{
+    struct rlimit rl {};
+    uint64_t fd_limit = 1024;  // also when the limit is infinite, as the reference does
+    if (getrlimit(RLIMIT_NOFILE, &rl) == 0 && rl.rlim_cur != RLIM_INFINITY) fd_limit = rl.rlim_cur;
+    std::vector<uint64_t> sizes(n_files, 0);
+    std::vector<uint8_t> gz(n_files, 0);
+    for (uint32_t i = 0; i < n_files; ++i) {
+        struct stat st {};
+        const std::string f(files[i]);
+        gz[i] = f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
+        if (stat(files[i], &st) == 0) sizes[i] = (uint64_t)st.st_size;  // a missing file adds nothing
+    }
+    return tg_auto_partitions_with(sizes.data(), gz.data(), n_files, k, threads, cache_size(2, 256u << 10),
+                                   cache_size(3, 30u << 20), fd_limit);
 }
 
 int64_t tg_format_tsv(uint32_t k, const uint64_t* keys, const uint32_t* counts, uint64_t n, char* out,
