@@ -468,38 +468,78 @@ def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
     return out
 
 
-def bench_end_to_end(name, text, threads):
-    """The tuna CLI on the workload's FASTQ file: read (mapped), parse on the
-    host threads, upload, count, format TSV on the device and write it; wall
-    time of the whole process, best of two runs (page cache warm)."""
+def _bgzf(text, threads, block=65280, level=1):
+    """BGZF members (SAM/BAM specification 4.1) of `text`, compressed on
+    `threads` threads, then the empty end-of-file member."""
+    import struct
+    import zlib
+    from concurrent.futures import ThreadPoolExecutor
+    mv = memoryview(text)
+
+    def one(i):
+        blk = mv[i:i + block]
+        c = zlib.compressobj(level, zlib.DEFLATED, -15)
+        body = c.compress(blk) + c.flush()
+        head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff" + struct.pack("<H", 6) + b"BC" + \
+            struct.pack("<HH", 2, 18 + len(body) + 8 - 1)
+        return head + body + struct.pack("<II", zlib.crc32(blk) & 0xFFFFFFFF, len(blk))
+
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(one, range(0, len(text), block)))
+    parts.append(bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000"))
+    return b"".join(parts)
+
+
+def bench_end_to_end(name, text, threads, formats=("plain",)):
+    """The tuna CLI on the workload's FASTQ file, per input format: plain
+    (mapped, parsed on the host threads), gzip (inflated on one thread while
+    the parse and the upload follow it, as the reference reads gzip input on
+    one decompressor thread) and BGZF (members inflated on all host threads);
+    then count, format TSV on the device and write it.  Wall time of the whole
+    process, best of two runs (page cache warm)."""
+    import gzip as gz
     wl = WORKLOADS[name]
     cli = os.path.join(ROOT, "tuna_amd", "tuna")
-    need = 2 * len(text) + (1 << 30)
+    need = 3 * len(text) + (1 << 30)
     base = "/dev/shm" if os.path.isdir("/dev/shm") and shutil.disk_usage("/dev/shm").free > 4 * need else None
     d = tempfile.mkdtemp(prefix="tg_e2e_", dir=base)
+    res = {}
     try:
-        fq = os.path.join(d, "reads.fq")
-        with open(fq, "wb") as f:
-            f.write(memoryview(text))
-        out = os.path.join(d, "out.tsv")
-        best = None
-        for _ in range(2):
+        for fmt in formats:
+            fq = os.path.join(d, {"plain": "reads.fq", "gzip": "reads.fq.gz", "bgzf": "reads.bgzf.fq.gz"}[fmt])
             t0 = time.perf_counter()
-            r = subprocess.run([cli, "-k", str(wl["k"]), "-m", str(wl["m"]), "-n", str(wl["n"]), "-ci", str(wl["ci"]),
-                                "-t", str(threads), "-hp", fq, out], capture_output=True, text=True, timeout=600)
-            wall = time.perf_counter() - t0
-            if r.returncode != 0:
-                return {"error": r.stderr[-300:]}
-            st = dict(l.split(": ", 1) for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
-            size = os.path.getsize(out)
-            if best is None or wall < best["wall_s"]:
-                best = {"wall_s": round(wall, 3), "phase1_s": float(st.get("phase1", "0").rstrip("s")),
-                        "phase2_s": float(st.get("phase2", "0").rstrip("s")), "unique_kmers": int(st.get("unique_kmers", 0)),
-                        "tsv_bytes": size}
-            os.remove(out)
-        return dict(best, fastq_bytes=len(text), threads=threads, output_dir=("tmpfs" if base else "tempdir"),
-                    command=f"tuna -k {wl['k']} -m {wl['m']} -n {wl['n']} -ci {wl['ci']} -t {threads} -hp "
-                            f"reads.fq out.tsv")
+            with open(fq, "wb") as f:
+                if fmt == "plain":
+                    f.write(memoryview(text))
+                elif fmt == "gzip":
+                    f.write(gz.compress(memoryview(text), 1))
+                else:
+                    f.write(_bgzf(text, threads))
+            log(f"{name}: wrote {fmt} input, {os.path.getsize(fq)} bytes ({time.perf_counter() - t0:.1f} s)")
+            out = os.path.join(d, "out.tsv")
+            best = None
+            for _ in range(2):
+                t0 = time.perf_counter()
+                r = subprocess.run([cli, "-k", str(wl["k"]), "-m", str(wl["m"]), "-n", str(wl["n"]), "-ci",
+                                    str(wl["ci"]), "-t", str(threads), "-hp", fq, out],
+                                   capture_output=True, text=True, timeout=600)
+                wall = time.perf_counter() - t0
+                if r.returncode != 0:
+                    best = {"error": r.stderr[-300:]}
+                    break
+                st = dict(l.split(": ", 1) for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+                size = os.path.getsize(out)
+                if best is None or wall < best["wall_s"]:
+                    best = {"wall_s": round(wall, 3), "phase1_s": float(st.get("phase1", "0").rstrip("s")),
+                            "phase2_s": float(st.get("phase2", "0").rstrip("s")),
+                            "unique_kmers": int(st.get("unique_kmers", 0)), "tsv_bytes": size}
+                os.remove(out)
+            res[fmt] = dict(best, input_bytes=os.path.getsize(fq),
+                            command=f"tuna -k {wl['k']} -m {wl['m']} -n {wl['n']} -ci {wl['ci']} -t {threads} -hp "
+                                    f"{os.path.basename(fq)} out.tsv")
+            os.remove(fq)
+            log(f"{name}: end to end, {fmt}: {res[fmt]}")
+        return dict(res, fastq_bytes=len(text), threads=threads, output_dir=("tmpfs" if base else "tempdir"))
     finally:
         shutil.rmtree(d, ignore_errors=True)
 
@@ -596,12 +636,13 @@ def main():
     if texts:
         e2e = {}
         for name, text in texts.items():
-            r = bench_end_to_end(name, text, threads)
+            fmts = ("plain", "gzip", "bgzf") if name == "short_reads" else ("plain",)
+            r = bench_end_to_end(name, text, threads, fmts)
             N = legs[name]["config"]["bases"]
-            if "wall_s" in r:
-                r["gbases_per_s"] = round(N / r["wall_s"] / 1e9, 3)
+            for fmt in fmts:
+                if "wall_s" in r.get(fmt, {}):
+                    r[fmt]["gbases_per_s"] = round(N / r[fmt]["wall_s"] / 1e9, 3)
             e2e[name] = r
-            log(f"{name}: end to end {r}")
         out["end_to_end"] = e2e
         texts.clear()
     if args.samples > 0 and args.only == "all":

@@ -301,28 +301,6 @@ inline Device& device()
     return d;
 }
 
-struct Chunks {
-    std::vector<uint8_t> bases;
-    std::vector<uint64_t> off{0};
-};
-
-inline void read_inputs(const std::vector<std::string>& files, Chunks& out)
-{
-    for (const auto& f : files) {
-        uint8_t* data = nullptr;
-        uint64_t len = 0;
-        check(tg_read_file(f.c_str(), &data, &len));
-        std::unique_ptr<uint8_t, void (*)(void*)> hold(data, tg_free);
-        std::vector<uint8_t> b(len);
-        std::vector<uint64_t> o(len / 2 + 2);
-        const int64_t n = tg_parse_buffer(data, len, b.data(), o.data(), o.size());
-        if (n < 0) throw std::runtime_error("cannot parse input: " + f);
-        const uint64_t base = out.bases.size();
-        out.bases.insert(out.bases.end(), b.begin(), b.begin() + (std::ptrdiff_t)o[n]);
-        for (int64_t i = 1; i <= n; ++i) out.off.push_back(base + o[i]);
-    }
-}
-
 template <uint16_t k, uint16_t m>
 inline void run(const std::vector<std::string>& files, const Options& opts,
                 std::vector<uint64_t>& keys, std::vector<uint32_t>& counts)
@@ -330,8 +308,6 @@ inline void run(const std::vector<std::string>& files, const Options& opts,
     static_assert(k >= 2 && k <= 64, "tuna (gfx950 engine): k must be in [2, 64]");
     static_assert(m >= 1 && m < k && m <= 32, "tuna: m must be in [1, min(k-1, 32)]");
     if (files.empty()) throw std::invalid_argument("tuna::count: no input files");
-    Chunks ch;
-    read_inputs(files, ch);
     uint32_t n = opts.partitions;
     if (n == 0) {
         std::vector<const char*> fp;
@@ -347,8 +323,10 @@ inline void run(const std::vector<std::string>& files, const Options& opts,
         d.ctx.reset(made);
     }
     tg_ctx* c = d.ctx.get();
-    check(tg_count_host(c, &p, ch.bases.data(), ch.bases.size(), ch.off.data(), ch.off.size() - 1,
-                        nullptr));
+    std::vector<const char*> paths;
+    for (const auto& f : files) paths.push_back(f.c_str());
+    // the files are read straight to the device, the upload overlapping the read
+    check(tg_count_files(c, &p, paths.data(), (uint32_t)paths.size(), opts.threads ? opts.threads : 1, nullptr));
     const uint64_t nout = tg_result_size(c);
     keys.resize(nout * ((k + 31) / 32));
     counts.resize(nout);

@@ -94,6 +94,17 @@ int tg_sync(tg_ctx* ctx);
 int tg_count_host(tg_ctx* ctx, const tg_params* p, const uint8_t* bases,
                   uint64_t n_bases, const uint64_t* off, uint64_t n_chunks,
                   tg_stats* st);
+/* Count the canonical k-mers of FASTA/FASTQ files (gzip when a name ends in
+ * ".gz"; BGZF files are inflated on `threads` threads, other gzip files on one
+ * thread while the calling thread parses what has arrived).  Each file's bases
+ * go to the device as they are parsed, so reading and the upload overlap; the
+ * result equals tg_count_host on the files' chunks concatenated in order.
+ * Replaces the reference's run over its input files (src/pipeline.hpp:64-178,
+ * reading through src/seq_source.hpp:190-205).  Input errors carry the
+ * reference's messages ("Cannot open file: <path>", "Empty file: <path>",
+ * "Cannot open: <path>", "Cannot read: <path>"). */
+int tg_count_files(tg_ctx* ctx, const tg_params* p, const char* const* paths, uint32_t n_files,
+                   uint32_t threads, tg_stats* st);
 /* Same with bases/off already resident in device memory (d_bases, d_off). */
 int tg_count_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases,
                     uint64_t n_bases, const uint64_t* d_off, uint64_t n_chunks,
@@ -298,6 +309,15 @@ int64_t tg_parse_buffer_mt(const uint8_t* data, uint64_t len, uint8_t* out_bases
  * reference messages: "Cannot open file: <path>", "Empty file: <path>". */
 int tg_read_file(const char* path, uint8_t** data, uint64_t* len);
 void tg_free(void* p);
+/* Parse one input file the way tg_count_files reads it: gzip inflation
+ * overlapped with a parse in sections of `section_bytes` of text (0: 32 MiB),
+ * BGZF members inflated on `threads` threads, plain files mapped and parsed
+ * on `threads` threads.  *bases (n_bases bytes) and *off (the returned chunk
+ * count + 1 entries) are malloc'd; release both with tg_free.  The output
+ * equals tg_parse_buffer on the file's whole text.  Returns the chunk count
+ * or a negative status. */
+int64_t tg_parse_file(const char* path, uint32_t threads, uint64_t section_bytes, uint8_t** bases,
+                      uint64_t* n_bases, uint64_t** off);
 /* A plain (not gzip) input file as a read-only mapping, parsed in place with
  * no copy; same error messages as tg_read_file.  Release with tg_unmap_file. */
 int tg_map_file(const char* path, const uint8_t** data, uint64_t* len);

@@ -37,7 +37,7 @@ EXPORTS = [
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
     "tg_set_debug_stats", "tg_debug_partitions", "tg_debug_min_coverage",
     "tg_result_tsv", "tg_result_tsv_noalloc", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
-    "tg_unmap_file",
+    "tg_unmap_file", "tg_parse_file", "tg_count_files",
 ]
 
 
@@ -159,6 +159,11 @@ def lib():
     L.tg_parse_buffer_mt.restype = ctypes.c_int64
     L.tg_parse_buffer_mt.argtypes = [_u8p, ctypes.c_uint64, _u8p, _u64p, ctypes.c_uint64, ctypes.c_uint32]
     L.tg_read_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p), _u64p]
+    L.tg_parse_file.restype = ctypes.c_int64
+    L.tg_parse_file.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(_u8p), _u64p,
+                                ctypes.POINTER(_u64p)]
+    L.tg_count_files.argtypes = [_vp, ctypes.POINTER(TgParams), ctypes.POINTER(ctypes.c_char_p), ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_void_p]
     L.tg_map_file.argtypes = [ctypes.c_char_p, ctypes.POINTER(_u8p), _u64p]
     L.tg_unmap_file.argtypes = [_u8p, ctypes.c_uint64]
     L.tg_unmap_file.restype = None
@@ -283,8 +288,26 @@ def read_file(path):
         L.tg_free(data)
 
 
-def parse_file(path):
-    return parse_buffer(read_file(path))
+def parse_file(path, threads=1, section_bytes=0):
+    """One input file -> (bases, chunk offsets), read as the engine reads it:
+    gzip text parsed in sections of `section_bytes` while it inflates, BGZF
+    members inflated on `threads` threads, plain files parsed on `threads`
+    threads.  Equal to parse_buffer(read_file(path)) for every file."""
+    L = lib()
+    b = _u8p()
+    n = ctypes.c_uint64(0)
+    o = _u64p()
+    nch = L.tg_parse_file(str(path).encode(), threads, section_bytes, ctypes.byref(b), ctypes.byref(n),
+                          ctypes.byref(o))
+    if nch < 0:
+        raise TunaGpuError(L.tg_last_error().decode())
+    try:
+        bases = np.frombuffer(ctypes.string_at(b, n.value), dtype=np.uint8).copy()
+        off = np.ctypeslib.as_array(o, shape=(nch + 1,)).copy()
+    finally:
+        L.tg_free(b)
+        L.tg_free(o)
+    return bases, off
 
 
 def synth(spec):
@@ -525,6 +548,15 @@ class Engine:
         st = TgStats()
         _check(lib().tg_count_host(self._h, ctypes.byref(p), _p(bases, _u8p), len(bases),
                                    _p(off, _u64p), len(off) - 1, ctypes.byref(st)))
+        self.last_stats = st
+        return self.result(k) if fetch else st
+
+    def count_files(self, files, k, m=21, partitions=16, ci=1, cx=2**64 - 1, threads=1, fetch=True):
+        """Count FASTA/FASTQ(.gz) files read straight to the device."""
+        p = _params(k, m, partitions, ci, cx)
+        st = TgStats()
+        names = (ctypes.c_char_p * max(len(files), 1))(*[str(f).encode() for f in files])
+        _check(lib().tg_count_files(self._h, ctypes.byref(p), names, len(files), threads, ctypes.byref(st)))
         self.last_stats = st
         return self.result(k) if fetch else st
 

@@ -5,7 +5,10 @@
 // src/pipeline.hpp:111-178 (partition_kmers_mem + count_and_write_mem).
 // Device buffers live in the context and are reused across jobs.
 #include "engine_impl.h"
+#include "host_io.h"
 #include "tg_device.h"
+
+#include <sys/stat.h>
 
 #include <chrono>
 #include <unordered_map>
@@ -928,6 +931,104 @@ int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t 
     uint64_t* d_o;
     if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
     return count_job(c, p, d_b, n_bases, d_o, n_chunks, off, st);
+}
+
+// Input files read into device memory as they are parsed: the bases of each
+// section the host reader finalizes are staged through two pinned buffers
+// and uploaded while the reader goes on, so inflating, parsing and the
+// upload overlap; the count then runs on the device-resident bases.
+// src/pipeline.hpp:64-178 (read, partition and count the input files).
+int tg_count_files(tg_ctx* c, const tg_params* p, const char* const* paths, uint32_t n_files, uint32_t threads,
+                   tg_stats* st)
+{
+    if (!c || !p || (!paths && n_files)) return fail(TG_EINVAL, "null argument");
+    int rc = check_params(p, 0, 0);
+    if (rc) return rc;
+    if (n_files == 0) return fail(TG_EINVAL, "no input files");
+    HIPOK(hipSetDevice(c->device));
+    // a first guess of the bases, regrown (keeping what arrived) when short
+    uint64_t guess = 64;
+    for (uint32_t i = 0; i < n_files; ++i) {
+        struct stat sb {};
+        const std::string f(paths[i] ? paths[i] : "");
+        const bool gz = f.size() > 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
+        if (stat(f.c_str(), &sb) == 0) guess += (uint64_t)sb.st_size * (gz ? 3u : 1u);
+    }
+    Buf& dst = c->bufs["file_bases"];
+    auto room = [&](uint64_t need, uint64_t used) -> int {
+        if (dst.cap >= need) return TG_OK;
+        const uint64_t want = std::max<uint64_t>(need + need / 2, dst.cap + dst.cap / 2);
+        void* q = nullptr;
+        if (hipMalloc(&q, want) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(TG_ENOMEM, "device allocation of %llu bytes for the input bases failed",
+                        (unsigned long long)want);
+        }
+        if (used) HIPOK(hipMemcpyAsync(q, dst.p, used, hipMemcpyDeviceToDevice, c->stream));
+        HIPOK(hipStreamSynchronize(c->stream));
+        if (dst.p) HIPOK(hipFree(dst.p));
+        dst.p = q;
+        dst.cap = want;
+        return TG_OK;
+    };
+    if ((rc = room(guess, 0))) return rc;
+    constexpr uint64_t STAGE = 32ull << 20;
+    struct Stage {
+        uint8_t* buf[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        hipStream_t s;
+        ~Stage()
+        {
+            (void)hipStreamSynchronize(s);
+            for (int i = 0; i < 2; ++i) {
+                if (buf[i]) (void)hipHostFree(buf[i]);
+                if (done[i]) (void)hipEventDestroy(done[i]);
+            }
+        }
+    } stage;
+    stage.s = c->stream;
+    for (int i = 0; i < 2; ++i) {
+        HIPOK(hipHostMalloc((void**)&stage.buf[i], STAGE, hipHostMallocDefault));
+        HIPOK(hipEventCreateWithFlags(&stage.done[i], hipEventDisableTiming));
+    }
+    int cur = 0;
+    uint64_t base0 = 0;
+    std::vector<uint64_t> off{0}, fo;
+    for (uint32_t i = 0; i < n_files; ++i) {
+        auto ready = [&](const uint8_t* q, uint64_t n, uint64_t at, bool pinned) -> int {
+            const uint64_t g = base0 + at;
+            int r = room(g + n + 64, g);
+            if (r) return r;
+            uint8_t* d = static_cast<uint8_t*>(dst.p) + g;
+            if (pinned) {  // page-locked already: one copy, done before the memory goes
+                HIPOK(hipMemcpyAsync(d, q, n, hipMemcpyHostToDevice, c->stream));
+                HIPOK(hipStreamSynchronize(c->stream));
+                return TG_OK;
+            }
+            for (uint64_t done = 0; done < n;) {
+                const uint64_t piece = std::min(n - done, STAGE);
+                HIPOK(hipEventSynchronize(stage.done[cur]));
+                memcpy(stage.buf[cur], q + done, piece);
+                HIPOK(hipMemcpyAsync(d + done, stage.buf[cur], piece, hipMemcpyHostToDevice, c->stream));
+                HIPOK(hipEventRecord(stage.done[cur], c->stream));
+                cur ^= 1;
+                done += piece;
+            }
+            return TG_OK;
+        };
+        uint64_t nb = 0;
+        if ((rc = tgi::parse_file_sections(paths[i], threads, 0, ready, fo, &nb))) return rc;
+        off.reserve(off.size() + fo.size() - 1);
+        for (size_t j = 1; j < fo.size(); ++j) off.push_back(base0 + fo[j]);
+        base0 += nb;
+    }
+    HIPOK(hipStreamSynchronize(c->stream));
+    const uint64_t N = base0, nch = off.size() - 1;
+    if ((rc = check_params(p, N, nch))) return rc;
+    uint64_t* d_o;
+    if ((rc = ensure_t(c, "in_off", nch + 1, &d_o))) return rc;
+    HIPOK(hipMemcpyAsync(d_o, off.data(), (nch + 1) * 8ull, hipMemcpyHostToDevice, c->stream));
+    return count_job(c, p, static_cast<const uint8_t*>(dst.p), N, d_o, nch, off.data(), st);
 }
 
 uint64_t tg_result_size(tg_ctx* c) { return c ? c->n_out : 0; }

@@ -5,6 +5,7 @@
 // (src/seq_source.hpp:33-39): headers ignored, sequence lines joined,
 // every byte outside [ACGTacgt] ends a chunk.
 #include "../../include/tuna_gpu.h"
+#include "host_io.h"
 #include "kff.h"
 
 #include <fcntl.h>
@@ -15,6 +16,9 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <functional>
 #include <cstdio>
 #include <fstream>
 #include <cmath>
@@ -170,8 +174,10 @@ int64_t parse_fasta(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, 
 // Scans the records from `pos` (a record start) up to the first record that
 // starts at or after `stop`.  `quirk` reports a header whose first byte is a
 // '\n' at block offset 63: that record spans five lines.
+// (fastq_scan_at leaves `pos` at the record start where it stopped, so a
+// scan over a growing buffer can resume there.)
 template <class Sink>
-void fastq_scan(const uint8_t* d, uint64_t len, uint64_t pos, uint64_t stop, Sink& sink, bool& quirk)
+void fastq_scan_at(const uint8_t* d, uint64_t len, uint64_t& pos, uint64_t stop, Sink& sink, bool& quirk)
 {
     for (;;) {
         // header line
@@ -206,6 +212,12 @@ void fastq_scan(const uint8_t* d, uint64_t len, uint64_t pos, uint64_t stop, Sin
             pos = lf + 1;
         }
     }
+}
+
+template <class Sink>
+void fastq_scan(const uint8_t* d, uint64_t len, uint64_t pos, uint64_t stop, Sink& sink, bool& quirk)
+{
+    fastq_scan_at(d, len, pos, stop, sink, quirk);
 }
 
 int64_t parse_fastq(const uint8_t* d, uint64_t len, uint8_t* ob, uint64_t* off, uint64_t cap)
@@ -997,6 +1009,417 @@ int tg_synth_copy(uint8_t* dst, uint64_t cap)
 void tg_synth_free(void)
 {
     std::string().swap(g_synth);
+}
+
+}  // extern "C"
+
+// ---- streamed input files ---------------------------------------------------
+// The reference reads a gzip file through zlib's gzread, one decompressor
+// feeding the parser (src/seq_source.hpp:48-120, :196-204).  Here a gzip file
+// is inflated on one thread while the calling thread parses the text that has
+// arrived, section by section, and hands each section's bases on (to the
+// device upload) as soon as they are final; a BGZF file (gzip members that
+// record their own sizes) is inflated on all threads at once; a plain file is
+// mapped and parsed on all threads.  Every route gives the bases and chunk
+// ends that tg_parse_buffer gives for the file's whole text.
+namespace {
+
+// A private anonymous mapping whose pages are committed on first touch: a
+// buffer that grows in place up to its reserved size.
+struct Reserved {
+    uint8_t* p = nullptr;
+    uint64_t cap = 0;
+    bool reserve(uint64_t n)
+    {
+        void* q = mmap(nullptr, n, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+        if (q == MAP_FAILED) return false;
+        p = static_cast<uint8_t*>(q);
+        cap = n;
+        return true;
+    }
+    ~Reserved()
+    {
+        if (p) munmap(p, cap);
+    }
+};
+
+// Bases appended to a reserved buffer, chunk ends to a vector.
+struct GrowSink {
+    uint8_t* ob;
+    uint64_t cap;
+    std::vector<uint64_t>& off;
+    uint64_t nb = 0;
+    bool full = false;
+    bool bases(const uint8_t* p, uint64_t n)
+    {
+        if (nb + n > cap) {
+            full = true;
+            return false;
+        }
+        memcpy(ob + nb, p, n);
+        nb += n;
+        return true;
+    }
+    bool end()
+    {
+        off.push_back(nb);
+        return true;
+    }
+};
+
+inline uint32_t le32(const uint8_t* p) { return p[0] | (p[1] << 8) | (p[2] << 16) | ((uint32_t)p[3] << 24); }
+
+// BGZF (the blocked gzip of the SAM/BAM specification, section 4.1): every
+// member has FEXTRA with a subfield 'B','C' of two bytes holding the member's
+// size minus one, and ends with its uncompressed size (ISIZE).  Fills the
+// member starts (and the file end) and returns true when the whole file is
+// such members.
+bool bgzf_members(const uint8_t* d, uint64_t len, std::vector<uint64_t>& starts)
+{
+    starts.clear();
+    uint64_t p = 0;
+    while (p < len) {
+        if (len - p < 26 || d[p] != 0x1f || d[p + 1] != 0x8b || d[p + 2] != 8 || !(d[p + 3] & 4)) return false;
+        const uint64_t xlen = d[p + 10] | (d[p + 11] << 8);
+        uint64_t q = p + 12, bsize = 0;
+        const uint64_t e = q + xlen;
+        if (e > len) return false;
+        while (q + 4 <= e) {
+            const uint32_t slen = d[q + 2] | (d[q + 3] << 8);
+            if (d[q] == 'B' && d[q + 1] == 'C' && slen == 2 && q + 6 <= e) bsize = (uint64_t)(d[q + 4] | (d[q + 5] << 8)) + 1;
+            q += 4 + slen;
+        }
+        if (bsize < 26 || p + bsize > len) return false;
+        starts.push_back(p);
+        p += bsize;
+    }
+    starts.push_back(len);
+    return starts.size() > 1;
+}
+
+// One gzip member into dst; true when it inflated to exactly `want` bytes.
+bool inflate_member(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t want)
+{
+    z_stream z{};
+    if (inflateInit2(&z, 16 + MAX_WBITS) != Z_OK) return false;
+    z.next_in = const_cast<Bytef*>(src);
+    z.avail_in = (uInt)n;
+    z.next_out = dst;
+    z.avail_out = (uInt)want;
+    uint8_t spare;
+    int r = inflate(&z, Z_FINISH);
+    if (r == Z_BUF_ERROR && z.avail_out == 0) {  // output full: the member must end here
+        z.next_out = &spare;
+        z.avail_out = 1;
+        r = inflate(&z, Z_FINISH);
+    }
+    const bool ok = r == Z_STREAM_END && z.total_out == want;
+    inflateEnd(&z);
+    return ok;
+}
+
+// Progress of the inflating thread.
+struct Flow {
+    uint64_t piece = 4u << 20;  // output per inflate call
+    std::atomic<uint64_t> produced{0};
+    std::atomic<int> state{0};  // 0 running, 1 done, 2 out of room
+    std::atomic<bool> stop{false};
+};
+
+// Every gzip member of [src, src+n) into out, publishing progress.  As
+// gzread does, a member may be followed by further members (concatenated
+// files), bytes after the last member that do not start a member are ignored,
+// and a truncated or damaged stream ends the text where it stops inflating.
+void inflate_stream(const uint8_t* src, uint64_t n, Reserved& out, Flow& f)
+{
+    z_stream z{};
+    if (inflateInit2(&z, 16 + MAX_WBITS) != Z_OK) {
+        f.state.store(1);
+        return;
+    }
+    uint64_t fed = 0, made = 0;
+    int rc = 1;
+    while (!f.stop.load(std::memory_order_relaxed)) {
+        if (z.avail_in == 0 && fed < n) {
+            const uint64_t take = std::min<uint64_t>(n - fed, 1u << 30);
+            z.next_in = const_cast<Bytef*>(src + fed);
+            z.avail_in = (uInt)take;
+            fed += take;
+        }
+        const uint64_t room = std::min<uint64_t>(out.cap - made, f.piece);
+        if (room == 0) {
+            rc = 2;
+            break;
+        }
+        z.next_out = out.p + made;
+        z.avail_out = (uInt)room;
+        const int r = inflate(&z, Z_NO_FLUSH);
+        made += room - z.avail_out;
+        f.produced.store(made, std::memory_order_release);
+        if (r == Z_STREAM_END) {
+            const uint64_t at = fed - z.avail_in;  // first byte after the member
+            if (at + 1 < n && src[at] == 0x1f && src[at + 1] == 0x8b) {
+                inflateReset(&z);
+                continue;
+            }
+            break;
+        }
+        if (r == Z_OK) continue;
+        if (r == Z_BUF_ERROR && (z.avail_in != 0 || fed < n)) continue;
+        break;  // input ended inside a member, or damaged data
+    }
+    inflateEnd(&z);
+    f.state.store(rc, std::memory_order_release);
+}
+
+// Waits until the text holds at least `want` more bytes than `have`, or the
+// inflation ended; returns the text length then.
+uint64_t wait_text(Flow& f, uint64_t have, uint64_t want, bool& final_)
+{
+    for (;;) {
+        const int st = f.state.load(std::memory_order_acquire);
+        if (st != 0) {
+            final_ = true;
+            return f.produced.load(std::memory_order_acquire);
+        }
+        const uint64_t n = f.produced.load(std::memory_order_acquire);
+        if (n >= have + want) {
+            final_ = false;
+            return n;
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+    }
+}
+
+// One past the `back`-th last line feed in [from, end) (the last one counts
+// 1); `from` when there are fewer.
+uint64_t after_lf_back(const uint8_t* d, uint64_t from, uint64_t end, int back)
+{
+    uint64_t p = end;
+    while (p > from) {
+        const void* q = memrchr(d + from, '\n', (size_t)(p - from));
+        if (!q) return from;
+        p = (uint64_t)((const uint8_t*)q - d);
+        if (--back == 0) return p + 1;
+    }
+    return from;
+}
+
+// Parses the growing text section by section.  A section ends where all
+// output before it is final: FASTA just after a line feed (a header that
+// started before it has ended there; a chunk running across it continues in
+// the next section), FASTQ after the sixth-last line feed (each record that
+// starts before it has its four lines, or five for the lexer's block-boundary
+// case, within the text).  The last section ends with the text, so the
+// whole-text rules (a DNA run reaching the end of a text whose length is a
+// multiple of 64) apply as in tg_parse_buffer.
+int parse_growing(const uint8_t* d, Flow& f, uint64_t section, GrowSink& sink, const tgi::SectionFn& ready,
+                  uint64_t& total)
+{
+    uint64_t have = 0, done_to = 0, reported = 0, pos = 0;
+    bool final_ = false, in_chunk = false, quirk = false, started = false;
+    int fmt = 0;  // 1 FASTA, 2 FASTQ
+    for (;;) {
+        have = wait_text(f, have, section, final_);
+        if (!fmt && have) fmt = d[0] == '@' ? 2 : 1;
+        if (fmt) {
+            const uint64_t stop = final_ ? have : after_lf_back(d, fmt == 1 ? done_to : pos, have, fmt == 1 ? 1 : 6);
+            if (fmt == 1 && stop > done_to) {
+                if (!started) {  // bytes before the first header are skipped
+                    const void* q = memchr(d + done_to, '>', (size_t)(stop - done_to));
+                    started = q != nullptr;
+                    pos = started ? (uint64_t)((const uint8_t*)q - d) : stop;
+                }
+                bool closed = false;
+                if (started && !fasta_scan(d, have, pos, stop, sink, in_chunk, closed)) return TG_ENOMEM;
+                pos = stop;
+                done_to = stop;
+            } else if (fmt == 2 && stop > pos) {
+                fastq_scan_at(d, have, pos, stop, sink, quirk);
+                if (sink.full) return TG_ENOMEM;
+            }
+            if (final_ && fmt == 1 && in_chunk) sink.end();
+            if (sink.nb > reported) {
+                const int rc = ready(sink.ob + reported, sink.nb - reported, reported, false);
+                if (rc) return rc;
+                reported = sink.nb;
+            }
+        }
+        if (final_) break;
+    }
+    total = have;
+    return f.state.load() == 2 ? TG_ENOMEM : TG_OK;
+}
+
+// A whole text parsed on `threads` threads into `ob`; fills off.
+int parse_whole(const uint8_t* d, uint64_t len, uint8_t* ob, uint32_t threads, std::vector<uint64_t>& off,
+                const std::string& path)
+{
+    std::vector<uint64_t> o;
+    for (uint64_t cap = len / 64 + 1024;;) {
+        o.resize(cap);
+        const int64_t n = tg_parse_buffer_mt(d, len, ob, o.data(), cap, threads);
+        if (n == TG_ECAP && cap < len / 2 + 2) {
+            cap = len / 2 + 2;
+            continue;
+        }
+        if (n < 0) {
+            set_host_error("cannot parse input: " + path);
+            return TG_EINPUT;
+        }
+        off.assign(o.begin(), o.begin() + n + 1);
+        return TG_OK;
+    }
+}
+
+}  // namespace
+
+namespace tgi {
+
+int parse_file_sections(const char* path, uint32_t threads, uint64_t section, const SectionFn& ready,
+                        std::vector<uint64_t>& off, uint64_t* n_bases)
+{
+    const std::string p(path);
+    const bool gz = p.size() > 3 && p.compare(p.size() - 3, 3, ".gz") == 0;
+    off.assign(1, 0);
+    *n_bases = 0;
+    threads = std::max(threads, 1u);
+    if (!section) section = 32ull << 20;
+    if (gz && access(path, R_OK) != 0) {
+        set_host_error("Cannot open: " + p);  // gzopen's message (src/seq_source.hpp:90)
+        return TG_EINPUT;
+    }
+    const uint8_t* data = nullptr;
+    uint64_t len = 0;
+    int rc = tg_map_file(path, &data, &len);
+    if (rc) {
+        if (gz && rc == TG_EINPUT) set_host_error("Cannot read: " + p);
+        return rc;
+    }
+    struct Unmap {
+        const uint8_t* d;
+        uint64_t n;
+        ~Unmap() { tg_unmap_file(d, n); }
+    } unmap{data, len};
+    // gzread reads a file that does not open with the gzip magic as it is
+    const bool inflate = gz && len >= 2 && data[0] == 0x1f && data[1] == 0x8b;
+    if (!inflate) {
+        // parsed into page-locked memory when it can be had (one upload, no
+        // staging), else into ordinary memory
+        uint8_t* pin = nullptr;
+        Reserved plain;
+        if (tg_host_alloc(std::max<uint64_t>(len, 1), (void**)&pin) != TG_OK) {
+            pin = nullptr;
+            if (!plain.reserve(len + 64)) {
+                set_host_error("out of host memory reading " + p);
+                return TG_ENOMEM;
+            }
+        }
+        struct Free {
+            uint8_t* q;
+            ~Free()
+            {
+                if (q) tg_host_free(q);
+            }
+        } fr{pin};
+        uint8_t* ob = pin ? pin : plain.p;
+        if ((rc = parse_whole(data, len, ob, threads, off, p))) return rc;
+        *n_bases = off.back();
+        return *n_bases ? ready(ob, *n_bases, 0, pin != nullptr) : TG_OK;
+    }
+    std::vector<uint64_t> members;
+    if (bgzf_members(data, len, members)) {
+        const size_t nm = members.size() - 1;
+        std::vector<uint64_t> at(nm + 1, 0);
+        for (size_t i = 0; i < nm; ++i) at[i + 1] = at[i] + le32(data + members[i + 1] - 4);
+        const uint64_t tl = at[nm];
+        Reserved text, bases;
+        if (!text.reserve(tl + 64) || !bases.reserve(tl + 64)) {
+            set_host_error("out of host memory reading " + p);
+            return TG_ENOMEM;
+        }
+        std::atomic<size_t> next{0};
+        std::atomic<size_t> bad{nm};
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < std::min<uint64_t>(threads, nm); ++t)
+            th.emplace_back([&] {
+                for (size_t i; (i = next.fetch_add(1)) < nm;)
+                    if (!inflate_member(data + members[i], members[i + 1] - members[i], text.p + at[i], at[i + 1] - at[i])) {
+                        size_t b = bad.load();
+                        while (i < b && !bad.compare_exchange_weak(b, i)) {}
+                    }
+            });
+        for (auto& t : th) t.join();
+        // as gzread does, the text ends where a damaged member begins
+        const uint64_t good = at[bad.load()];
+        if (good == 0) {
+            set_host_error("Cannot read: " + p);
+            return TG_EINPUT;
+        }
+        if ((rc = parse_whole(text.p, good, bases.p, threads, off, p))) return rc;
+        *n_bases = off.back();
+        return *n_bases ? ready(bases.p, *n_bases, 0, false) : TG_OK;
+    }
+    // one thread inflating, this one parsing what has arrived
+    Reserved text, bases;
+    const uint64_t room = std::min<uint64_t>(std::max<uint64_t>(len * 1100, 1ull << 26), 1ull << 42);
+    if (!text.reserve(room + 64) || !bases.reserve(room + 64)) {
+        set_host_error("out of host memory reading " + p);
+        return TG_ENOMEM;
+    }
+    Flow f;
+    f.piece = std::min<uint64_t>(std::max<uint64_t>(section / 8, 16), 4u << 20);
+    std::thread inflater([&] { inflate_stream(data, len, text, f); });
+    GrowSink sink{bases.p, bases.cap, off};
+    uint64_t total = 0;
+    rc = parse_growing(text.p, f, section, sink, ready, total);
+    f.stop.store(true);
+    inflater.join();
+    if (rc == TG_ENOMEM) set_host_error("out of host memory reading " + p);
+    if (rc) return rc;
+    if (total == 0) {
+        set_host_error("Cannot read: " + p);
+        return TG_EINPUT;
+    }
+    *n_bases = sink.nb;
+    return TG_OK;
+}
+
+}  // namespace tgi
+
+extern "C" {
+
+int64_t tg_parse_file(const char* path, uint32_t threads, uint64_t section_bytes, uint8_t** bases,
+                      uint64_t* n_bases, uint64_t** off)
+{
+    if (!path || !bases || !n_bases || !off) return TG_EINVAL;
+    *bases = nullptr;
+    *off = nullptr;
+    *n_bases = 0;
+    std::vector<uint64_t> o;
+    std::vector<uint8_t> b;
+    uint64_t n = 0;
+    const int rc = tgi::parse_file_sections(path, threads, section_bytes,
+                                            [&](const uint8_t* q, uint64_t cnt, uint64_t at, bool) {
+                                                if (at != b.size()) return (int)TG_EINVAL;
+                                                b.insert(b.end(), q, q + cnt);
+                                                return (int)TG_OK;
+                                            },
+                                            o, &n);
+    if (rc) return rc;
+    *bases = (uint8_t*)malloc(std::max<uint64_t>(n, 1));
+    *off = (uint64_t*)malloc(o.size() * 8);
+    if (!*bases || !*off) {
+        free(*bases);
+        free(*off);
+        *bases = nullptr;
+        *off = nullptr;
+        return TG_ENOMEM;
+    }
+    if (n) memcpy(*bases, b.data(), n);
+    memcpy(*off, o.data(), o.size() * 8);
+    *n_bases = n;
+    return (int64_t)(o.size() - 1);
 }
 
 }  // extern "C"

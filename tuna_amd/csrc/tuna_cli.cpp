@@ -530,9 +530,12 @@ int run(Config& cfg)
         std::cerr << "[1/2] partitioning  (" << cfg.num_threads << " thread"
                   << (cfg.num_threads > 1 ? "s" : "") << ", in-memory, gfx950) ...\n";
     const auto t_part = std::chrono::steady_clock::now();
-    HostChunks h;
-    read_inputs(cfg, h);
     tg_params p{cfg.k, cfg.l, cfg.num_partitions, cfg.ci, cfg.cx};
+    // -tp, -kt and -dbg work from a host copy of the bases; a plain count
+    // reads the files straight to the device, the upload overlapping the read
+    const bool host_copy = cfg.partition_only || cfg.keep_tmp || cfg.debug_stats;
+    HostChunks h;
+    if (host_copy) read_inputs(cfg, h);
     if (cfg.partition_only) {
         uint64_t nsk = 0;
         uint32_t nsp = 0;
@@ -554,7 +557,14 @@ int run(Config& cfg)
         return 0;
     }
     tg_stats st{};
-    check(tg_count_host(ctx, &p, h.bases, h.n_bases, h.off.data(), h.off.size() - 1, &st));
+    if (host_copy) {
+        check(tg_count_host(ctx, &p, h.bases, h.n_bases, h.off.data(), h.off.size() - 1, &st));
+    } else {
+        std::vector<const char*> paths;
+        for (const auto& f : cfg.input_files) paths.push_back(f.c_str());
+        check(tg_count_files(ctx, &p, paths.data(), (uint32_t)paths.size(), std::max(1u, cfg.num_threads), &st));
+        h.seqs = st.n_chunks;
+    }
     // phase split on the wall clock: parse + device phase 1 | device phase 2 + output
     const double t_all = since(t_part);
     const double t_phase1 = std::max(0.0, t_all - st.ms_phase2 * 1e-3);
