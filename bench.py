@@ -43,12 +43,15 @@ WORKLOADS = {
         "spec": ("kind=repeat_fq genome=100000000 seed=3 gc=0.45 cov=20 rl=150 sub=0.003 "
                  "fam=50 flen=300 div=0.05 rep=0.35 tan=0.10"),
         "k": 31, "m": 23, "ci": 1, "n": 4096,
+        # the batch parameters tg_autotune_many picks for this job shape
+        "config": "kmers_per_bucket=16384,max_tiles=1024,load_factor=0.75",
     },
     "short_reads": {
         "desc": ("short-read FASTQ: 5 Mbp genome seed 2, 150 bp reads at 50x (1,666,667 reads, 250 Mbp), "
                  "0.5% substitutions, N 0.01%, k=31 m=21 ci=2 n=256"),
         "spec": "kind=reads_fq genome=5000000 seed=2 gc=0.5 reads=1666667 rl=150 sub=0.005 nrate=0.0001",
         "k": 31, "m": 21, "ci": 2, "n": 256,
+        "config": "kmers_per_bucket=32768,max_tiles=1024,load_factor=0.9",
     },
 }
 HEADLINE = "repeat_reads"
@@ -385,9 +388,10 @@ def load_traffic(kernel, workload):
 
 
 def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
-    """One workload, inputs resident in HBM: autotuned batch parameters, warm-up,
-    `steps` timed jobs, per-kernel HIP-event times of 3 more jobs, and the
-    algorithmic-bytes roofline of the job and of its dominant kernel."""
+    """One workload, inputs resident in HBM: batch parameters (the ones
+    given, else autotuned), warm-up, `steps` timed jobs, per-kernel HIP-event
+    times of 3 more jobs, and the algorithmic-bytes roofline of the job and of
+    its dominant kernel."""
     wl = WORKLOADS[name]
     K, M, CI, NPART = wl["k"], wl["m"], wl["ci"], wl["n"]
     d_b = eng.dev_upload(bases)
@@ -561,6 +565,9 @@ def main():
     ap.add_argument("--config", default=None,
                     help="single legs: batch parameters as key=value,... instead of autotuning them "
                          "(e.g. kmers_per_bucket=4096,max_tiles=1024,load_factor=0.9)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="single legs: autotune the batch parameters instead of using the workload's "
+                         "(the ones the autotuner picked for it, fixed so that profiles match the run)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default=None,
                     help="with --only single: the one workload to run (default: both)")
     ap.add_argument("--only", choices=["all", "single", "batched", "serving", "copy"], default="all",
@@ -604,7 +611,8 @@ def main():
         bases, off = T.parse_buffer(text, threads=threads)
         log(f"{name}: generated {len(text)} bytes FASTQ -> {len(bases)} bases in {len(off) - 1} chunks "
             f"({time.perf_counter() - t0:.1f} s)")
-        legs[name] = bench_single(T, eng, name, bases, off, args.steps, args.warmup, args.config)
+        cfg = args.config or (None if args.autotune else WORKLOADS[name].get("config"))
+        legs[name] = bench_single(T, eng, name, bases, off, args.steps, args.warmup, cfg)
         if name == "short_reads" and args.only == "all":
             legs[name]["output"] = bench_output(T, eng, WORKLOADS[name]["k"])
         if name == HEADLINE and args.only == "all":
