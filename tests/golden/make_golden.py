@@ -6,6 +6,7 @@ superkmer_io.hpp, seq_source.hpp/helicase) — on small hand-made inputs and
 stores inputs and outputs as JSON.  Needs /root/reference (this container);
 the committed JSON files are what the tests read.
 """
+import hashlib
 import json
 import os
 import random
@@ -20,6 +21,10 @@ REF = os.path.join(ROOT, "oracle", "_ref", "ref_tuna")
 
 KM = [(31, 21), (31, 23), (21, 15), (63, 21), (11, 5), (15, 7), (5, 3), (9, 8), (33, 15),
       (25, 11), (17, 2), (64, 31), (32, 16), (3, 2), (45, 9)]
+
+
+# (k, m) pairs of the count fixtures: the workloads' shapes and small ones
+COUNT_KM = [(31, 21), (31, 23), (21, 15), (63, 21), (11, 5), (5, 3), (33, 15), (64, 31)]
 
 
 def rs(rng, n, alph="ACGT"):
@@ -95,6 +100,25 @@ def main():
             out["window"]["%d|%d|%s" % (k, m, s)] = [int(x) for x in v]
     with open(os.path.join(HERE, "reference_vectors.json"), "w") as f:
         json.dump(out, f, separators=(",", ":"), sort_keys=True)
+    # counts from the reference's own table (the count mode of the driver
+    # runs Streaming_Kmer_Hash_Table over the reference's partition buffers):
+    # per case the number of reported k-mers, the sum of their counts and the
+    # SHA-256 of the sorted "<label>\t<count>\n" lines
+    counts = {}
+    for name, text in inputs().items():
+        path = os.path.join(tmp, name)
+        for (k, m) in COUNT_KM:
+            for ci, cx in ((1, 2**64 - 1), (2, 2**64 - 1), (1, 1), (2, 3)):
+                o = path + ".tsv"
+                subprocess.run([REF, "count", str(k), str(m), "16", "2", str(ci), str(cx), o, path],
+                               capture_output=True, text=True, check=True)
+                lines = sorted(open(o, "rb").read().splitlines(keepends=True))
+                body = b"".join(lines)
+                counts["%s|%d|%d|%d|%d" % (name, k, m, ci, cx)] = {
+                    "n": len(lines), "sum": sum(int(l.split(b"\t")[1]) for l in lines),
+                    "sha256": hashlib.sha256(body).hexdigest()}
+    with open(os.path.join(HERE, "reference_counts.json"), "w") as f:
+        json.dump(counts, f, separators=(",", ":"), sort_keys=True)
 
 
 if __name__ == "__main__":

@@ -304,3 +304,23 @@ def test_tandem_repeats_and_homopolymers(ceng, k, m):
     bases, off = O.chunks_of(strings)
     check_counts(ceng, bases, off, k, m, 16, label="tandem repeats")
     check_counts(ceng, bases, off, k, m, 16, ci=3, cx=40, label="tandem repeats, ci=3 cx=40")
+
+
+REF_COUNTS = json.load(open(os.path.join(HERE, "golden", "reference_counts.json")))
+
+
+def test_counts_match_reference_table(ceng):
+    """Every golden input and (k, m, ci, cx) of the reference-table fixture:
+    the device result, formatted as TSV on the device, sorted, has the number
+    of lines, the count sum and the SHA-256 that the reference's own
+    Streaming_Kmer_Hash_Table gave (tests/golden/make_golden.py)."""
+    import hashlib
+    for key in sorted(REF_COUNTS):
+        name, k, m, ci, cx = key.split("|")
+        k, m, ci, cx = int(k), int(m), int(ci), int(cx)
+        b, off = T.parse_buffer(GOLD["inputs"][name].encode())
+        st = ceng.count(b, off, k, m, 16, ci, cx, fetch=False)
+        lines = sorted(ceng.result_tsv(k).splitlines(keepends=True))
+        want = REF_COUNTS[key]
+        got = (len(lines), sum(int(l.split(b"\t")[1]) for l in lines), hashlib.sha256(b"".join(lines)).hexdigest())
+        assert st.n_out == want["n"] and got == (want["n"], want["sum"], want["sha256"]), key

@@ -4,6 +4,7 @@ Each oracle function is checked against (a) vectors produced by the reference
 code itself (tests/golden/reference_vectors.json, from oracle/_ref) and (b)
 independent mathematical facts: closed forms, strand symmetry, brute force.
 """
+import hashlib
 import json
 import os
 import random
@@ -253,3 +254,24 @@ def test_parallel_count_equals_brute_force(k, m, n, ci, cx, threads):
     sk = np.lexsort(tuple(pk[:, i] for i in range(pk.shape[1])))
     so = np.lexsort(tuple(ok[:, i] for i in range(ok.shape[1])))
     assert np.array_equal(pk[sk], ok[so]) and np.array_equal(pc[sk], oc[so])
+
+
+REF_COUNTS = json.load(open(os.path.join(HERE, "golden", "reference_counts.json")))
+
+
+@pytest.mark.parametrize("key", sorted(REF_COUNTS))
+def test_count_matches_reference_table(key):
+    """orc_count against the reference's own counting table: the fixture holds,
+    for each input and (k, m, ci, cx), the number of reported k-mers, the sum of
+    their counts and the SHA-256 of the sorted TSV lines that the reference's
+    Streaming_Kmer_Hash_Table produced over the reference's partition buffers
+    (tests/golden/make_golden.py)."""
+    name, k, m, ci, cx = key.split("|")
+    k, ci, cx = int(k), int(ci), int(cx)
+    b, off = O.parse(GOLD["inputs"][name].encode())
+    keys, cnt, _, _ = O.count(b, off, k, ci, cx)
+    lines = sorted(O.tsv_line(int(keys[i, 0]) | (int(keys[i, 1]) << 64 if keys.shape[1] > 1 else 0), k,
+                              int(cnt[i])).encode() for i in range(len(cnt)))
+    want = REF_COUNTS[key]
+    assert len(lines) == want["n"] and int(cnt.sum()) == want["sum"]
+    assert hashlib.sha256(b"".join(lines)).hexdigest() == want["sha256"]
