@@ -287,3 +287,18 @@ def test_single_result_survives_tile_change(eng):
         say(f"  single job of {len(c1)} k-mers: same bytes after a tile change")
     finally:
         eng.set_batch_config(**{k_: getattr(old, k_) for k_, _ in old._fields_})
+
+
+def test_result_store_across_key_widths(eng):
+    """Counts that alternate between one-word (k <= 32) and two-word keys
+    (k > 32) keep their result store sized by their results: a width change
+    does not grow it."""
+    b, off = T.parse_buffer(T.synth("kind=reads_fq genome=3000 seed=3 reads=40 rl=100 sub=0.01 nrate=0.0"))
+    want = {}
+    for k in (21, 33):
+        ok, oc, _, _ = O.count(b, off, k)
+        want[k] = len(oc)
+    for i in range(60):
+        k = (21, 33)[i % 2]
+        gk, gc = eng.count(b, off, k, 11, 16)
+        assert len(gc) == want[k], (i, k)

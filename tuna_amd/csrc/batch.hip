@@ -447,7 +447,10 @@ int ensure_graph(BatchState* s, hipStream_t st)
 int grow_store(ResultStore& r, hipStream_t st, uint64_t need, uint32_t nw)
 {
     if (r.cap >= need && r.nw == nw) return TG_OK;
-    uint64_t cap = std::max<uint64_t>(need, r.cap + r.cap / 2);
+    // grown geometrically while the key width stays; a new width keeps
+    // nothing, so it is sized from the need alone
+    const uint64_t grown = r.nw == nw ? r.cap + r.cap / 2 : 0;
+    uint64_t cap = std::max<uint64_t>(need, grown);
     cap = std::max<uint64_t>(cap, 1u << 20);
     uint64_t* k2 = nullptr;
     uint32_t* c2 = nullptr;
