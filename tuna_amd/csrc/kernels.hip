@@ -403,20 +403,16 @@ __device__ __forceinline__ uint32_t rec_len(uint32_t meta)
     return RW == 2 ? (meta & REC_LEN_MASK) : (meta & REC_FAT_LEN_MASK);
 }
 
-// Fat record of the piece starting at global base `start` with meta `meta`:
-// its bases realigned so the first is in bits 31:30 of word 1, bases past the
-// piece cleared (the packed stream keeps 8 spare words, so the reads stay in
-// bounds).
+// Fat record of the piece starting at global base `start` with meta `meta`,
+// from the packed words in[] = packed[start / 16 ...]: its bases realigned so
+// the first is in bits 31:30 of word 1, bases past the piece cleared (the
+// packed stream keeps 8 spare words, so the reads stay in bounds).
 template <int RW>
-__device__ __forceinline__ Rec<RW> fat_record(const uint32_t* packed, uint32_t start, uint32_t meta)
+__device__ __forceinline__ Rec<RW> fat_build(const uint32_t (&in)[RW], uint32_t start, uint32_t meta)
 {
     Rec<RW> r;
     const uint32_t len = meta & REC_FAT_LEN_MASK;
-    const uint32_t* src = packed + (start >> 4);
     const uint32_t sh = 2u * (start & 15u);
-    uint32_t in[RW];
-#pragma unroll
-    for (int q = 0; q < RW; ++q) in[q] = src[q];
     r.w[0] = meta;
 #pragma unroll
     for (int q = 0; q < RW - 1; ++q) {
@@ -446,22 +442,56 @@ struct SkStage {
 // The record writes of a flush, out of line (a flush comes every few dozen
 // k-mers and is inlined at every k-mer of the unrolled window loop otherwise);
 // its arguments are plain values so that nothing is copied to the stack.
-template <int RW>
+// TWO: two records per lane per round (their packed words loaded together);
+// the inlined flush of the small windows keeps one, as a second register set
+// would spill there.
+template <int RW, bool TWO>
 __device__ __forceinline__ void sk_write_recs(const uint2* buf, uint32_t cnt, uint32_t rank, uint32_t nact,
                                               uint64_t at, uint32_t* recs, uint32_t* metas,
                                               const uint32_t* packed)
 {
-    for (uint32_t i = rank; i < cnt; i += nact) {
-        const uint2 e = buf[i];
-        if (RW == 2) {
+    if (RW == 2) {
+        for (uint32_t i = rank; i < cnt; i += nact) {
+            const uint2 e = buf[i];
             Rec<RW> r;
             r.w[0] = e.x;
             r.w[1] = e.y;
             store_rec<RW>(recs, at + i, r);
-        } else {
-            store_rec<RW>(recs, at + i, fat_record<RW>(packed, e.x, e.y));
+            metas[at + i] = e.y;
         }
+        return;
+    }
+    if (!TWO) {
+        for (uint32_t i = rank; i < cnt; i += nact) {
+            const uint2 e = buf[i];
+            const uint32_t* s1 = packed + (e.x >> 4);
+            uint32_t in1[RW];
+#pragma unroll
+            for (int q = 0; q < RW; ++q) in1[q] = s1[q];
+            store_rec<RW>(recs, at + i, fat_build<RW>(in1, e.x, e.y));
+            metas[at + i] = e.y;
+        }
+        return;
+    }
+    // a lane without a second record loads its first one's words again
+    for (uint32_t i = rank; i < cnt; i += 2u * nact) {
+        const uint32_t i2 = i + nact;
+        const bool two = i2 < cnt;
+        const uint2 e = buf[i], e2 = buf[two ? i2 : i];
+        const uint32_t* s1 = packed + (e.x >> 4);
+        const uint32_t* s2 = packed + (e2.x >> 4);
+        uint32_t in1[RW], in2[RW];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) {
+            in1[q] = s1[q];
+            in2[q] = s2[q];
+        }
+        store_rec<RW>(recs, at + i, fat_build<RW>(in1, e.x, e.y));
         metas[at + i] = e.y;
+        if (two) {
+            store_rec<RW>(recs, at + i2, fat_build<RW>(in2, e2.x, e2.y));
+            metas[at + i2] = e2.y;
+        }
     }
 }
 
@@ -470,7 +500,7 @@ __device__ __noinline__ void sk_write_out(const uint2* buf, uint32_t cnt, uint32
                                           uint64_t at, uint32_t* recs, uint32_t* metas,
                                           const uint32_t* packed)
 {
-    sk_write_recs<RW>(buf, cnt, rank, nact, at, recs, metas, packed);
+    sk_write_recs<RW, true>(buf, cnt, rank, nact, at, recs, metas, packed);
 }
 
 // OL: the record writes out of line (the large windows, whose unrolled loop
@@ -485,7 +515,7 @@ __device__ __forceinline__ void sk_flush(SkStage& S, const SkArgs& a, uint32_t c
     if (rank == 0) base = atomicAdd(S.wg_n, cnt);
     base = __builtin_amdgcn_readfirstlane(base);
     if (OL) sk_write_out<RW>(S.buf, cnt, rank, nact, S.run + base, a.recs, a.metas, a.packed);
-    else sk_write_recs<RW>(S.buf, cnt, rank, nact, S.run + base, a.recs, a.metas, a.packed);
+    else sk_write_recs<RW, false>(S.buf, cnt, rank, nact, S.run + base, a.recs, a.metas, a.packed);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -912,18 +942,24 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
     constexpr uint32_t U = RW == 8 ? 4 : 8;
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     uint32_t rr = 0;  // run of this lane's last record (non-decreasing)
+    // slots first (run lookups), then all U loads back to back: lanes past the
+    // end load slot 0, which always exists
     for (uint64_t base = lo + (uint64_t)wv * 64u * U; base < hi; base += (uint64_t)nwv * 64u * U) {
-        Rec<RW> v[U];
+        uint64_t at[U];
         bool ok[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint64_t i = base + u * 64u + lane;
             ok[u] = i < hi;
+            at[u] = 0;
             if (ok[u]) {
                 while (s_off[rr + 1] <= i) ++rr;
-                v[u] = load_rec<RW>(RW == 1 ? a.metas : a.recs, (uint64_t)(s_base[rr] + (int64_t)i));
+                at[u] = (uint64_t)(s_base[rr] + (int64_t)i);
             }
         }
+        Rec<RW> v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) v[u] = load_rec<RW>(RW == 1 ? a.metas : a.recs, at[u]);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
             if (ok[u]) f(v[u]);
@@ -1048,15 +1084,18 @@ void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
 
 // Scatter, level 1: every tile writes its records into coarse buckets of F
 // consecutive sub-partitions (at most 256 buckets).  The tile goes through its
-// records in chunks that fit in LDS (64 KiB); a chunk is grouped by bucket in
-// LDS (histogram, scan, place) and then written out bucket by bucket, so
-// consecutive threads write consecutive addresses of one bucket.
+// records in chunks that fit in LDS (128 KiB of fat records, one workgroup per
+// CU; 64 KiB of slim ones); a chunk is grouped by bucket in LDS (histogram,
+// scan, place) and then written out bucket by bucket, so consecutive threads
+// write consecutive addresses of one bucket.  Longer runs per bucket write
+// faster: runs of 512 B move 16 B records at about 3.5 TB/s, runs of 256 B
+// at about 3.2 TB/s.
 constexpr uint32_t COARSE_RUNS = 512;  // run descriptors cached per tile
 
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint32_t* tmp)
 {
-    constexpr uint32_t CH = 16384u / RW;  // records per chunk (64 KiB)
+    constexpr uint32_t CH = (RW == 2 ? 16384u : 32768u) / RW;  // records per chunk
     constexpr uint32_t PER = CH / ROUTE_T;
     __shared__ Rec<RW> s_rec[CH];
     __shared__ uint32_t s_hist[256], s_start[256], s_cur[256], s_ord[256];
@@ -1090,12 +1129,14 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     for (uint64_t cb = lo; cb < hi; cb += CH) {
         for (uint32_t c = tid; c < nc; c += blockDim.x) s_hist[c] = 0;
         __syncthreads();
-        Rec<RW> v[PER];
-        uint32_t bk[PER], pos[PER];
+        // record slots first (run lookups), then all PER loads back to back
+        // (lanes past the end load record 0, which always exists), then the
+        // histogram
+        uint64_t at[PER];
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
             const uint64_t i = cb + j * ROUTE_T + tid;
-            bk[j] = 0xFFFFFFFFu;
+            at[j] = 0;
             if (i < hi) {
                 int64_t base;
                 if (cached) {
@@ -1110,7 +1151,17 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
                     }
                     base = (int64_t)a.run_start[l] - (int64_t)a.run_off[l];
                 }
-                v[j] = load_rec<RW>(a.recs, (uint64_t)(base + (int64_t)i));
+                at[j] = (uint64_t)(base + (int64_t)i);
+            }
+        }
+        Rec<RW> v[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) v[j] = load_rec<RW>(a.recs, at[j]);
+        uint32_t bk[PER], pos[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            bk[j] = 0xFFFFFFFFu;
+            if (cb + j * ROUTE_T + tid < hi) {
                 bk[j] = rec_meta<RW>(v[j]) >> shift;
                 pos[j] = atomicAdd(&s_hist[bk[j]], 1u);
             }
@@ -1177,11 +1228,13 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
         __syncthreads();
         constexpr uint32_t U = 8;  // loads in flight per thread
         for (uint64_t i0 = b + tid; i0 < e; i0 += U * ROUTE_T) {
+            // unconditional loads (past the end: the region's first word), so
+            // that all U are in flight together
             uint32_t mv[U];
 #pragma unroll
             for (uint32_t j = 0; j < U; ++j) {
                 const uint64_t i = i0 + (uint64_t)j * ROUTE_T;
-                mv[j] = i < e ? a.tmp_meta[i] : 0u;
+                mv[j] = a.tmp_meta[i < e ? i : b];
             }
 #pragma unroll
             for (uint32_t j = 0; j < U; ++j)
@@ -1206,14 +1259,19 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
     for (uint64_t cb = b; cb < e; cb += CH) {
         for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
         __syncthreads();
+        // all PER loads back to back (lanes past the end load the chunk's
+        // first record), then the histogram
         Rec<RW> v[PER];
-        uint32_t fk[PER], pos[PER];
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
             const uint64_t i = cb + j * ROUTE_T + tid;
+            v[j] = load_rec<RW>(tmp, i < e ? i : cb);
+        }
+        uint32_t fk[PER], pos[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
             fk[j] = 0xFFFFFFFFu;
-            if (i < e) {
-                v[j] = load_rec<RW>(tmp, i);
+            if (cb + j * ROUTE_T + tid < e) {
                 fk[j] = (rec_meta<RW>(v[j]) >> shift) & gm;
                 pos[j] = atomicAdd(&hist[fk[j]], 1u);
             }
