@@ -129,6 +129,26 @@ def test_superkmers_match_oracle(eng, k, m):
     assert a == b
 
 
+@pytest.mark.parametrize("k,m", [(31, 21), (21, 15), (63, 21), (15, 7), (5, 3)])
+def test_head_strips_caps_and_pieces(eng, ceng, k, m):
+    # every chunk fits one strip (at most 2048 k-mers), so every block takes
+    # the head-strip decision: superkmers capped at 256 - k k-mers
+    # (homopolymers and short tandem repeats longer than 255 bases), fat
+    # pieces, chunks of exactly k bases and of fewer
+    rng = random.Random(4000 + k)
+    strings = ["A" * 700, "AC" * 400, rs(rng, 300) + "T" * 500 + rs(rng, 40), "CAG" * 300,
+               rs(rng, 2000), "G" * k, "C" * (k - 1), rs(rng, k)]
+    strings += [rs(rng, rng.randint(1, 260)) for _ in range(200)]
+    rng.shuffle(strings)
+    bases, off = O.chunks_of(strings)
+    say(f"head strips k={k} m={m}")
+    gs, gl, gm, gp, nsp = eng.superkmers(bases, off, k, m, 16)
+    os_, ol, om, oh, km = O.superkmers(bases, off, k, m)
+    assert sorted(zip(gs.tolist(), gl.tolist(), gm.tolist())) == sorted(zip(os_.tolist(), ol.tolist(), om.tolist()))
+    st = check_counts(ceng, bases, off, k, m, 16, label="head")
+    assert st.n_superkmers == len(os_)
+
+
 @pytest.mark.parametrize("k,m", [(31, 21), (21, 15), (63, 21), (2, 1), (5, 3), (15, 7), (32, 16),
                                  (33, 15), (45, 9), (64, 31), (25, 11), (57, 6), (64, 1)])
 def test_counts_match_oracle_random_chunks(ceng, k, m):

@@ -585,6 +585,54 @@ __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, 
     sk_push<RW, OL>(S, a, emit, rec);
 }
 
+// The same decision for a head strip (the strip starts at the chunk's first
+// k-mer: ts = tb = 0), which needs neither look-back nor the flag: a cut
+// (minimizer change, or the cap: t >= tcap) starts a superkmer below te and
+// stops the lane at or past te; a piece ends at t >= tpiece.  Every threshold
+// is set at the cut or piece end that starts a piece, so a k-mer costs three
+// compares and the selects of the new piece's fields.  tcap starts at 0, so
+// k-mer 0 starts the first superkmer (nothing is emitted before it: have).
+// nsk counts superkmer starts: each started superkmer's first piece is
+// emitted exactly once (at the next cut or piece end, at the stop, or as the
+// chunk-end record).
+// The head state lives in registers of the general one (a head strip has
+// ts = tb = 0 and needs neither t0 nor P0 otherwise): tcap in t0, tpiece in
+// t0p (set to ~0 when the head loop starts), the current piece's global start
+// in P0, k - 1 - its start in ts, the current piece's meta bits in hsel and a
+// continuation piece's in tb.
+template <int RW, bool OL>
+__device__ __forceinline__ void sk_kmer_head(SkLane& L, const SkArgs& a, SkStage& S, uint32_t t,
+                                             uint64_t Hc, uint32_t Pc)
+{
+    uint32_t& tcap = L.t0;
+    uint32_t& tpiece = L.t0p;
+    uint32_t& sx = L.P0;
+    uint32_t& lb = L.ts;
+    uint32_t& mcur = L.hsel;
+    uint32_t& mcont = L.tb;
+    const bool cut = Hc != L.prevH || t >= tcap;
+    const bool ev = cut || t >= tpiece;
+    const bool emit = ev && L.have;
+    const uint2 rec = make_uint2(sx, (t + lb) | mcur);
+    const bool stop = cut && t >= L.te;
+    const bool start = cut && !stop;
+    const uint32_t hs = route_hsel(Hc, L.rbase, L.rl, a.xb) << REC_HSEL_SHIFT;
+    const uint32_t mhead = ((Pc - t) << REC_MINPOS_SHIFT) | hs;
+    mcur = start ? mhead : (ev ? mcont : mcur);
+    mcont = start ? (hs | REC_CONT) : mcont;
+    sx = ev ? (uint32_t)L.cb + t : sx;
+    lb = ev ? a.k - 1u - t : lb;
+    tpiece = RW != 2 && ev ? t + a.piece : tpiece;
+    tcap = start ? t + a.capk : tcap;
+    L.nsk += start ? 1u : 0u;
+    L.have = true;
+    L.prevH = Hc;
+    const bool last = !stop && t == L.K - 1u;
+    L.tail = last;
+    L.done = stop || last;
+    sk_push<RW, OL>(S, a, emit, rec);
+}
+
 __device__ __forceinline__ void sk_init(SkLane& L, const SkArgs& a, uint32_t s, const uint64_t* s_init)
 {
     const uint32_t c = a.strip_chunk[s];
@@ -671,7 +719,12 @@ __device__ __forceinline__ void sk_backward(uint64_t (&sv)[WMAX], uint8_t (&sp)[
     }
 }
 
-template <int WMAX, int RW>
+// MODE 0: every block, general decision (the fix-up pass); MODE 1: only the
+// blocks whose strips all start at their chunk's first k-mer (every block of
+// a read input), with the head decision; MODE 2: only the other blocks,
+// general decision.  The main pass launches MODE 1 and MODE 2 over the same
+// grid, so each block is counted by exactly one of them.
+template <int WMAX, int RW, int MODE>
 __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <= 12 ? 5 : 1))) void k_superkmers(SkArgs a)
 {
     // one LDS object, so the hash tables sit at address 0 and a table read
@@ -687,11 +740,7 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
     const uint64_t* s_init = sh.init;
     auto& s_buf = sh.buf;
     uint32_t& s_wgn = sh.wgn;
-    sk_tables(a, sh.tab, sh.tab + 16, sh.init);
     const uint32_t wv = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_wgn = 0;
-    __syncthreads();
-
     const uint32_t idx = blockIdx.x * SK_BLOCK + threadIdx.x;
     // strips (or flagged strips) of this launch; with device-side sizes the
     // launch covers a capacity and the live count is read here
@@ -699,9 +748,17 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
     if (a.n_dev) n = (uint32_t)min((uint64_t)n, *a.n_dev);
     if (a.fix_dev) {
         const uint32_t nf = *a.flag_count;
-        if (nf > n && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.err, 1u);
+        if (nf > n && blockIdx.x == 0 && threadIdx.x == 0 && MODE != 2) atomicOr(a.err, 1u);
         n = min(n, nf);
     }
+    if (MODE != 0) {
+        // a block without strips counts as a head block
+        const bool hd = idx >= n || a.strip_tb[a.list ? a.list[idx] : idx] == 0u;
+        if ((__syncthreads_and(hd) != 0) != (MODE == 1)) return;
+    }
+    sk_tables(a, sh.tab, sh.tab + 16, sh.init);
+    if (threadIdx.x == 0) s_wgn = 0;
+    __syncthreads();
     SkStage S;
     S.buf = s_buf[wv];
     S.wg_n = &s_wgn;
@@ -734,6 +791,13 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
         uint32_t bs = L.ts;
         uint64_t pv = 0;
         uint32_t pp = 0;
+        constexpr bool head = MODE == 1;
+        if (head) {
+            // head state (sk_kmer_head): tcap = t0 = 0 (k-mer 0 cuts), no
+            // piece end yet; the other fields are set at that first cut
+            L.t0 = 0;
+            L.t0p = 0xFFFFFFFFu;
+        }
         while (!L.done) {
 #pragma unroll
             for (int r = 0; r < WMAX; ++r) {
@@ -751,7 +815,8 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
                         Hc = pv;
                         Pc = bs + W + pp;
                     }
-                    sk_kmer<RW, (WMAX > 16)>(L, a, S, t, Hc, Pc);
+                    if constexpr (head) sk_kmer_head<RW, (WMAX > 16)>(L, a, S, t, Hc, Pc);
+                    else sk_kmer<RW, (WMAX > 16)>(L, a, S, t, Hc, Pc);
                     if (!L.done) {
                         sk_roll(L, tab);
                         const uint64_t h = L.f ^ L.rv;
@@ -773,8 +838,13 @@ __global__ __launch_bounds__(SK_BLOCK) __attribute__((amdgpu_waves_per_eu(WMAX <
             a.flag_list[slot] = s;
         }
         tail = L.tail;
-        tail_rec = sk_entry<RW>(L, a, L.t0p, L.K);
-        nsk = L.nsk + ((tail && L.t0p == L.t0) ? 1u : 0u);
+        if (head) {
+            tail_rec = make_uint2(L.P0, (L.K + L.ts) | L.hsel);
+            nsk = L.nsk;
+        } else {
+            tail_rec = sk_entry<RW>(L, a, L.t0p, L.K);
+            nsk = L.nsk + ((tail && L.t0p == L.t0) ? 1u : 0u);
+        }
         job = L.job;
     }
     // every lane of the wave is here again: the wave's counts, the chunk-end
@@ -813,7 +883,16 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
 {
     if (a.n == 0) return;
     const unsigned blocks = (a.n + SK_BLOCK - 1) / SK_BLOCK;
-#define TG_SK(WM, RWV) k_superkmers<WM, RWV><<<blocks, SK_BLOCK, 0, st>>>(a)
+    // the fix-up pass (a list of flagged strips) has no head strip
+#define TG_SK(WM, RWV)                                                    \
+    do {                                                                   \
+        if (a.list) {                                                      \
+            k_superkmers<WM, RWV, 0><<<blocks, SK_BLOCK, 0, st>>>(a);      \
+        } else {                                                           \
+            k_superkmers<WM, RWV, 1><<<blocks, SK_BLOCK, 0, st>>>(a);      \
+            k_superkmers<WM, RWV, 2><<<blocks, SK_BLOCK, 0, st>>>(a);      \
+        }                                                                  \
+    } while (0)
 #define TG_SK_RW(WM) \
     do { if (rw == 2) TG_SK(WM, 2); else if (rw == 4) TG_SK(WM, 4); else TG_SK(WM, 8); } while (0)
     if (a.W <= 8) TG_SK_RW(8);
