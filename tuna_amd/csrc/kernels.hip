@@ -1018,7 +1018,7 @@ __device__ __forceinline__ void for_tile_records(const RouteArgs& a, const Route
         }
         return;
     }
-    constexpr uint32_t U = RW == 8 ? 4 : 8;
+    constexpr uint32_t U = RW == 8 ? 4 : (RW == 1 ? 16 : 8);  // loads in flight per lane
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     uint32_t rr = 0;  // run of this lane's last record (non-decreasing)
     // slots first (run lookups), then all U loads back to back: lanes past the
@@ -1647,7 +1647,8 @@ __device__ __forceinline__ void kmer_at(const uint32_t* src, uint32_t wi, uint32
 // reverse complement.  kmask: the valid bits of the forward word that holds
 // the first base (fl when NW == 1, fh when NW == 2); rsh: the bit offset of
 // the new reverse-complement base in that word.
-template <int NW>
+// KH (one-word keys): 1 when k >= 17, 0 when k <= 16, -1 to decide at run time.
+template <int NW, int KH = -1>
 __device__ __forceinline__ void roll_base(uint64_t& fh, uint64_t& fl, uint64_t& rh, uint64_t& rl,
                                           uint32_t b, uint64_t kmask, uint32_t rsh)
 {
@@ -1656,7 +1657,7 @@ __device__ __forceinline__ void roll_base(uint64_t& fh, uint64_t& fl, uint64_t& 
         uint32_t flo = (uint32_t)fl, fhi = (uint32_t)(fl >> 32);
         uint32_t rlo = (uint32_t)rl, rhi = (uint32_t)(rl >> 32);
         const uint32_t cb = b ^ 3u;
-        if (rsh >= 32u) {  // k >= 17: the low forward word needs no mask
+        if (KH == 1 || (KH < 0 && rsh >= 32u)) {  // k >= 17: the low forward word needs no mask
             fhi = __builtin_amdgcn_alignbit(fhi, flo, 30u) & (uint32_t)(kmask >> 32);
             flo = (flo << 2) | b;
             rlo = __builtin_amdgcn_alignbit(rhi, rlo, 2u);
@@ -1789,6 +1790,8 @@ __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, 
                                                  uint32_t lane)
 {
     static_assert(CAP <= (1u << DEFER_PSH), "slot field");
+    // qn is wave-uniform: the loop is a scalar branch
+    qn = __builtin_amdgcn_readfirstlane(qn);
     while (qn >= 64u || (all && qn)) {
         const uint32_t take = min(qn, 64u), base = qn - take;
         const bool has = lane < take;
@@ -1826,7 +1829,7 @@ __device__ __forceinline__ uint32_t defer_passes(uint64_t* keys, uint32_t* cnt, 
             qk[at] = x;
             qs[at] = slot | (p << DEFER_PSH) | (w << DEFER_WSH);
         }
-        qn = base + (uint32_t)__popcll(am);
+        qn = __builtin_amdgcn_readfirstlane(base + (uint32_t)__popcll(am));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1863,7 +1866,7 @@ constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8
 // BATCH: a persistent launch; workgroups take items from a device queue until
 // the device-side item count is reached, spill keys carry the item's job, and
 // every item records its output segment for the per-job gather.
-template <int NW, uint32_t CAP, bool BATCH>
+template <int NW, uint32_t CAP, bool BATCH, int KH>
 __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
 {
     constexpr uint32_t NWAVE = count_block(NW) / 64;
@@ -2036,11 +2039,11 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                             s_qs[wv][at] = ((ps + 1u) & (CAP - 1u)) | (1u << DEFER_PSH) |
                                            ((decltype(WT)::value ? pw : 1u) << DEFER_WSH);
                         }
-                        qn += (uint32_t)__popcll(mm);
+                        qn = __builtin_amdgcn_readfirstlane(qn + (uint32_t)__popcll(mm));
                     }
-                    if (qn > DEFER_CAP - 64u)
-                        qn = defer_passes<CAP, BATCH>(s_k0, s_cnt, s_qk[wv], s_qs[wv], qn, false, a, tag,
-                                                      spilled, lane);
+                    if (__builtin_amdgcn_readfirstlane(qn) > DEFER_CAP - 64u)
+                        qn = __builtin_amdgcn_readfirstlane(defer_passes<CAP, BATCH>(s_k0, s_cnt, s_qk[wv], s_qs[wv], qn, false, a, tag,
+                                                      spilled, lane));
                 };
                 // k-mer i: issue its probe, then roll the lane on to k-mer i+1
                 auto step = [&](auto WT, uint32_t i, uint64_t& x, uint32_t& slot, uint64_t& old, bool& act,
@@ -2080,7 +2083,7 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                         } else {
                             pc += 2u;
                             if constexpr (decltype(WT)::value) ++rpos;
-                            roll_base<1>(fh, fl, rh, rl, b, kmask, rsh);
+                            roll_base<1, KH>(fh, fl, rh, rl, b, kmask, rsh);
                         }
                     }
                 };
@@ -2284,8 +2287,9 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
 void launch_count(hipStream_t st, const CountArgs& a, uint32_t nw)
 {
     if (a.n_items == 0) return;
-    if (nw == 1) k_count<1, LDS_CAP_NW1, false><<<a.n_items, count_block(1), 0, st>>>(a);
-    else k_count<2, LDS_CAP_NW2, false><<<a.n_items, count_block(2), 0, st>>>(a);
+    if (nw == 1 && a.k >= 17) k_count<1, LDS_CAP_NW1, false, 1><<<a.n_items, count_block(1), 0, st>>>(a);
+    else if (nw == 1) k_count<1, LDS_CAP_NW1, false, 0><<<a.n_items, count_block(1), 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, false, -1><<<a.n_items, count_block(2), 0, st>>>(a);
 }
 
 // Resident workgroups of the persistent count launch: occupancy x CUs.
@@ -2296,10 +2300,10 @@ uint32_t count_persistent_blocks(uint32_t nw)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (nw == 1)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, (const void*)k_count<1, LDS_CAP_NW1, true>, count_block(1), 0);
+            &per, (const void*)k_count<1, LDS_CAP_NW1, true, 1>, count_block(1), 0);
     else
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, (const void*)k_count<2, LDS_CAP_NW2, true>, count_block(2), 0);
+            &per, (const void*)k_count<2, LDS_CAP_NW2, true, -1>, count_block(2), 0);
     if (cus <= 0) cus = 256;
     if (per <= 0) per = 1;
     return (uint32_t)(cus * per);
@@ -2307,8 +2311,9 @@ uint32_t count_persistent_blocks(uint32_t nw)
 
 void launch_count_batch(hipStream_t st, const CountArgs& a, uint32_t nw, uint32_t blocks)
 {
-    if (nw == 1) k_count<1, LDS_CAP_NW1, true><<<blocks, count_block(1), 0, st>>>(a);
-    else k_count<2, LDS_CAP_NW2, true><<<blocks, count_block(2), 0, st>>>(a);
+    if (nw == 1 && a.k >= 17) k_count<1, LDS_CAP_NW1, true, 1><<<blocks, count_block(1), 0, st>>>(a);
+    else if (nw == 1) k_count<1, LDS_CAP_NW1, true, 0><<<blocks, count_block(1), 0, st>>>(a);
+    else k_count<2, LDS_CAP_NW2, true, -1><<<blocks, count_block(2), 0, st>>>(a);
 }
 
 template <int NW>
