@@ -305,7 +305,7 @@ template <uint16_t k, uint16_t m>
 inline void run(const std::vector<std::string>& files, const Options& opts,
                 std::vector<uint64_t>& keys, std::vector<uint32_t>& counts)
 {
-    static_assert(k >= 2 && k <= 64, "tuna (gfx950 engine): k must be in [2, 64]");
+    static_assert(k >= 2 && k <= 256, "tuna: k must be in [2, 256]");
     static_assert(m >= 1 && m < k && m <= 32, "tuna: m must be in [1, min(k-1, 32)]");
     if (files.empty()) throw std::invalid_argument("tuna::count: no input files");
     uint32_t n = opts.partitions;

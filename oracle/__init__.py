@@ -149,11 +149,16 @@ def pack_record(ascii_bytes, min_pos, k, m):
     return bytes(out[:n])
 
 
+def key_words(k):
+    """64-bit words of a k-mer key: ceil(k/32)."""
+    return (k + 31) // 32
+
+
 def canonical(s, k):
     a = as_u8(s)
-    w = np.zeros(2, dtype=np.uint64)
+    w = np.zeros(max(2, key_words(k)), dtype=np.uint64)
     lib().orc_canonical(_p(a, _u8p), k, _p(w, _u64p))
-    return int(w[0]) | (int(w[1]) << 64 if k > 32 else 0)
+    return sum(int(w[i]) << (64 * i) for i in range(key_words(k)))
 
 
 def count(bases, off, k, ci=1, cx=2**64 - 1):
@@ -161,7 +166,7 @@ def count(bases, off, k, ci=1, cx=2**64 - 1):
     bases = as_u8(bases)
     off = np.ascontiguousarray(off, dtype=np.uint64)
     nch = len(off) - 1
-    nw = 2 if k > 32 else 1
+    nw = key_words(k)
     K = 0
     if nch:
         L = np.diff(off.astype(np.int64))
@@ -208,7 +213,7 @@ def count_parallel(bases, off, k, m, n_parts=256, ci=1, cx=2**64 - 1, threads=1,
 
 
 def tsv_line(key, k, count):
-    w = np.array([key & (2**64 - 1), key >> 64], dtype=np.uint64)
+    w = np.array([(key >> (64 * i)) & (2**64 - 1) for i in range(max(2, key_words(k)))], dtype=np.uint64)
     buf = ctypes.create_string_buffer(k + 16)
     n = lib().orc_tsv_line(_p(w, _u64p), k, count, buf)
     return buf.raw[:n].decode()

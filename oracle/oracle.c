@@ -347,8 +347,40 @@ static inline u128 kmer_rc(const uint8_t* a, int k)
     return x;
 }
 
+/* k > 64: ceil(k/32) words, word 0 the lowest (the same layout, Kmer.hpp:44-48) */
+#define ORC_MAXW 8
+static void kw_push(uint64_t* w, int nw, uint64_t b)
+{
+    for (int i = nw - 1; i > 0; --i) w[i] = (w[i] << 2) | (w[i - 1] >> 62);
+    w[0] = (w[0] << 2) | b;
+}
+static void kw_fwd(const uint8_t* a, int k, int nw, uint64_t* w)
+{
+    memset(w, 0, sizeof(uint64_t) * (size_t)nw);
+    for (int i = 0; i < k; ++i) kw_push(w, nw, orc_kache_code(a[i]));
+}
+static void kw_rc(const uint8_t* a, int k, int nw, uint64_t* w)
+{
+    memset(w, 0, sizeof(uint64_t) * (size_t)nw);
+    for (int i = k - 1; i >= 0; --i) kw_push(w, nw, 3u - orc_kache_code(a[i]));
+}
+static int kw_cmp(const uint64_t* a, const uint64_t* b, int nw)
+{
+    for (int i = nw - 1; i >= 0; --i)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+
 void orc_canonical(const uint8_t* a, int k, uint64_t* words)
 {
+    if (k > 64) {
+        const int nw = (k + 31) / 32;
+        uint64_t f[ORC_MAXW], r[ORC_MAXW];
+        kw_fwd(a, k, nw, f);
+        kw_rc(a, k, nw, r);
+        memcpy(words, kw_cmp(f, r, nw) <= 0 ? f : r, sizeof(uint64_t) * (size_t)nw);
+        return;
+    }
     const u128 f = kmer_fwd(a, k), r = kmer_rc(a, k);
     const u128 c = f < r ? f : r;
     words[0] = (uint64_t)c;
@@ -379,6 +411,12 @@ static int cmp_u128(const void* x, const void* y)
     return a < b ? -1 : (a > b ? 1 : 0);
 }
 
+static int g_kw_nw; /* key words of the multi-word sort (k > 64) */
+static int cmp_kw(const void* x, const void* y)
+{
+    return kw_cmp((const uint64_t*)x, (const uint64_t*)y, g_kw_nw);
+}
+
 int64_t orc_count(const uint8_t* bases, const uint64_t* off, uint64_t n_chunks,
                   int k, uint32_t ci, uint64_t cx, uint64_t* keys,
                   uint32_t* counts, uint64_t cap, uint64_t* total_kmers,
@@ -392,7 +430,33 @@ int64_t orc_count(const uint8_t* bases, const uint64_t* off, uint64_t n_chunks,
     if (total_kmers) *total_kmers = K;
     int64_t nout = 0;
     uint64_t nu = 0;
-    if (k <= 32) {
+    if (k > 64) {
+        /* multi-word keys: every canonical k-mer written out, sorted, runs counted */
+        const int nw = (k + 31) / 32;
+        uint64_t* a = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nw * (K + 1));
+        uint64_t i = 0;
+        for (uint64_t c = 0; c < n_chunks; ++c) {
+            const uint64_t L = off[c + 1] - off[c];
+            if (L < (uint64_t)k) continue;
+            for (uint64_t j = 0; j + (uint64_t)k <= L; ++j) orc_canonical(bases + off[c] + j, k, a + nw * i++);
+        }
+        g_kw_nw = nw;
+        qsort(a, K, sizeof(uint64_t) * (size_t)nw, cmp_kw);
+        for (uint64_t x = 0; x < K;) {
+            uint64_t y = x + 1;
+            while (y < K && kw_cmp(a + nw * y, a + nw * x, nw) == 0) ++y;
+            const uint64_t cnt = y - x;
+            ++nu;
+            if (cnt >= ci && cnt <= cx) {
+                if ((uint64_t)nout >= cap) { free(a); return -1; }
+                memcpy(keys + (uint64_t)nw * (uint64_t)nout, a + nw * x, sizeof(uint64_t) * (size_t)nw);
+                counts[nout] = (uint32_t)cnt;
+                ++nout;
+            }
+            x = y;
+        }
+        free(a);
+    } else if (k <= 32) {
         uint64_t* a = (uint64_t*)malloc(sizeof(uint64_t) * (K + 1));
         uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * (K + 1));
         const uint64_t mask = k == 32 ? ~0ULL : ((1ULL << (2 * k)) - 1u);

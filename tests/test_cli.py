@@ -56,22 +56,27 @@ def test_cli_help_and_version():
     assert r.returncode == 0 and r.stderr.startswith("tuna 1.4.0")
 
 
-def test_cli_rejects_k_above_engine_limit(tmp_path):
+def test_cli_k_above_64_without_partition_files(tmp_path):
+    # k > 64 is counted without superkmer records: -tp, -kt and -dbg are refused
+    # before any device work; k above 256 is refused by the parser
     f = tmp_path / "x.fa"
     f.write_text(">a\nACGT\n")
-    r = run_cli("-k", "127", "-m", "21", str(f), str(tmp_path / "o.tsv"))
-    assert r.returncode == 1 and "unsupported combination k=127 m=21" in r.stderr
+    for flag in ("-tp", "-kt", "-dbg"):
+        r = run_cli("-k", "127", "-m", "21", flag, str(f), str(tmp_path / "o.tsv"))
+        assert r.returncode == 1 and "unsupported combination k=127 with -tp, -kt or -dbg" in r.stderr, flag
+    r = run_cli("-k", "257", "-m", "21", str(f), str(tmp_path / "o.tsv"))
+    assert r.returncode == 1 and "tuna: error:" in r.stderr
 
 
 def test_tsv_formatter_matches_oracle():
     rng = random.Random(3)
-    for k in (5, 31, 32, 33, 63):
-        nw = 2 if k > 32 else 1
+    for k in (5, 31, 32, 33, 63, 64, 65, 96, 127, 128, 200, 256):
+        nw = (k + 31) // 32
         keys, counts, lines = [], [], []
         for _ in range(50):
             key = rng.getrandbits(2 * k)
             c = rng.randint(1, 10**9)
-            keys.append([key & (2**64 - 1), key >> 64][:nw])
+            keys.append([(key >> (64 * i)) & (2**64 - 1) for i in range(nw)])
             counts.append(c)
             lines.append(O.tsv_line(key, k, c))
         ka = np.array(keys, dtype=np.uint64).reshape(-1)
@@ -192,7 +197,7 @@ def oracle_tsv(files_text, k, ci=1, cx=2**64 - 1):
     keys, cnt, _, _ = O.count(bases, off, k, ci, cx)
     out = {}
     for i in range(len(cnt)):
-        key = int(keys[i, 0]) | (int(keys[i, 1]) << 64 if keys.shape[1] > 1 else 0)
+        key = sum(int(keys[i, j]) << (64 * j) for j in range(keys.shape[1]))
         out[O.tsv_line(key, k, 1).split("\t")[0]] = int(cnt[i])
     return out
 
@@ -220,6 +225,29 @@ def test_cli_counts_match_oracle(tmp_path):
     assert int(stats["unique_kmers"]) == len(exp) and int(stats["n_parts"]) == 64
     b, o = O.parse(bytes(text))
     assert int(stats["superkmers"]) == len(O.superkmers(b, o, 31, 21)[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,m", [(127, 21), (200, 31)])
+def test_cli_counts_large_k_match_oracle(tmp_path, k, m):
+    # k > 64 (the reference's default dispatch includes k = 127): TSV and KFF
+    text = T.synth("kind=reads_fq genome=30000 seed=4 cov=12 rl=250 sub=0.004 nrate=0.0002")
+    fq = tmp_path / "r.fq"
+    fq.write_bytes(bytes(text))
+    out = tmp_path / "o.tsv"
+    r = run_cli("-k", str(k), "-m", str(m), "-ci", "2", str(fq), str(out))
+    assert r.returncode == 0, r.stderr
+    exp = oracle_tsv([bytes(text)], k, 2)
+    assert read_tsv(out) == exp
+    stats = dict(l.split(": ") for l in r.stderr.splitlines() if ": " in l and not l.startswith("tuna"))
+    b, o = O.parse(bytes(text))
+    assert int(stats["unique_kmers"]) == len(exp)
+    assert int(stats["superkmers"]) == len(O.superkmers(b, o, k, m)[0])
+    kff = tmp_path / "o.kff"
+    r = run_cli("-k", str(k), "-m", str(m), "-ci", "2", str(fq), str(kff))
+    assert r.returncode == 0, r.stderr
+    kk, recs = parse_kff(kff)
+    assert kk == k and recs == exp
 
 
 @pytest.mark.gpu
@@ -368,6 +396,19 @@ int main(int argc, char** argv) {
         ta.join(); tb.join();
         if (got_a != alone_a || got_b != alone_b) { fprintf(stderr, "threaded mismatch\n"); return 9; }
     }
+    // k > 64: keys of four words
+    auto m127 = tuna::count_to<127, 21>(files, o);
+    auto raw127 = tuna::count_to_raw<127, 21>(files, o);
+    if (m127.size() != raw127.size() || m127.empty()) return 10;
+    std::unordered_map<tuna::Kmer<127>, uint32_t, tuna::Kmer_Hasher<127>> by127;
+    for (auto& kv : raw127) {
+        std::string s; kv.first.get_label(s);
+        if (s.size() != 127 || m127.at(s) != kv.second) return 11;
+        if (kv.first.canonical() != kv.first || tuna::Kmer<127>(s) != kv.first) return 12;
+        by127[kv.first] = kv.second;
+    }
+    if (by127.size() != raw127.size()) return 13;
+    fprintf(stderr, "k127 %zu\n", m127.size());
     try { tuna::count<31>({}, [](std::string_view, uint32_t) {}); return 4; }
     catch (const std::invalid_argument& e) { fprintf(stderr, "%s\n", e.what()); }
     return 0;
@@ -398,6 +439,7 @@ def test_cpp_api_count_to_and_raw(tmp_path):
         a, b = line.split("\t")
         got[a] = int(b)
     assert got == oracle_tsv([bytes(text)], 31, 2)
+    assert f"k127 {len(oracle_tsv([bytes(text)], 127, 2))}" in r.stderr
 
 
 @pytest.mark.gpu

@@ -47,7 +47,8 @@ def inputs():
     return [T.parse_buffer(low), T.parse_buffer(reads), T.parse_buffer(genome)]
 
 
-@pytest.mark.parametrize("k,m", [(2, 1), (15, 7), (31, 21), (32, 15), (33, 17), (63, 21), (64, 31)])
+@pytest.mark.parametrize("k,m", [(2, 1), (15, 7), (31, 21), (32, 15), (33, 17), (63, 21), (64, 31), (65, 21),
+                                 (127, 21), (200, 31)])
 def test_device_tsv_and_kff_records_match_host(eng, k, m, tmp_path):
     for i, (b, o) in enumerate(inputs()):
         say(f"format k={k} input {i}: {len(b)} bases")
@@ -64,6 +65,9 @@ def test_device_tsv_and_kff_records_match_host(eng, k, m, tmp_path):
         kb = (k + 3) // 4 + 4
         assert len(rec) == n * kb
         whole = host_kff(tmp_path / "h.kff", k, keys, cnt)
+        if n == 0:  # every read shorter than k
+            assert rec == b"" and len(eng.result_tsv(k)) == 0
+            continue
         pos = whole.find(rec)
         assert pos > 0 and whole[pos + len(rec):].endswith(b"KFF")
         if n > 7:

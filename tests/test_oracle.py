@@ -189,11 +189,11 @@ def _brute_counts(strings, k):
 
 
 def _key_to_label(row, k):
-    key = int(row[0]) | (int(row[1]) << 64 if len(row) > 1 else 0)
+    key = sum(int(row[i]) << (64 * i) for i in range(len(row)))
     return "".join("ACGT"[(key >> (2 * (k - 1 - i))) & 3] for i in range(k))
 
 
-@pytest.mark.parametrize("k", [2, 5, 15, 31, 32, 33, 45, 63, 64])
+@pytest.mark.parametrize("k", [2, 5, 15, 31, 32, 33, 45, 63, 64, 65, 96, 97, 127, 128, 200, 256])
 def test_count_equals_python_brute_force(k):
     rng = random.Random(k)
     strings = [rs(rng, rng.randint(1, 300)) for _ in range(30)] + ["acgt" * 30, "T" * 100]
@@ -217,7 +217,7 @@ def test_count_filters():
 
 def test_canonical_invariants():
     rng = random.Random(6)
-    for k in (1, 7, 31, 32, 33, 63, 64):
+    for k in (1, 7, 31, 32, 33, 63, 64, 65, 127, 128, 129, 256):
         for _ in range(20):
             s = rs(rng, k)
             assert O.canonical(s, k) == O.canonical(rc(s), k)

@@ -118,6 +118,11 @@ class TunaGpuError(RuntimeError):
     pass
 
 
+def key_words(k):
+    """64-bit words per key of a result: ceil(k/32), the reference Kmer<k> layout."""
+    return (k + 31) // 32
+
+
 def lib():
     """Load libtuna_gpu.so (raises if it has not been built)."""
     global _lib
@@ -377,7 +382,7 @@ class Server:
     def __init__(self, k, m=21, ci=1, cx=2**64 - 1, lanes=2, max_batch_jobs=64, max_batch_bases=0,
                  hbm_budget=0, device=0):
         self.k = k
-        self.nw = 2 if k > 32 else 1
+        self.nw = key_words(k)
         p = _params(k, m, 16, ci, cx)
         cfg = TgServerConfig(lanes, max_batch_jobs, max_batch_bases, hbm_budget)
         self._h = _vp()
@@ -570,7 +575,7 @@ class Engine:
         return st
 
     def result(self, k):
-        nw = 2 if k > 32 else 1
+        nw = key_words(k)
         n = lib().tg_result_size(self._h)
         keys = np.zeros(max(n, 1) * nw, dtype=np.uint64)
         cnt = np.zeros(max(n, 1), dtype=np.uint32)
@@ -656,7 +661,7 @@ class Engine:
         return best.as_dict(), ms.value
 
     def many_result(self, job, k):
-        nw = 2 if k > 32 else 1
+        nw = key_words(k)
         n = lib().tg_many_result_size(self._h, job)
         keys = np.zeros(max(n, 1) * nw, dtype=np.uint64)
         cnt = np.zeros(max(n, 1), dtype=np.uint32)

@@ -633,6 +633,8 @@ int count_many_device(tg_ctx* ctx, const tg_params* p, const uint8_t* d_bases, u
 {
     int rc = check_params(p, 0, 0);
     if (rc) return rc;
+    if (p->k > 64)
+        return fail(TG_EINVAL, "the batched engine counts k <= 64; count jobs of k = %u one at a time", p->k);
     if (!jt.job_chunk || !jt.job_base) return fail(TG_EINVAL, "null job table");
     if (jt.job_chunk[jt.n] > n_chunks || jt.job_base[jt.n] > n_bases)
         return fail(TG_EINVAL, "job table exceeds the input (%llu chunks, %llu bases)",

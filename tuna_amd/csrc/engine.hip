@@ -43,6 +43,9 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
                const uint64_t* d_off, uint64_t nch, Phase1& o, bool slim = false, uint32_t nsp_min = 0)
 {
     hipStream_t st = c->stream;
+    if (p->k > 64)
+        return fail(TG_EINVAL, "k = %u: superkmer records, partition exports and debug statistics are built "
+                               "for k <= 64 (k > 64 is counted directly)", p->k);
     o.k = p->k;
     o.m = p->m;
     o.W = p->k - p->m + 1;
@@ -664,10 +667,199 @@ int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
     return TG_OK;
 }
 
+// k > 64: the superkmers are counted (main pass, then the flagged strips with
+// full look-back), every k-mer goes into one open-addressed HBM table of keys
+// of 4 or 8 words (power of two, about 1.5 slots per k-mer, doubled while a
+// key finds no slot and memory allows), and the table is drained through the
+// ci/cx filter twice: once to size the result, once to write it with
+// ceil(k/32) words per key.  Indices are 64-bit, so one pass takes the whole
+// job.
+int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
+               uint64_t nch, tg_stats* st)
+{
+    int rc = check_params(p, N, nch);
+    if (rc) return rc;
+    const uint32_t k = p->k, m = p->m, W = k - m + 1;
+    if (W > tg::WIDE_WMAX) return fail(TG_EINVAL, "k - m + 1 = %u exceeds %u", W, tg::WIDE_WMAX);
+    HIPOK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const uint32_t nw = k <= 128 ? 4u : 8u, onw = (k + 31) / 32;
+    tg_stats sum{};
+    sum.n_bases = N;
+    sum.n_chunks = nch;
+    sum.n_parts = round_pow2(p->n_partitions ? p->n_partitions : 16u);
+    HIPOK(hipEventRecord(c->ev_a, s));
+    // pack and strips
+    const uint32_t S = 2048;
+    const uint64_t n_words = (N + 15) / 16 + 8;
+    uint32_t* packed;
+    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp;
+    if ((rc = ensure_t(c, "packed", n_words, &packed)) || (rc = ensure_t(c, "nstr", nch + 1, &nstr)) ||
+        (rc = ensure_t(c, "nkm", nch + 1, &nkm)) || (rc = ensure_t(c, "nstr_x", nch + 1, &nstr_x)) ||
+        (rc = ensure_t(c, "nkm_x", nch + 1, &nkm_x)) ||
+        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)))
+        return rc;
+    {
+        Prof pr(c, "pack");
+        tg::launch_pack(s, d_bases, N, packed, n_words);
+    }
+    {
+        Prof pr(c, "strips");
+        tg::launch_chunk_counts(s, d_off, nch, k, S, nstr, nkm);
+        tg::launch_scan_u64(s, nstr, nch, nstr_x, tmp);
+        tg::launch_scan_u64(s, nkm, nch, nkm_x, tmp);
+    }
+    HIPOK(hipGetLastError());
+    uint64_t tot[2];
+    HIPOK(hipMemcpyAsync(&tot[0], nstr_x + nch, 8, hipMemcpyDeviceToHost, s));
+    HIPOK(hipMemcpyAsync(&tot[1], nkm_x + nch, 8, hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    const uint64_t n_strips = tot[0], n_kmers = tot[1];
+    if (n_strips >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
+    uint32_t *strip_chunk, *strip_tb, *flag_list, *flags;
+    uint64_t* strip_slot;
+    unsigned long long* sk_count;
+    const uint64_t ns1 = n_strips + 1;
+    if ((rc = ensure_t(c, "strip_chunk", ns1, &strip_chunk)) || (rc = ensure_t(c, "strip_tb", ns1, &strip_tb)) ||
+        (rc = ensure_t(c, "strip_slot", ns1, &strip_slot)) || (rc = ensure_t(c, "flag_list", ns1, &flag_list)) ||
+        (rc = ensure_t(c, "sk_flags", 64, &flags)) || (rc = ensure_t(c, "sk_count", 8, &sk_count)))
+        return rc;
+    {
+        Prof pr(c, "strips");
+        tg::launch_fill_strips(s, nch, nstr_x, nkm_x, nkm, S, strip_chunk, strip_tb, strip_slot);
+        tg::launch_fill_u32(s, flags, 0u, 64);
+        tg::launch_fill_u64(s, (uint64_t*)sk_count, 0ull, 1);
+    }
+    // superkmers, counted: the cap of the reference's header type
+    // (2k - m <= 255: one byte, 255 bases; else two bytes, 65535 bases)
+    const uint32_t hdr_max = 2u * k - m <= 255u ? 255u : 65535u;
+    tg::SkArgs a{};
+    a.packed = packed;
+    a.off = d_off;
+    a.strip_chunk = strip_chunk;
+    a.strip_tb = strip_tb;
+    a.strip_slot = strip_slot;
+    a.n = (uint32_t)n_strips;
+    a.k = k;
+    a.m = m;
+    a.W = W;
+    a.S = S;
+    a.capk = hdr_max + 1u - k;
+    a.piece = 0;
+    a.lookback = a.capk + 1u;
+    a.err = flags;
+    a.sk_count = sk_count;
+    a.flag_list = flag_list;
+    a.flag_count = flags + 1;
+    {
+        Prof pr(c, "superkmers");
+        tg::launch_superkmers_wide(s, a);
+    }
+    HIPOK(hipGetLastError());
+    uint32_t n_flag = 0;
+    HIPOK(hipMemcpyAsync(&n_flag, flags + 1, 4, hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    if (n_flag) {
+        // strips whose look-back held no cut: counted again from the chunk start
+        a.list = flag_list;
+        a.n = n_flag;
+        a.lookback = tg::LOOKBACK_FULL;
+        a.flag_count = flags + 2;  // nothing is flagged with full look-back
+        Prof pr(c, "superkmers_fixup");
+        tg::launch_superkmers_wide(s, a);
+    }
+    HIPOK(hipEventRecord(c->ev_b, s));
+    // the k-mer table
+    size_t free_b = 0, total_b = 0;
+    HIPOK(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t slot_bytes = 8ull * nw + 4ull;
+    uint64_t slots = 1ull << 16;
+    while (slots < n_kmers + n_kmers / 2) slots <<= 1;
+    uint64_t* tkeys = nullptr;
+    uint32_t* tcnt = nullptr;
+    unsigned long long* ctr = nullptr;
+    for (int attempt = 0;; ++attempt) {
+        // keep a quarter of the free memory for the result
+        if (slots * slot_bytes > free_b / 4 * 3 + c->bufs["wide_keys"].cap + c->bufs["wide_cnt"].cap)
+            return fail(TG_ENOMEM, "the k-mer table for k = %u needs %llu slots of %llu bytes; device memory "
+                                   "holds fewer", k, (unsigned long long)slots, (unsigned long long)slot_bytes);
+        if ((rc = ensure_t(c, "wide_keys", slots * nw, &tkeys)) || (rc = ensure_t(c, "wide_cnt", slots, &tcnt)) ||
+            (rc = ensure_t(c, "wide_ctr", 4, &ctr)))
+            return rc;
+        tg::launch_fill_u32(s, tcnt, 0u, slots);
+        tg::launch_fill_u32(s, flags + 8, 0u, 1);
+        tg::WideArgs w{};
+        w.packed = packed;
+        w.off = d_off;
+        w.strip_chunk = strip_chunk;
+        w.strip_tb = strip_tb;
+        w.n_strips = (uint32_t)n_strips;
+        w.n_dev = nstr_x + nch;
+        w.S = S;
+        w.k = k;
+        w.keys = tkeys;
+        w.cnt = tcnt;
+        w.mask = slots - 1;
+        w.err = flags + 8;
+        {
+            Prof pr(c, "count");
+            tg::launch_wide_insert(s, w, nw);
+        }
+        HIPOK(hipGetLastError());
+        uint32_t err = 0;
+        HIPOK(hipMemcpyAsync(&err, flags + 8, 4, hipMemcpyDeviceToHost, s));
+        HIPOK(hipStreamSynchronize(s));
+        if (err) {
+            if (attempt >= 3) return fail(TG_ENOMEM, "k-mer table overflow after %d retries", attempt);
+            slots <<= 1;
+            continue;
+        }
+        // drain: size, then write
+        unsigned long long hc[2] = {0, 0};
+        tg::launch_fill_u64(s, (uint64_t*)ctr, 0ull, 4);
+        tg::launch_wide_drain(s, w, nw, p->ci, p->cx, onw, nullptr, nullptr, ctr);
+        HIPOK(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
+        HIPOK(hipStreamSynchronize(s));
+        uint64_t* out_keys;
+        uint32_t* out_cnt;
+        if ((rc = ensure_t(c, "wide_out_keys", (hc[0] + 1) * onw, &out_keys)) ||
+            (rc = ensure_t(c, "wide_out_cnt", hc[0] + 1, &out_cnt)))
+            return rc;
+        {
+            Prof pr(c, "job_gather");
+            tg::launch_fill_u64(s, (uint64_t*)ctr, 0ull, 4);
+            tg::launch_wide_drain(s, w, nw, p->ci, p->cx, onw, out_keys, out_cnt, ctr);
+        }
+        HIPOK(hipGetLastError());
+        unsigned long long nsk = 0;
+        HIPOK(hipMemcpyAsync(&nsk, sk_count, 8, hipMemcpyDeviceToHost, s));
+        HIPOK(hipEventRecord(c->ev_c, s));
+        HIPOK(hipEventSynchronize(c->ev_c));
+        c->n_out = hc[0];
+        c->nw = onw;
+        c->d_keys = out_keys;
+        c->d_cnt = out_cnt;
+        sum.n_kmers = n_kmers;
+        sum.n_superkmers = nsk;
+        sum.n_unique = hc[1];
+        sum.n_out = hc[0];
+        break;
+    }
+    float m1 = 0, m2 = 0;
+    hipEventElapsedTime(&m1, c->ev_a, c->ev_b);
+    hipEventElapsedTime(&m2, c->ev_b, c->ev_c);
+    sum.ms_phase1 = m1;
+    sum.ms_phase2 = m2;
+    collect_prof(c);
+    if (st) *st = sum;
+    return TG_OK;
+}
+
 // One job: in one pass, or in slices when it is larger than the slice limit.
 int count_job(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
               uint64_t nch, const uint64_t* h_off, tg_stats* st)
 {
+    if (p && p->k > 64) return count_wide(c, p, d_bases, N, d_off, nch, st);
     const uint64_t limit = slice_limit(c);
     if (N > limit) {
         c->dbg_inserted.clear();

@@ -195,8 +195,8 @@ inline void collect_prof(tg_ctx* c)
 inline int check_params(const tg_params* p, uint64_t n_bases, uint64_t n_chunks)
 {
     if (!p) return fail(TG_EINVAL, "null parameters");
-    if (p->k < 2 || p->k > 64)
-        return fail(TG_EINVAL, "k-mer length (-k %u) must be in [2, 64] on the GPU engine", p->k);
+    if (p->k < 2 || p->k > 256)
+        return fail(TG_EINVAL, "k-mer length (-k %u) must be in [2, 256]", p->k);
     if (p->m < 1 || p->m > 32)
         return fail(TG_EINVAL, "minimizer length (-m %u) must be in [1, 32]", p->m);
     if (p->m >= p->k)

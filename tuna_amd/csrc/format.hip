@@ -18,9 +18,11 @@
 namespace tg {
 
 constexpr uint32_t FMT_BLOCK = 256;
-constexpr uint32_t TSV_TILE = 512;                 // lines per workgroup
+constexpr uint32_t TSV_TILE = 512;                 // lines per workgroup (k <= 64)
 constexpr uint32_t TSV_LINE_MAX = 64u + 1u + 10u + 1u;  // k <= 64, tab, count, newline
-constexpr uint32_t KFF_REC_MAX = 16u + 4u;
+constexpr uint32_t TSV_TILE_W = 256;               // lines per workgroup (64 < k <= 256)
+constexpr uint32_t TSV_LINE_MAX_W = 256u + 1u + 10u + 1u;
+constexpr uint32_t KFF_REC_MAX = 64u + 4u;         // k <= 256
 
 __device__ __forceinline__ uint32_t dec_digits(uint32_t c)
 {
@@ -45,13 +47,47 @@ __device__ __forceinline__ void load_key(const uint64_t* keys, uint32_t nw, uint
     hi = nw == 2 ? keys[e * nw + 1] : 0ull;
 }
 
+// The k letters of key e (nw words, word 0 the lowest) into dst: one or two
+// words from registers; wider keys one word at a time as the bases pass
+// through it (32 bases per word, first base in the top word).
+__device__ __forceinline__ void put_label(uint8_t* dst, const uint64_t* keys, uint32_t nw, uint64_t e,
+                                          uint32_t k)
+{
+    constexpr uint32_t ACGT = 0x54474341u;  // "ACGT", byte b = letter of code b
+    if (nw <= 2) {
+        uint64_t lo, hi;
+        load_key(keys, nw, e, lo, hi);
+        for (uint32_t j = 0; j < k; ++j) dst[j] = (uint8_t)(ACGT >> (8u * key_base(lo, hi, k, j)));
+        return;
+    }
+    uint32_t cw = 0xFFFFFFFFu;
+    uint64_t w = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t bit = 2u * (k - 1u - j), wi = bit >> 6;
+        if (wi != cw) {
+            w = keys[e * nw + wi];
+            cw = wi;
+        }
+        dst[j] = (uint8_t)(ACGT >> (8u * ((uint32_t)(w >> (bit & 63u)) & 3u)));
+    }
+}
+
+// base j of key e, any width
+__device__ __forceinline__ uint32_t key_base_w(const uint64_t* keys, uint32_t nw, uint64_t e, uint32_t k,
+                                               uint32_t j)
+{
+    const uint32_t bit = 2u * (k - 1u - j);
+    return (uint32_t)(keys[e * nw + (bit >> 6)] >> (bit & 63u)) & 3u;
+}
+
+template <uint32_t TILE>
 __global__ __launch_bounds__(FMT_BLOCK) void k_tsv_len(const uint32_t* __restrict__ cnt, uint64_t n,
                                                        uint32_t k, uint64_t* __restrict__ tile_bytes)
 {
     __shared__ uint32_t s_red[FMT_BLOCK / 64];
-    const uint64_t t0 = (uint64_t)blockIdx.x * TSV_TILE;
+    const uint64_t t0 = (uint64_t)blockIdx.x * TILE;
     uint32_t sum = 0;
-    for (uint32_t i = threadIdx.x; i < TSV_TILE; i += FMT_BLOCK)
+    for (uint32_t i = threadIdx.x; i < TILE; i += FMT_BLOCK)
         if (t0 + i < n) sum += k + 2u + dec_digits(cnt[t0 + i]);
 #pragma unroll
     for (uint32_t d = 32; d; d >>= 1) sum += __shfl_xor(sum, d, 64);
@@ -85,16 +121,17 @@ __device__ __forceinline__ void store_image(const uint8_t* img, uint32_t a, uint
     }
 }
 
+template <uint32_t TILE, uint32_t LINE>
 __global__ __launch_bounds__(FMT_BLOCK) void k_tsv_write(const uint64_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ cnt, uint64_t n,
                                                          uint32_t k, uint32_t nw,
                                                          const uint64_t* __restrict__ tile_off,
                                                          uint8_t* __restrict__ out)
 {
-    constexpr uint32_t PER = TSV_TILE / FMT_BLOCK;
-    __shared__ __align__(16) uint8_t s_img[TSV_TILE * TSV_LINE_MAX + 8];
+    constexpr uint32_t PER = TILE / FMT_BLOCK;
+    __shared__ __align__(16) uint8_t s_img[TILE * LINE + 8];
     __shared__ uint32_t s_red[FMT_BLOCK / 64];
-    const uint64_t t0 = (uint64_t)blockIdx.x * TSV_TILE;
+    const uint64_t t0 = (uint64_t)blockIdx.x * TILE;
     const uint64_t g0 = tile_off[blockIdx.x];
     const uint32_t a = (uint32_t)(g0 & 3u);
     // lines of this thread: t0 + threadIdx.x*PER + q (consecutive, so one
@@ -122,14 +159,11 @@ __global__ __launch_bounds__(FMT_BLOCK) void k_tsv_write(const uint64_t* __restr
         total += s_red[w];
     }
     uint32_t o = a + base + x - mine;
-    constexpr uint32_t ACGT = 0x54474341u;  // "ACGT", byte b = letter of code b
 #pragma unroll
     for (uint32_t q = 0; q < PER; ++q) {
         const uint64_t e = t0 + threadIdx.x * PER + q;
         if (e >= n) break;
-        uint64_t lo, hi;
-        load_key(keys, nw, e, lo, hi);
-        for (uint32_t j = 0; j < k; ++j) s_img[o + j] = (uint8_t)(ACGT >> (8u * key_base(lo, hi, k, j)));
+        put_label(s_img + o, keys, nw, e, k);
         o += k;
         s_img[o++] = '\t';
         uint32_t c = cnt[e];
@@ -157,15 +191,16 @@ __global__ __launch_bounds__(FMT_BLOCK) void k_kff_write(const uint64_t* __restr
     const uint64_t r0 = (uint64_t)blockIdx.x * FMT_BLOCK;
     const uint64_t e = r0 + threadIdx.x;
     if (e < n) {
-        uint64_t lo, hi;
-        load_key(keys, nw, e, lo, hi);
+        uint64_t lo = 0, hi = 0;
+        if (nw <= 2) load_key(keys, nw, e, lo, hi);
         uint8_t* r = s_img + threadIdx.x * rb;
         for (uint32_t b = 0; b < kb; ++b) {
             uint32_t v = 0;
 #pragma unroll
             for (uint32_t t = 0; t < 4; ++t) {
                 const uint32_t j = 4u * b + t;
-                if (j < k) v |= key_base(lo, hi, k, j) << (6u - 2u * t);
+                if (j < k)
+                    v |= (nw <= 2 ? key_base(lo, hi, k, j) : key_base_w(keys, nw, e, k, j)) << (6u - 2u * t);
             }
             r[b] = (uint8_t)v;
         }
@@ -193,8 +228,8 @@ constexpr uint64_t WRITE_CHUNK = 1ull << 21;
 int check_fmt(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n)
 {
     if (!c) return fail(TG_EINVAL, "null context");
-    if (k < 1 || k > 64) return fail(TG_EINVAL, "k must be in [1, 64] for formatting");
-    if ((k > 32 ? 2u : 1u) != c->nw && c->n_out) return fail(TG_EINVAL, "k does not match the result's key width");
+    if (k < 1 || k > 256) return fail(TG_EINVAL, "k must be in [1, 256] for formatting");
+    if ((k + 31u) / 32u != c->nw && c->n_out) return fail(TG_EINVAL, "k does not match the result's key width");
     if (first > c->n_out || n > c->n_out - first) return fail(TG_EINVAL, "entries out of range");
     return TG_OK;
 }
@@ -203,7 +238,8 @@ int check_fmt(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n)
 // total goes to *total (host, after a stream synchronisation).
 int tsv_device(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n, uint8_t** d_out, uint64_t* total)
 {
-    const uint64_t tiles = (n + tg::TSV_TILE - 1) / tg::TSV_TILE;
+    const uint64_t tile = k <= 64 ? tg::TSV_TILE : tg::TSV_TILE_W;
+    const uint64_t tiles = (n + tile - 1) / tile;
     uint64_t *tb, *toff, *tmp;
     uint8_t* out;
     int rc;
@@ -214,9 +250,15 @@ int tsv_device(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n, uint8_t** d_ou
     const uint64_t* keys = c->d_keys + first * c->nw;
     const uint32_t* cnt = c->d_cnt + first;
     if (tiles) {
-        tg::k_tsv_len<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(cnt, n, k, tb);
+        if (k <= 64) tg::k_tsv_len<tg::TSV_TILE><<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(cnt, n, k, tb);
+        else tg::k_tsv_len<tg::TSV_TILE_W><<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(cnt, n, k, tb);
         tg::launch_scan_u64(c->stream, tb, tiles, toff, tmp);
-        tg::k_tsv_write<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(keys, cnt, n, k, c->nw, toff, out);
+        if (k <= 64)
+            tg::k_tsv_write<tg::TSV_TILE, tg::TSV_LINE_MAX>
+                <<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(keys, cnt, n, k, c->nw, toff, out);
+        else
+            tg::k_tsv_write<tg::TSV_TILE_W, tg::TSV_LINE_MAX_W>
+                <<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(keys, cnt, n, k, c->nw, toff, out);
         HIPOK(hipGetLastError());
         HIPOK(hipMemcpyAsync(total, toff + tiles, 8, hipMemcpyDeviceToHost, c->stream));
     } else {
@@ -411,6 +453,9 @@ int64_t tg_result_tsv_noalloc(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n,
 {
     int rc;
     if ((rc = check_fmt(c, k, first, n))) return rc;
+    if (k > 64)
+        return fail(TG_EINVAL, "tg_result_tsv_noalloc formats k <= 64 (its fixed scratch holds lines of at "
+                               "most 76 bytes); use tg_result_tsv for k = %u", k);
     if (n > NOALLOC_TSV_MAX)
         return fail(TG_EINVAL, "tg_result_tsv_noalloc formats at most %llu lines per call (%llu asked)",
                     (unsigned long long)NOALLOC_TSV_MAX, (unsigned long long)n);
@@ -431,9 +476,10 @@ int64_t tg_result_tsv_noalloc(tg_ctx* c, uint32_t k, uint64_t first, uint64_t n,
     uint8_t* d_out = base + NOALLOC_OUT_OFF;
     uint64_t total = 0;
     if (tiles) {
-        tg::k_tsv_len<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_cnt + first, n, k, tb);
+        tg::k_tsv_len<tg::TSV_TILE><<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_cnt + first, n, k, tb);
         tg::launch_scan_u64(c->stream, tb, tiles, toff, tmp);
-        tg::k_tsv_write<<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_keys + first * c->nw,
+        tg::k_tsv_write<tg::TSV_TILE, tg::TSV_LINE_MAX>
+            <<<(unsigned)tiles, tg::FMT_BLOCK, 0, c->stream>>>(c->d_keys + first * c->nw,
                                                                            c->d_cnt + first, n, k, c->nw,
                                                                            toff, d_out);
         HIPOK(hipGetLastError());

@@ -842,7 +842,7 @@ int64_t tg_format_tsv(uint32_t k, const uint64_t* keys, const uint32_t* counts, 
                       uint64_t cap)
 {
     static const char B[4] = {'A', 'C', 'G', 'T'};
-    const uint32_t nw = k > 32 ? 2 : 1;
+    const uint32_t nw = (k + 31) / 32;  // Kmer<k> words
     uint64_t o = 0;
     for (uint64_t i = 0; i < n; ++i) {
         if (o + k + 12 > cap) return TG_ECAP;
@@ -875,7 +875,7 @@ int tg_write_kff(const char* path, uint32_t k, const uint64_t* keys, const uint3
     }
     const std::string h = tgi::kff_header(k, n);
     fwrite(h.data(), 1, h.size(), f);
-    const uint32_t kb = (k + 3) / 4, nw = k > 32 ? 2 : 1;
+    const uint32_t kb = (k + 3) / 4, nw = (k + 31) / 32;
     std::vector<uint8_t> buf;
     buf.reserve(1u << 20);
     for (uint64_t i = 0; i < n; ++i) {

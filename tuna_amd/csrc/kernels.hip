@@ -559,7 +559,8 @@ __device__ __forceinline__ uint2 sk_entry(const SkLane& L, const SkArgs& a, uint
 // so the active lanes stay converged for the one ballot in sk_push.  With fat
 // records a piece ends after a.piece k-mers (a cut that starts no new
 // superkmer).
-template <int RW, bool OL>
+// PUSH false: the superkmers are only counted (nsk), no record is staged.
+template <int RW, bool OL, bool PUSH = true>
 __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, uint32_t t,
                                         uint64_t Hc, uint32_t Pc)
 {
@@ -582,7 +583,7 @@ __device__ __forceinline__ void sk_kmer(SkLane& L, const SkArgs& a, SkStage& S, 
     L.tail = last && L.have && L.t0 >= L.tb;   // [t0p, K) is emitted after the loop
     L.flagged = L.flagged || flag;
     L.done = stop || last;
-    sk_push<RW, OL>(S, a, emit, rec);
+    if (PUSH) sk_push<RW, OL>(S, a, emit, rec);
 }
 
 // The same decision for a head strip (the strip starts at the chunk's first
@@ -903,6 +904,254 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
     else TG_SK_RW(64);
 #undef TG_SK_RW
 #undef TG_SK
+}
+
+// ============================================================ k > 64 ====
+constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;  // slot being written (count word as slot state)
+// Keys of more than 64 bases take a path of their own: the superkmers are
+// counted (the `superkmers:` total) without records, and every k-mer goes
+// straight into one open-addressed table in HBM (no routing: the keys are few
+// enough per input that k > 64 is used on, and a table slot of 4 or 8 key
+// words would leave room for few keys in LDS).
+
+// Superkmers of k > 64, counted only.  The window holds W <= 256 m-mer hashes
+// per lane in LDS ([position][lane], one wave per workgroup) and keeps their
+// values only: a cut depends on the value of the window minimum, never on its
+// position.  The decision is the general one (look-back, flags, cap) with the
+// cap of the reference's header type (2k - m <= 255: 255 bases, else 65535).
+__device__ __forceinline__ void wide_backward(uint64_t* sv, uint32_t W, uint32_t lane)
+{
+    uint64_t cur = 0;
+    for (int q = (int)W - 1; q >= 0; --q) {
+        const uint64_t v = sv[q * 64 + lane];
+        if ((uint32_t)q == W - 1u || v < cur) cur = v;
+        sv[q * 64 + lane] = cur;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_superkmers_wide(SkArgs a)
+{
+    __shared__ uint64_t s_tab[34];  // forward [0, 16), reverse [16, 32), init [32, 34)
+    extern __shared__ uint64_t s_sv[];  // [W][64]
+    sk_tables(a, s_tab, s_tab + 16, s_tab + 32);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x;
+    const uint32_t idx = blockIdx.x * 64u + lane;
+    uint32_t n = a.n;
+    if (a.n_dev) n = (uint32_t)min((uint64_t)n, *a.n_dev);
+    if (a.fix_dev) {
+        const uint32_t nf = *a.flag_count;
+        if (nf > n && idx == 0) atomicOr(a.err, 1u);
+        n = min(n, nf);
+    }
+    if (idx >= n) return;
+    const uint32_t s = a.list ? a.list[idx] : idx;
+    SkLane L;
+    sk_init(L, a, s, s_tab + 32);
+    SkStage S{};
+    const uint32_t W = a.W;
+    for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, s_tab);
+    for (uint32_t r = 0; r < W; ++r) {
+        sk_roll(L, s_tab);
+        s_sv[r * 64 + lane] = L.f ^ L.rv;
+    }
+    wide_backward(s_sv, W, lane);
+    uint32_t bs = L.ts;
+    uint64_t pv = 0;
+    while (!L.done) {
+        for (uint32_t r = 0; r < W && !L.done; ++r) {
+            const uint64_t v = s_sv[r * 64 + lane];
+            const uint64_t Hc = (r == 0 || v < pv) ? v : pv;
+            sk_kmer<2, false, false>(L, a, S, bs + r, Hc, 0u);
+            if (!L.done) {
+                sk_roll(L, s_tab);
+                const uint64_t h = L.f ^ L.rv;
+                s_sv[r * 64 + lane] = h;
+                if (r == 0 || h < pv) pv = h;
+            }
+        }
+        if (!L.done) {
+            wide_backward(s_sv, W, lane);
+            bs += W;
+        }
+    }
+    if (L.flagged) {
+        const uint32_t slot = atomicAdd(a.flag_count, 1u);
+        a.flag_list[slot] = s;
+    }
+    const uint32_t nsk = L.nsk + ((L.tail && L.t0p == L.t0) ? 1u : 0u);
+    if (nsk) atomicAdd(&a.sk_count[L.job], (unsigned long long)nsk);
+}
+
+void launch_superkmers_wide(hipStream_t st, const SkArgs& a)
+{
+    if (a.n == 0) return;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void*)k_superkmers_wide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(WIDE_WMAX * 64u * sizeof(uint64_t)));
+    });
+    const unsigned blocks = (a.n + 63u) / 64u;
+    k_superkmers_wide<<<blocks, 64, (size_t)a.W * 64u * sizeof(uint64_t), st>>>(a);
+}
+
+// k-mer i of the packed stream: base j at bits 31-2(j%16) of word j/16.
+__device__ __forceinline__ uint32_t packed_base(const uint32_t* __restrict__ packed, uint64_t j)
+{
+    return (packed[j >> 4] >> (30u - 2u * (uint32_t)(j & 15u))) & 3u;
+}
+
+// Multi-word keys (word 0 = the last 32 bases, Kmer<k>'s layout; the key ends
+// in word tw = (2k-1)/64, whose low topb = 2k - 64 tw bits it uses, and the
+// words above tw stay zero): push a base into the forward k-mer on the right,
+// its complement into the reverse complement on the left at bit rtop =
+// 2(k-1) - 64 tw of word tw.
+template <int NW>
+__device__ __forceinline__ void wide_roll(uint64_t (&f)[NW], uint64_t (&r)[NW], uint32_t b, uint32_t tw,
+                                          uint64_t topmask, uint32_t rtop)
+{
+#pragma unroll
+    for (int i = NW - 1; i > 0; --i) f[i] = (f[i] << 2) | (f[i - 1] >> 62);
+    f[0] = (f[0] << 2) | b;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) f[i] = (uint32_t)i > tw ? 0ull : ((uint32_t)i == tw ? f[i] & topmask : f[i]);
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) r[i] = (r[i] >> 2) | (r[i + 1] << 62);
+    r[NW - 1] >>= 2;
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+        if ((uint32_t)i == tw) r[i] |= (uint64_t)(b ^ 3u) << rtop;
+}
+
+template <int NW>
+__device__ __forceinline__ bool wide_le(const uint64_t (&x)[NW], const uint64_t (&y)[NW])
+{
+#pragma unroll
+    for (int i = NW - 1; i >= 0; --i)
+        if (x[i] != y[i]) return x[i] < y[i];
+    return true;
+}
+
+// Table of keys of NW words: the count word is the slot state (0 empty,
+// LOCKED32 being written, otherwise the published count), as in the spill
+// table.  A key that finds no slot within WIDE_PROBES sets *err (the host then
+// counts again with a larger table).
+constexpr uint32_t WIDE_PROBES = 4096;
+
+template <int NW>
+__device__ __forceinline__ bool wide_insert(const WideArgs& w, const uint64_t (&x)[NW])
+{
+    if (__hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) h = mix64(h ^ x[i]);
+    uint64_t slot = h & w.mask;
+    for (uint32_t p = 0; p <= w.mask && p < WIDE_PROBES;) {
+        uint32_t* cp = &w.cnt[slot];
+        const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == 0u) {
+            if (atomicCAS(cp, 0u, LOCKED32) == 0u) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i)
+                    __hip_atomic_store(&w.keys[NW * slot + i], x[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return true;
+            }
+            continue;
+        }
+        if (c == LOCKED32) continue;
+        bool eq = true;
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
+            eq = eq && __hip_atomic_load(&w.keys[NW * slot + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x[i];
+        if (eq) {
+            atomicAdd(cp, 1u);
+            return true;
+        }
+        slot = (slot + 1) & w.mask;
+        ++p;
+    }
+    atomicOr(w.err, 1u);
+    return false;
+}
+
+// One lane per strip: the strip's k-mers [tb, te) of its chunk, the first built
+// from k bases, the others rolled one base at a time.
+template <int NW>
+__global__ __launch_bounds__(256) void k_wide_insert(WideArgs w)
+{
+    const uint64_t n = min((uint64_t)w.n_strips, *w.n_dev);
+    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t c = w.strip_chunk[s];
+    const uint64_t cb = w.off[c];
+    const uint64_t K = w.off[c + 1] - cb - w.k + 1u;
+    const uint64_t tb = w.strip_tb[s];
+    const uint64_t te = min(tb + w.S, K);
+    const uint32_t tw = (2u * w.k - 1u) / 64u;
+    const uint32_t topb = 2u * w.k - 64u * tw;  // 1..64
+    const uint64_t topmask = topb >= 64u ? ~0ull : ((1ull << topb) - 1u);
+    const uint32_t rtop = topb - 2u;
+    uint64_t f[NW], r[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) f[i] = r[i] = 0;
+    uint64_t g = cb + tb;
+    for (uint32_t i = 0; i + 1u < w.k; ++i) wide_roll<NW>(f, r, packed_base(w.packed, g + i), tw, topmask, rtop);
+    g += w.k - 1u;
+    for (uint64_t t = tb; t < te; ++t, ++g) {
+        wide_roll<NW>(f, r, packed_base(w.packed, g), tw, topmask, rtop);
+        if (!wide_insert<NW>(w, wide_le<NW>(f, r) ? f : r)) return;
+    }
+}
+
+// Drain: every published slot is a distinct k-mer; those with ci <= count <=
+// cx are written out with onw = ceil(k/32) words each (the reference Kmer<k>
+// layout; the table's higher words are zero), one reservation per wave.
+// counters[0]: written, counters[1]: distinct.  out_keys == nullptr: count only.
+template <int NW>
+__global__ __launch_bounds__(256) void k_wide_drain(WideArgs w, uint32_t ci, uint64_t cx, uint32_t onw,
+                                                    uint64_t* out_keys, uint32_t* out_cnt,
+                                                    unsigned long long* counters)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t slots = w.mask + 1u;
+    uint32_t uniq = 0;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x; i0 < slots; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        const uint32_t c = i < slots ? w.cnt[i] : 0u;
+        const bool occ = c != 0u && c != LOCKED32;
+        const bool keep = occ && c >= ci && (uint64_t)c <= cx;
+        uniq += occ ? 1u : 0u;
+        const uint64_t km = __ballot(keep);
+        uint64_t base = 0;
+        if ((threadIdx.x & 63u) == 0 && km)
+            base = atomicAdd(&counters[0], (unsigned long long)__popcll(km));
+        base = __shfl(base, 0, 64);
+        if (keep && out_keys) {
+            const uint64_t o = base + lane_rank(km);
+            for (uint32_t q = 0; q < onw; ++q) out_keys[onw * o + q] = w.keys[NW * i + q];
+            out_cnt[o] = c;
+        }
+    }
+    for (uint32_t d = 32; d; d >>= 1) uniq += __shfl_xor(uniq, d, 64);
+    if ((threadIdx.x & 63u) == 0 && uniq) atomicAdd(&counters[1], (unsigned long long)uniq);
+}
+
+void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw)
+{
+    if (w.n_strips == 0) return;
+    const unsigned blocks = (unsigned)((w.n_strips + 255u) / 256u);
+    if (nw == 4) k_wide_insert<4><<<blocks, 256, 0, st>>>(w);
+    else k_wide_insert<8><<<blocks, 256, 0, st>>>(w);
+}
+
+void launch_wide_drain(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
+                       uint64_t* out_keys, uint32_t* out_cnt, unsigned long long* counters)
+{
+    uint64_t blocks = (w.mask + 1u + 255u) / 256u;
+    if (blocks > 8192) blocks = 8192;
+    if (nw == 4) k_wide_drain<4><<<(unsigned)blocks, 256, 0, st>>>(w, ci, cx, onw, out_keys, out_cnt, counters);
+    else k_wide_drain<8><<<(unsigned)blocks, 256, 0, st>>>(w, ci, cx, onw, out_keys, out_cnt, counters);
 }
 
 // =========================================================== routing ====
@@ -1485,7 +1734,6 @@ void launch_zero_regions(hipStream_t st, const ZeroRegions& z)
 }
 
 // ========================================================== counting ====
-constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 // never in a one-word table: slots hold EMPTY64 or a key below 2^64 - 2^62
 constexpr uint64_t NOMATCH64 = 0xFFFFFFFFFFFFFFFEull;
 constexpr uint32_t SPILL_PROBES = 4096;

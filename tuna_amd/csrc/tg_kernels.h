@@ -70,6 +70,23 @@ struct SkArgs {
     uint32_t* flag_count;
 };
 
+// k > 64 (keys of 4 or 8 words): the k-mers of every strip go into one
+// open-addressed HBM table; the count word is the slot state.
+constexpr uint32_t WIDE_WMAX = 256;  // m-mers per window (k - m + 1) on that path
+struct WideArgs {
+    const uint32_t* packed;
+    const uint64_t* off;        // chunk offsets (global base index), n_chunks+1
+    const uint32_t* strip_chunk;
+    const uint32_t* strip_tb;
+    uint32_t n_strips;          // capacity of the launch
+    const uint64_t* n_dev;      // live strip total
+    uint32_t S, k;
+    uint64_t* keys;             // NW words per slot
+    uint32_t* cnt;              // 0 empty, 0xFFFFFFFF being written, else the count
+    uint64_t mask;              // slots - 1 (a power of two)
+    uint32_t* err;              // set when a key found no slot
+};
+
 struct RouteArgs {
     const uint32_t* recs;      // record storage of the runs (RW words per record)
     const uint32_t* metas;     // meta word per record slot (histograms read only these)
@@ -179,6 +196,10 @@ void launch_scan_u64(hipStream_t st, const uint64_t* in, uint64_t n, uint64_t* o
 void launch_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t n, uint64_t* out);
 // rw: words per record (2 = slim, 4 or 8 = fat)
 void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw);
+void launch_superkmers_wide(hipStream_t st, const SkArgs& a);
+void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw);
+void launch_wide_drain(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
+                       uint64_t* out_keys, uint32_t* out_cnt, unsigned long long* counters);
 void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t rw);
 void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint64_t* part_sk,
                           uint64_t* part_km);

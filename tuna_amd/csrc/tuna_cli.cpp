@@ -617,10 +617,11 @@ int main(int argc, char* argv[])
         print_usage(argv[0]);
         return 1;
     }
-    if (cfg.k > 64) {
-        // src/pipeline.hpp:545-551 (unsupported combination), engine limit
-        std::cerr << "tuna: error: unsupported combination k=" << cfg.k << " m=" << cfg.l << "\n"
-                  << "  The gfx950 engine supports k in [2,64] with m in [1,min(k-1,32)].\n";
+    if (cfg.k > 64 && (cfg.partition_only || cfg.keep_tmp || cfg.debug_stats)) {
+        // k > 64 is counted without superkmer records (src/pipeline.hpp:545-551
+        // is the reference's message form for combinations a build lacks)
+        std::cerr << "tuna: error: unsupported combination k=" << cfg.k << " with -tp, -kt or -dbg\n"
+                  << "  Partition files and debug statistics are built for k in [2,64].\n";
         return 1;
     }
     if (cfg.num_partitions == 0) {
