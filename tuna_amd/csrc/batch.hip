@@ -65,7 +65,7 @@ struct BatchBufs {
     uint64_t cap_words = 0;
     uint64_t *boff, *nstr, *nkm, *nstr_x, *nkm_x, *scan_tmp;
     uint32_t *chunk_job, *strip_chunk, *strip_tb, *flag_list, *run_start, *run_cnt, *flags;
-    uint64_t *strip_slot, *run_cnt64, *run_off;
+    uint64_t *strip_slot, *strip_long, *run_cnt64, *run_off;
     uint32_t *recs, *sorted, *route_tmp;  // rw words per record
     uint32_t* route_meta;                 // meta word of every coarse-ordered record
     uint64_t* split_off;                  // fine scatter's split-group starts
@@ -212,6 +212,7 @@ int alloc_class(BatchState* s, const BatchClass& c)
         (rc = balloc(s, &b.scan_tmp, scan_words)) || (rc = balloc(s, &b.chunk_job, nch1)) ||
         (rc = balloc(s, &b.strip_chunk, ns1)) || (rc = balloc(s, &b.strip_tb, ns1)) ||
         (rc = balloc(s, &b.strip_slot, ns1)) || (rc = balloc(s, &b.flag_list, ns1)) ||
+        (rc = balloc(s, &b.strip_long, tg::fill_list_words(c.cap_strips))) ||
         (rc = balloc(s, &b.flags, F_WORDS)) || (rc = balloc(s, &b.run_start, b.n_runs + 1)) ||
         (rc = balloc(s, &b.run_cnt, b.n_runs + 1)) || (rc = balloc(s, &b.run_cnt64, b.n_runs + 1)) ||
         (rc = balloc(s, &b.run_off, b.n_runs + 2)) ||
@@ -273,11 +274,11 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
     }
     {
         Prof pr(prof, "strips");
-        tg::launch_chunk_counts(st, b.boff, c.cap_ch, c.k, STRIP, b.nstr, b.nkm);
+        tg::launch_chunk_counts(st, b.boff, c.cap_ch, c.k, STRIP, b.nstr, b.nkm, b.strip_long);
         tg::launch_scan_u64(st, b.nstr, c.cap_ch, b.nstr_x, b.scan_tmp);
         tg::launch_scan_u64(st, b.nkm, c.cap_ch, b.nkm_x, b.scan_tmp);
         tg::launch_fill_strips(st, c.cap_ch, b.nstr_x, b.nkm_x, b.nkm, STRIP, b.strip_chunk, b.strip_tb,
-                               b.strip_slot);
+                               b.strip_slot, b.strip_long, tg::fill_list_words(c.cap_strips));
     }
     // superkmers: main pass over the strip capacity, fix-up over FIXB blocks
     tg::SkArgs a{};

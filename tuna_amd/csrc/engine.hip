@@ -68,14 +68,17 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         tg::launch_pack(st, d_bases, N, o.packed, n_words);
     }
     // -- strips
-    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp;
+    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp, *strip_long;
+    // strips: at most N/S + one per chunk
+    const uint64_t long_words = tg::fill_list_words(N / o.S + nch + 1);
     if ((rc = ensure_t(c, "nstr", nch + 1, &nstr)) || (rc = ensure_t(c, "nkm", nch + 1, &nkm)) ||
         (rc = ensure_t(c, "nstr_x", nch + 1, &nstr_x)) || (rc = ensure_t(c, "nkm_x", nch + 1, &nkm_x)) ||
-        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)))
+        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)) ||
+        (rc = ensure_t(c, "strip_long", long_words, &strip_long)))
         return rc;
     {
         Prof pr(c, "strips");
-        tg::launch_chunk_counts(st, d_off, nch, o.k, o.S, nstr, nkm);
+        tg::launch_chunk_counts(st, d_off, nch, o.k, o.S, nstr, nkm, strip_long);
         tg::launch_scan_u64(st, nstr, nch, nstr_x, tmp);
         tg::launch_scan_u64(st, nkm, nch, nkm_x, tmp);
     }
@@ -100,7 +103,8 @@ int run_phase1(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         return rc;
     {
         Prof pr(c, "strips");
-        tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, strip_slot);
+        tg::launch_fill_strips(st, nch, nstr_x, nkm_x, nkm, o.S, strip_chunk, strip_tb, strip_slot, strip_long,
+                               long_words);
     }
     // record space: the slot space of the strips (a strip of K k-mers has at
     // most K records), then fix-up regions of SK_BLOCK*S records per block
@@ -668,12 +672,12 @@ int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
 }
 
 // k > 64: the superkmers are counted (main pass, then the flagged strips with
-// full look-back), every k-mer goes into one open-addressed HBM table of keys
-// of 4 or 8 words (power of two, about 1.5 slots per k-mer, doubled while a
-// key finds no slot and memory allows), and the table is drained through the
-// ci/cx filter twice: once to size the result, once to write it with
-// ceil(k/32) words per key.  Indices are 64-bit, so one pass takes the whole
-// job.
+// full look-back), every k-mer goes into one open-addressed HBM table whose
+// slots hold a key's position in the packed stream and its count (a power of
+// two of at least 1.5 slots per k-mer, doubled while a key finds no slot and
+// memory allows), and the table is drained through the ci/cx filter: per-block
+// totals, their scan, then ceil(k/32) words per written key in slot order.
+// Indices are 64-bit, so one pass takes the whole job.
 int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
                uint64_t nch, tg_stats* st)
 {
@@ -681,6 +685,7 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     if (rc) return rc;
     const uint32_t k = p->k, m = p->m, W = k - m + 1;
     if (W > tg::WIDE_WMAX) return fail(TG_EINVAL, "k - m + 1 = %u exceeds %u", W, tg::WIDE_WMAX);
+    if (N >= (1ull << 40) - 1u) return fail(TG_EINVAL, "k > 64 counts inputs of fewer than 2^40 bases");
     HIPOK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const uint32_t nw = k <= 128 ? 4u : 8u, onw = (k + 31) / 32;
@@ -693,11 +698,14 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     const uint32_t S = 2048;
     const uint64_t n_words = (N + 15) / 16 + 8;
     uint32_t* packed;
-    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp;
+    const uint32_t SI = 256;  // strips of the insert (below)
+    uint64_t *nstr, *nkm, *nstr_x, *nkm_x, *tmp, *strip_long;
+    const uint64_t long_words = tg::fill_list_words(N / std::min(S, SI) + nch + 1);
     if ((rc = ensure_t(c, "packed", n_words, &packed)) || (rc = ensure_t(c, "nstr", nch + 1, &nstr)) ||
         (rc = ensure_t(c, "nkm", nch + 1, &nkm)) || (rc = ensure_t(c, "nstr_x", nch + 1, &nstr_x)) ||
         (rc = ensure_t(c, "nkm_x", nch + 1, &nkm_x)) ||
-        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)))
+        (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nch + 1) + 64, &tmp)) ||
+        (rc = ensure_t(c, "strip_long", long_words, &strip_long)))
         return rc;
     {
         Prof pr(c, "pack");
@@ -705,7 +713,7 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     }
     {
         Prof pr(c, "strips");
-        tg::launch_chunk_counts(s, d_off, nch, k, S, nstr, nkm);
+        tg::launch_chunk_counts(s, d_off, nch, k, S, nstr, nkm, strip_long);
         tg::launch_scan_u64(s, nstr, nch, nstr_x, tmp);
         tg::launch_scan_u64(s, nkm, nch, nkm_x, tmp);
     }
@@ -726,7 +734,8 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         return rc;
     {
         Prof pr(c, "strips");
-        tg::launch_fill_strips(s, nch, nstr_x, nkm_x, nkm, S, strip_chunk, strip_tb, strip_slot);
+        tg::launch_fill_strips(s, nch, nstr_x, nkm_x, nkm, S, strip_chunk, strip_tb, strip_slot, strip_long,
+                               long_words);
         tg::launch_fill_u32(s, flags, 0u, 64);
         tg::launch_fill_u64(s, (uint64_t*)sk_count, 0ull, 1);
     }
@@ -769,40 +778,62 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         tg::launch_superkmers_wide(s, a);
     }
     HIPOK(hipEventRecord(c->ev_b, s));
-    // the k-mer table
+    // strips of the insert: 256 k-mers, so that a long chunk gives many lanes
+    // (a lane's k-mers are one chain of dependent table probes)
+    uint64_t n_ins = n_strips;
+    if (SI != S) {
+        {
+            Prof pr(c, "strips");
+            tg::launch_chunk_counts(s, d_off, nch, k, SI, nstr, nkm, strip_long);
+            tg::launch_scan_u64(s, nstr, nch, nstr_x, tmp);
+        }
+        HIPOK(hipGetLastError());
+        HIPOK(hipMemcpyAsync(&n_ins, nstr_x + nch, 8, hipMemcpyDeviceToHost, s));
+        HIPOK(hipStreamSynchronize(s));
+        if (n_ins >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
+        if ((rc = ensure_t(c, "strip_chunk", n_ins + 1, &strip_chunk)) ||
+            (rc = ensure_t(c, "strip_tb", n_ins + 1, &strip_tb)) ||
+            (rc = ensure_t(c, "strip_slot", n_ins + 1, &strip_slot)))
+            return rc;
+        Prof pr(c, "strips");
+        tg::launch_fill_strips(s, nch, nstr_x, nkm_x, nkm, SI, strip_chunk, strip_tb, strip_slot, strip_long,
+                               long_words);
+    }
+    // the k-mer table: 16-byte slots {tag, count}, a power of two of at least
+    // 1.5 slots per k-mer
     size_t free_b = 0, total_b = 0;
     HIPOK(hipMemGetInfo(&free_b, &total_b));
-    const uint64_t slot_bytes = 8ull * nw + 4ull;
+    constexpr uint64_t slot_bytes = 16;
     uint64_t slots = 1ull << 16;
     while (slots < n_kmers + n_kmers / 2) slots <<= 1;
-    uint64_t* tkeys = nullptr;
-    uint32_t* tcnt = nullptr;
-    unsigned long long* ctr = nullptr;
+    uint64_t* tab = nullptr;
+    uint64_t *blk, *blk_x;
     for (int attempt = 0;; ++attempt) {
         // keep a quarter of the free memory for the result
-        if (slots * slot_bytes > free_b / 4 * 3 + c->bufs["wide_keys"].cap + c->bufs["wide_cnt"].cap)
+        if (slots * slot_bytes > free_b / 4 * 3 + c->bufs["wide_tab"].cap)
             return fail(TG_ENOMEM, "the k-mer table for k = %u needs %llu slots of %llu bytes; device memory "
                                    "holds fewer", k, (unsigned long long)slots, (unsigned long long)slot_bytes);
-        if ((rc = ensure_t(c, "wide_keys", slots * nw, &tkeys)) || (rc = ensure_t(c, "wide_cnt", slots, &tcnt)) ||
-            (rc = ensure_t(c, "wide_ctr", 4, &ctr)))
+        const uint32_t nb = tg::wide_drain_blocks(slots);
+        if ((rc = ensure_t(c, "wide_tab", 2 * slots, &tab)) || (rc = ensure_t(c, "wide_blk", 2 * nb, &blk)) ||
+            (rc = ensure_t(c, "wide_blk_x", 2 * (nb + 1), &blk_x)) ||
+            (rc = ensure_t(c, "scan_tmp", tg::scan_tmp_words(nb + 1) + 64, &tmp)))
             return rc;
-        tg::launch_fill_u32(s, tcnt, 0u, slots);
-        tg::launch_fill_u32(s, flags + 8, 0u, 1);
         tg::WideArgs w{};
         w.packed = packed;
         w.off = d_off;
         w.strip_chunk = strip_chunk;
         w.strip_tb = strip_tb;
-        w.n_strips = (uint32_t)n_strips;
+        w.n_strips = (uint32_t)n_ins;
         w.n_dev = nstr_x + nch;
-        w.S = S;
+        w.S = SI;
         w.k = k;
-        w.keys = tkeys;
-        w.cnt = tcnt;
+        w.tab = tab;
         w.mask = slots - 1;
         w.err = flags + 8;
         {
             Prof pr(c, "count");
+            tg::launch_fill_u64(s, tab, 0ull, 2 * slots);
+            tg::launch_fill_u32(s, flags + 8, 0u, 1);
             tg::launch_wide_insert(s, w, nw);
         }
         HIPOK(hipGetLastError());
@@ -814,11 +845,17 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
             slots <<= 1;
             continue;
         }
-        // drain: size, then write
-        unsigned long long hc[2] = {0, 0};
-        tg::launch_fill_u64(s, (uint64_t*)ctr, 0ull, 4);
-        tg::launch_wide_drain(s, w, nw, p->ci, p->cx, onw, nullptr, nullptr, ctr);
-        HIPOK(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
+        // drain: per-block totals (written, distinct), their scans, the keys
+        uint64_t hc[2] = {0, 0};
+        {
+            Prof pr(c, "drain");
+            tg::launch_wide_drain_count(s, w, p->ci, p->cx, blk, blk + nb);
+            tg::launch_scan_u64(s, blk, nb, blk_x, tmp);
+            tg::launch_scan_u64(s, blk + nb, nb, blk_x + nb + 1, tmp);
+        }
+        HIPOK(hipGetLastError());
+        HIPOK(hipMemcpyAsync(&hc[0], blk_x + nb, 8, hipMemcpyDeviceToHost, s));
+        HIPOK(hipMemcpyAsync(&hc[1], blk_x + 2 * nb + 1, 8, hipMemcpyDeviceToHost, s));
         HIPOK(hipStreamSynchronize(s));
         uint64_t* out_keys;
         uint32_t* out_cnt;
@@ -827,8 +864,7 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
             return rc;
         {
             Prof pr(c, "job_gather");
-            tg::launch_fill_u64(s, (uint64_t*)ctr, 0ull, 4);
-            tg::launch_wide_drain(s, w, nw, p->ci, p->cx, onw, out_keys, out_cnt, ctr);
+            tg::launch_wide_drain_write(s, w, nw, p->ci, p->cx, onw, blk_x, out_keys, out_cnt);
         }
         HIPOK(hipGetLastError());
         unsigned long long nsk = 0;

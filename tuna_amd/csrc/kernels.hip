@@ -83,8 +83,10 @@ void launch_pack(hipStream_t st, const uint8_t* bases, uint64_t n, uint32_t* out
 
 // ============================================================ strips ====
 __global__ void k_chunk_counts(const uint64_t* __restrict__ off, uint64_t nch, uint32_t k,
-                               uint32_t S, uint64_t* __restrict__ nstr, uint64_t* __restrict__ nkm)
+                               uint32_t S, uint64_t* __restrict__ nstr, uint64_t* __restrict__ nkm,
+                               unsigned long long* __restrict__ long_list)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) long_list[0] = 0;  // the strip fill's piece count
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nch; c += stride) {
         // a decreasing offset pair is an empty chunk (keeps every later index in bounds)
@@ -96,42 +98,89 @@ __global__ void k_chunk_counts(const uint64_t* __restrict__ off, uint64_t nch, u
 }
 
 void launch_chunk_counts(hipStream_t st, const uint64_t* off, uint64_t n_chunks, uint32_t k,
-                         uint32_t S, uint64_t* nstr, uint64_t* nkm)
+                         uint32_t S, uint64_t* nstr, uint64_t* nkm, uint64_t* long_list)
 {
     uint64_t blocks = (n_chunks + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    k_chunk_counts<<<(unsigned)blocks, 256, 0, st>>>(off, n_chunks, k, S, nstr, nkm);
+    k_chunk_counts<<<(unsigned)blocks, 256, 0, st>>>(off, n_chunks, k, S, nstr, nkm,
+                                                      reinterpret_cast<unsigned long long*>(long_list));
 }
+
+// Strip table: a thread per chunk writes the strips of a chunk of at most
+// FILL_SHORT strips; a longer chunk is listed in pieces of FILL_PIECE strips
+// (long_list[1..], their number in long_list[0]) and every piece is written by
+// a workgroup of k_fill_strips_long, so one long chunk (a genome) is not
+// written by a single thread.  Pieces per strip: at most 1/33 + 1/4096 < 1/32.
+constexpr uint32_t FILL_SHORT = 32, FILL_PIECE = 4096;
+
+uint64_t fill_list_words(uint64_t cap_strips) { return cap_strips / FILL_SHORT + 2; }
 
 __global__ void k_fill_strips(uint64_t nch, const uint64_t* __restrict__ nstr_excl,
                               const uint64_t* __restrict__ nkm_excl,
                               const uint64_t* __restrict__ nkm, uint32_t S,
                               uint32_t* __restrict__ strip_chunk, uint32_t* __restrict__ strip_tb,
-                              uint64_t* __restrict__ strip_slot)
+                              uint64_t* __restrict__ strip_slot, unsigned long long* __restrict__ long_list,
+                              uint64_t long_cap)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nch; c += stride) {
         const uint64_t K = nkm[c];
         const uint64_t base = nstr_excl[c], kb = nkm_excl[c];
-        uint64_t i = 0;
-        for (uint64_t t = 0; t < K; t += S, ++i) {
+        const uint64_t ns = (K + S - 1) / S;
+        if (ns > FILL_SHORT) {
+            const uint64_t pieces = (ns + FILL_PIECE - 1) / FILL_PIECE;
+            const uint64_t j = atomicAdd(&long_list[0], (unsigned long long)pieces);
+            for (uint64_t q = 0; q < pieces && j + q < long_cap; ++q) long_list[1 + j + q] = c | (q << 32);
+            continue;
+        }
+        for (uint64_t i = 0; i < ns; ++i) {
             strip_chunk[base + i] = (uint32_t)c;
-            strip_tb[base + i] = (uint32_t)t;
-            strip_slot[base + i] = kb + t;
+            strip_tb[base + i] = (uint32_t)(i * S);
+            strip_slot[base + i] = kb + i * S;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_strips_long(const uint64_t* __restrict__ nstr_excl,
+                                                          const uint64_t* __restrict__ nkm_excl,
+                                                          const uint64_t* __restrict__ nkm, uint32_t S,
+                                                          uint32_t* __restrict__ strip_chunk,
+                                                          uint32_t* __restrict__ strip_tb,
+                                                          uint64_t* __restrict__ strip_slot,
+                                                          const unsigned long long* __restrict__ long_list,
+                                                          uint64_t long_cap)
+{
+    const uint64_t n = min((uint64_t)long_list[0], long_cap);
+    for (uint64_t e = blockIdx.x; e < n; e += gridDim.x) {
+        const uint64_t v = long_list[1 + e];
+        const uint64_t c = v & 0xFFFFFFFFull, q = v >> 32;
+        const uint64_t ns = (nkm[c] + S - 1) / S;
+        const uint64_t base = nstr_excl[c], kb = nkm_excl[c];
+        const uint64_t i1 = min(ns, (q + 1) * FILL_PIECE);
+        for (uint64_t i = q * FILL_PIECE + threadIdx.x; i < i1; i += 256u) {
+            strip_chunk[base + i] = (uint32_t)c;
+            strip_tb[base + i] = (uint32_t)(i * S);
+            strip_slot[base + i] = kb + i * S;
         }
     }
 }
 
 void launch_fill_strips(hipStream_t st, uint64_t n_chunks, const uint64_t* nstr_excl,
                         const uint64_t* nkm_excl, const uint64_t* nkm, uint32_t S,
-                        uint32_t* strip_chunk, uint32_t* strip_tb, uint64_t* strip_slot)
+                        uint32_t* strip_chunk, uint32_t* strip_tb, uint64_t* strip_slot,
+                        uint64_t* long_list, uint64_t list_words)
 {
     uint64_t blocks = (n_chunks + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    k_fill_strips<<<(unsigned)blocks, 256, 0, st>>>(n_chunks, nstr_excl, nkm_excl, nkm, S,
-                                                     strip_chunk, strip_tb, strip_slot);
+    auto* ll = reinterpret_cast<unsigned long long*>(long_list);
+    const uint64_t cap = list_words - 1;
+    k_fill_strips<<<(unsigned)blocks, 256, 0, st>>>(n_chunks, nstr_excl, nkm_excl, nkm, S, strip_chunk,
+                                                     strip_tb, strip_slot, ll, cap);
+    const unsigned lb = (unsigned)std::min<uint64_t>(std::max<uint64_t>(cap, 1u), 1024u);
+    k_fill_strips_long<<<lb, 256, 0, st>>>(nstr_excl, nkm_excl, nkm, S, strip_chunk, strip_tb, strip_slot, ll,
+                                            cap);
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n);
@@ -906,13 +955,15 @@ void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw)
 #undef TG_SK
 }
 
+// Count word of the HBM spill and merge tables as slot state: being written.
+constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
+
 // ============================================================ k > 64 ====
-constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;  // slot being written (count word as slot state)
 // Keys of more than 64 bases take a path of their own: the superkmers are
 // counted (the `superkmers:` total) without records, and every k-mer goes
-// straight into one open-addressed table in HBM (no routing: the keys are few
-// enough per input that k > 64 is used on, and a table slot of 4 or 8 key
-// words would leave room for few keys in LDS).
+// straight into one open-addressed table in HBM that stores each key's
+// position in the packed stream (no routing: a key of 4 or 8 words would leave
+// room for few keys in an LDS table).
 
 // Superkmers of k > 64, counted only.  The window holds W <= 256 m-mer hashes
 // per lane in LDS ([position][lane], one wave per workgroup) and keeps their
@@ -995,10 +1046,21 @@ void launch_superkmers_wide(hipStream_t st, const SkArgs& a)
     k_superkmers_wide<<<blocks, 64, (size_t)a.W * 64u * sizeof(uint64_t), st>>>(a);
 }
 
-// k-mer i of the packed stream: base j at bits 31-2(j%16) of word j/16.
+// Base j of the packed stream sits at bits 31-2(j%16) of word j/16.
 __device__ __forceinline__ uint32_t packed_base(const uint32_t* __restrict__ packed, uint64_t j)
 {
     return (packed[j >> 4] >> (30u - 2u * (uint32_t)(j & 15u))) & 3u;
+}
+
+// The 32 bases from base s of the packed stream, base s in bits 63:62.  Reads
+// words s/16 .. s/16+2 (the stream carries 8 words of padding past its end).
+__device__ __forceinline__ uint64_t packed32(const uint32_t* __restrict__ packed, uint64_t s)
+{
+    const uint64_t q = s >> 4;
+    const uint32_t sh = 2u * (uint32_t)(s & 15u);
+    const uint64_t hi = ((uint64_t)packed[q] << 32) | packed[q + 1];
+    const uint64_t lo = packed[q + 2];
+    return (hi << sh) | ((lo << sh) >> 32);
 }
 
 // Multi-word keys (word 0 = the last 32 bases, Kmer<k>'s layout; the key ends
@@ -1023,6 +1085,50 @@ __device__ __forceinline__ void wide_roll(uint64_t (&f)[NW], uint64_t (&r)[NW], 
         if ((uint32_t)i == tw) r[i] |= (uint64_t)(b ^ 3u) << rtop;
 }
 
+// The forward k-mer whose first base is base p of the packed stream, in the
+// key layout: word w < tw holds the 32 bases from p + k - 32 - 32w, word tw
+// the first k - 32 tw bases, right-aligned.
+template <int NW>
+__device__ __forceinline__ void wide_key_at(const uint32_t* __restrict__ packed, uint64_t p, uint32_t k,
+                                            uint32_t tw, uint64_t (&y)[NW])
+{
+    const uint32_t nb = k - 32u * tw;  // 1..32
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if ((uint32_t)w < tw) y[w] = packed32(packed, p + k - 32u - 32u * (uint32_t)w);
+        else if ((uint32_t)w == tw) y[w] = packed32(packed, p) >> (64u - 2u * nb);
+        else y[w] = 0;
+    }
+}
+
+// Reverse complement of a key: the 2-bit digits of the 64 NW-bit value are
+// mirrored (words swapped, digits reversed inside each word), the value moved
+// down by 64 NW - 2k bits and the k digits complemented.
+template <int NW>
+__device__ __forceinline__ void wide_rc(const uint64_t (&x)[NW], uint32_t k, uint32_t tw, uint64_t topmask,
+                                        uint64_t (&y)[NW])
+{
+    uint64_t t[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        uint64_t v = __builtin_bitreverse64(x[w]);
+        v = ((v >> 1) & 0x5555555555555555ull) | ((v & 0x5555555555555555ull) << 1);
+        t[NW - 1 - w] = v;
+    }
+    const uint32_t D = 64u * NW - 2u * k, dw = D >> 6, db = D & 63u;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        uint64_t a = 0, b = 0;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+            if ((uint32_t)j == (uint32_t)w + dw) a = t[j];
+            if ((uint32_t)j == (uint32_t)w + dw + 1u) b = t[j];
+        }
+        const uint64_t v = (a >> db) | (db ? (b << (64u - db)) : 0ull);
+        y[w] = (uint32_t)w < tw ? ~v : ((uint32_t)w == tw ? (~v & topmask) : 0ull);
+    }
+}
+
 template <int NW>
 __device__ __forceinline__ bool wide_le(const uint64_t (&x)[NW], const uint64_t (&y)[NW])
 {
@@ -1032,51 +1138,73 @@ __device__ __forceinline__ bool wide_le(const uint64_t (&x)[NW], const uint64_t 
     return true;
 }
 
-// Table of keys of NW words: the count word is the slot state (0 empty,
-// LOCKED32 being written, otherwise the published count), as in the spill
-// table.  A key that finds no slot within WIDE_PROBES sets *err (the host then
-// counts again with a larger table).
+template <int NW>
+__device__ __forceinline__ bool wide_eq(const uint64_t (&x)[NW], const uint64_t (&y)[NW])
+{
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) e = e && x[i] == y[i];
+    return e;
+}
+
+// The k-mer table holds where a key is, not the key: slot = {tag, count}, two
+// words.  tag = fingerprint (bits 63:41, the top of the key's hash) | bit 40
+// (the canonical key is the reverse complement of the bases at the position)
+// | position + 1 (bits 39:0, base index of the k-mer's first base in the
+// packed stream).  A tag is claimed with one 64-bit compare-and-swap and never
+// changes afterwards; a key is compared with a slot's by reading the slot's
+// bases from the packed stream, which no kernel writes while the table is
+// filled, and counts are added with relaxed atomics.  No slot state is
+// published in parts, so neither side needs an acquire or release.
+constexpr uint32_t WIDE_POS_BITS = 40;
+constexpr uint64_t WIDE_POS_MASK = (1ull << WIDE_POS_BITS) - 1ull;
+constexpr uint64_t WIDE_FP_MASK = ~((1ull << (WIDE_POS_BITS + 1)) - 1ull);
+// A key that finds no slot within WIDE_PROBES sets *err (the host then counts
+// again with a table twice as large).
 constexpr uint32_t WIDE_PROBES = 4096;
 
+struct __align__(16) WideSlot {
+    uint64_t tag;
+    uint64_t cnt;  // low 32 bits: the count
+};
+
 template <int NW>
-__device__ __forceinline__ bool wide_insert(const WideArgs& w, const uint64_t (&x)[NW])
+__device__ __forceinline__ bool wide_insert(const WideArgs& w, const uint64_t (&c)[NW], const uint64_t (&o)[NW],
+                                            uint64_t pos, uint32_t orient, uint32_t tw)
 {
-    if (__hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
-    for (int i = 0; i < NW; ++i) h = mix64(h ^ x[i]);
+    for (int i = 0; i < NW; ++i) h = mix64(h ^ c[i]);
+    const uint64_t fp = h & WIDE_FP_MASK;
+    const uint64_t mine = fp | ((uint64_t)orient << WIDE_POS_BITS) | (pos + 1u);
     uint64_t slot = h & w.mask;
-    for (uint32_t p = 0; p <= w.mask && p < WIDE_PROBES;) {
-        uint32_t* cp = &w.cnt[slot];
-        const uint32_t c = __hip_atomic_load(cp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (c == 0u) {
-            if (atomicCAS(cp, 0u, LOCKED32) == 0u) {
-#pragma unroll
-                for (int i = 0; i < NW; ++i)
-                    __hip_atomic_store(&w.keys[NW * slot + i], x[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(cp, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t p = 0; p <= w.mask && p < WIDE_PROBES; ++p) {
+        uint64_t* tp = w.tab + 2u * slot;
+        uint64_t v = __hip_atomic_load(tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v == 0u) {
+            v = atomicCAS((unsigned long long*)tp, 0ull, (unsigned long long)mine);
+            if (v == 0u) {
+                atomicAdd(reinterpret_cast<uint32_t*>(tp + 1), 1u);
                 return true;
             }
-            continue;
         }
-        if (c == LOCKED32) continue;
-        bool eq = true;
-#pragma unroll
-        for (int i = 0; i < NW; ++i)
-            eq = eq && __hip_atomic_load(&w.keys[NW * slot + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x[i];
-        if (eq) {
-            atomicAdd(cp, 1u);
-            return true;
+        if (((v ^ fp) & WIDE_FP_MASK) == 0u) {
+            uint64_t y[NW];
+            wide_key_at<NW>(w.packed, (v & WIDE_POS_MASK) - 1u, w.k, tw, y);
+            if ((v >> WIDE_POS_BITS) & 1u ? wide_eq<NW>(y, o) : wide_eq<NW>(y, c)) {
+                atomicAdd(reinterpret_cast<uint32_t*>(tp + 1), 1u);
+                return true;
+            }
         }
-        slot = (slot + 1) & w.mask;
-        ++p;
+        slot = (slot + 1u) & w.mask;
     }
     atomicOr(w.err, 1u);
     return false;
 }
 
-// One lane per strip: the strip's k-mers [tb, te) of its chunk, the first built
-// from k bases, the others rolled one base at a time.
+// One lane per strip: the strip's k-mers [tb, te) of its chunk, the first read
+// from the packed stream (its reverse complement computed), the others rolled
+// one base at a time.
 template <int NW>
 __global__ __launch_bounds__(256) void k_wide_insert(WideArgs w)
 {
@@ -1093,48 +1221,93 @@ __global__ __launch_bounds__(256) void k_wide_insert(WideArgs w)
     const uint64_t topmask = topb >= 64u ? ~0ull : ((1ull << topb) - 1u);
     const uint32_t rtop = topb - 2u;
     uint64_t f[NW], r[NW];
-#pragma unroll
-    for (int i = 0; i < NW; ++i) f[i] = r[i] = 0;
-    uint64_t g = cb + tb;
-    for (uint32_t i = 0; i + 1u < w.k; ++i) wide_roll<NW>(f, r, packed_base(w.packed, g + i), tw, topmask, rtop);
-    g += w.k - 1u;
+    uint64_t g = cb + tb;  // first base of k-mer t
+    wide_key_at<NW>(w.packed, g, w.k, tw, f);
+    wide_rc<NW>(f, w.k, tw, topmask, r);
     for (uint64_t t = tb; t < te; ++t, ++g) {
-        wide_roll<NW>(f, r, packed_base(w.packed, g), tw, topmask, rtop);
-        if (!wide_insert<NW>(w, wide_le<NW>(f, r) ? f : r)) return;
+        if (t != tb) wide_roll<NW>(f, r, packed_base(w.packed, g + w.k - 1u), tw, topmask, rtop);
+        if (((t - tb) & 31u) == 0u && __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        const bool fwd = wide_le<NW>(f, r);
+        const bool ok = fwd ? wide_insert<NW>(w, f, r, g, 0u, tw) : wide_insert<NW>(w, r, f, g, 1u, tw);
+        if (!ok) return;
     }
 }
 
-// Drain: every published slot is a distinct k-mer; those with ci <= count <=
-// cx are written out with onw = ceil(k/32) words each (the reference Kmer<k>
-// layout; the table's higher words are zero), one reservation per wave.
-// counters[0]: written, counters[1]: distinct.  out_keys == nullptr: count only.
-template <int NW>
-__global__ __launch_bounds__(256) void k_wide_drain(WideArgs w, uint32_t ci, uint64_t cx, uint32_t onw,
-                                                    uint64_t* out_keys, uint32_t* out_cnt,
-                                                    unsigned long long* counters)
+// Drain, two passes over equal block ranges of the table (slot order, so the
+// result order is fixed): per-block totals of written (ci <= count <= cx) and
+// distinct keys, then, at the scanned block offsets, every written key with
+// onw = ceil(k/32) words (the reference Kmer<k> layout) and its count.
+__global__ __launch_bounds__(256) void k_wide_drain_count(WideArgs w, uint64_t per_block, uint32_t ci, uint64_t cx,
+                                                          uint64_t* __restrict__ blk_keep,
+                                                          uint64_t* __restrict__ blk_occ)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t slots = w.mask + 1u;
-    uint32_t uniq = 0;
-    for (uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x; i0 < slots; i0 += stride) {
-        const uint64_t i = i0 + threadIdx.x;
-        const uint32_t c = i < slots ? w.cnt[i] : 0u;
-        const bool occ = c != 0u && c != LOCKED32;
-        const bool keep = occ && c >= ci && (uint64_t)c <= cx;
-        uniq += occ ? 1u : 0u;
+    __shared__ uint32_t s_k[4], s_o[4];
+    const WideSlot* tab = reinterpret_cast<const WideSlot*>(w.tab);
+    const uint64_t b0 = blockIdx.x * per_block;
+    uint32_t keep = 0, occ = 0;
+    for (uint64_t i = b0 + threadIdx.x; i < b0 + per_block; i += 256u) {
+        const WideSlot e = tab[i];
+        const uint32_t c = (uint32_t)e.cnt;
+        occ += e.tag != 0u ? 1u : 0u;
+        keep += (e.tag != 0u && c >= ci && (uint64_t)c <= cx) ? 1u : 0u;
+    }
+    for (uint32_t d = 32; d; d >>= 1) {
+        keep += __shfl_xor(keep, d, 64);
+        occ += __shfl_xor(occ, d, 64);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        s_k[threadIdx.x >> 6] = keep;
+        s_o[threadIdx.x >> 6] = occ;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        blk_keep[blockIdx.x] = (uint64_t)s_k[0] + s_k[1] + s_k[2] + s_k[3];
+        blk_occ[blockIdx.x] = (uint64_t)s_o[0] + s_o[1] + s_o[2] + s_o[3];
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_wide_drain_write(WideArgs w, uint64_t per_block, uint32_t ci, uint64_t cx,
+                                                          uint32_t onw, const uint64_t* __restrict__ blk_off,
+                                                          uint64_t* __restrict__ out_keys,
+                                                          uint32_t* __restrict__ out_cnt)
+{
+    __shared__ uint32_t s_w[4];
+    const WideSlot* tab = reinterpret_cast<const WideSlot*>(w.tab);
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t tw = (2u * w.k - 1u) / 64u;
+    const uint32_t topb = 2u * w.k - 64u * tw;
+    const uint64_t topmask = topb >= 64u ? ~0ull : ((1ull << topb) - 1u);
+    uint64_t base = blk_off[blockIdx.x];
+    const uint64_t b0 = blockIdx.x * per_block;
+    for (uint64_t i0 = b0; i0 < b0 + per_block; i0 += 256u) {
+        const WideSlot e = tab[i0 + threadIdx.x];
+        const uint32_t c = (uint32_t)e.cnt;
+        const bool keep = e.tag != 0u && c >= ci && (uint64_t)c <= cx;
         const uint64_t km = __ballot(keep);
-        uint64_t base = 0;
-        if ((threadIdx.x & 63u) == 0 && km)
-            base = atomicAdd(&counters[0], (unsigned long long)__popcll(km));
-        base = __shfl(base, 0, 64);
-        if (keep && out_keys) {
-            const uint64_t o = base + lane_rank(km);
-            for (uint32_t q = 0; q < onw; ++q) out_keys[onw * o + q] = w.keys[NW * i + q];
+        if (lane == 0) s_w[wv] = (uint32_t)__popcll(km);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t nq = s_w[q];
+            before += q < wv ? nq : 0u;
+            tot += nq;
+        }
+        __syncthreads();
+        if (keep) {
+            const uint64_t o = base + before + lane_rank(km);
+            uint64_t y[NW], z[NW];
+            wide_key_at<NW>(w.packed, (e.tag & WIDE_POS_MASK) - 1u, w.k, tw, y);
+            wide_rc<NW>(y, w.k, tw, topmask, z);
+            const bool rcf = (e.tag >> WIDE_POS_BITS) & 1u;
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+                if ((uint32_t)q < onw) out_keys[onw * o + q] = rcf ? z[q] : y[q];
             out_cnt[o] = c;
         }
+        base += tot;
     }
-    for (uint32_t d = 32; d; d >>= 1) uniq += __shfl_xor(uniq, d, 64);
-    if ((threadIdx.x & 63u) == 0 && uniq) atomicAdd(&counters[1], (unsigned long long)uniq);
 }
 
 void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw)
@@ -1145,13 +1318,26 @@ void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw)
     else k_wide_insert<8><<<blocks, 256, 0, st>>>(w);
 }
 
-void launch_wide_drain(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
-                       uint64_t* out_keys, uint32_t* out_cnt, unsigned long long* counters)
+uint32_t wide_drain_blocks(uint64_t slots)
 {
-    uint64_t blocks = (w.mask + 1u + 255u) / 256u;
-    if (blocks > 8192) blocks = 8192;
-    if (nw == 4) k_wide_drain<4><<<(unsigned)blocks, 256, 0, st>>>(w, ci, cx, onw, out_keys, out_cnt, counters);
-    else k_wide_drain<8><<<(unsigned)blocks, 256, 0, st>>>(w, ci, cx, onw, out_keys, out_cnt, counters);
+    // slots is a power of two >= 2^16: every block gets a multiple of 256
+    return (uint32_t)std::min<uint64_t>(slots / 256u, 4096u);
+}
+
+void launch_wide_drain_count(hipStream_t st, const WideArgs& w, uint32_t ci, uint64_t cx, uint64_t* blk_keep,
+                             uint64_t* blk_occ)
+{
+    const uint32_t nb = wide_drain_blocks(w.mask + 1u);
+    k_wide_drain_count<<<nb, 256, 0, st>>>(w, (w.mask + 1u) / nb, ci, cx, blk_keep, blk_occ);
+}
+
+void launch_wide_drain_write(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
+                             const uint64_t* blk_off, uint64_t* out_keys, uint32_t* out_cnt)
+{
+    const uint32_t nb = wide_drain_blocks(w.mask + 1u);
+    const uint64_t per = (w.mask + 1u) / nb;
+    if (nw == 4) k_wide_drain_write<4><<<nb, 256, 0, st>>>(w, per, ci, cx, onw, blk_off, out_keys, out_cnt);
+    else k_wide_drain_write<8><<<nb, 256, 0, st>>>(w, per, ci, cx, onw, blk_off, out_keys, out_cnt);
 }
 
 // =========================================================== routing ====

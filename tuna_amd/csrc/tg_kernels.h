@@ -71,7 +71,8 @@ struct SkArgs {
 };
 
 // k > 64 (keys of 4 or 8 words): the k-mers of every strip go into one
-// open-addressed HBM table; the count word is the slot state.
+// open-addressed HBM table whose slots hold a key's position in the packed
+// stream (positions below 2^40) and its count.
 constexpr uint32_t WIDE_WMAX = 256;  // m-mers per window (k - m + 1) on that path
 struct WideArgs {
     const uint32_t* packed;
@@ -81,9 +82,8 @@ struct WideArgs {
     uint32_t n_strips;          // capacity of the launch
     const uint64_t* n_dev;      // live strip total
     uint32_t S, k;
-    uint64_t* keys;             // NW words per slot
-    uint32_t* cnt;              // 0 empty, 0xFFFFFFFF being written, else the count
-    uint64_t mask;              // slots - 1 (a power of two)
+    uint64_t* tab;              // two words per slot: tag (0 empty), count
+    uint64_t mask;              // slots - 1 (a power of two, >= 2^16)
     uint32_t* err;              // set when a key found no slot
 };
 
@@ -185,11 +185,15 @@ struct BatchJobs {
 
 // launchers (defined in kernels.hip)
 void launch_pack(hipStream_t st, const uint8_t* bases, uint64_t n, uint32_t* out, uint64_t n_words);
+// long_list: fill_list_words(strip capacity) words, cleared by the chunk
+// counts and used by the strip fill that follows them on the same stream
+uint64_t fill_list_words(uint64_t cap_strips);
 void launch_chunk_counts(hipStream_t st, const uint64_t* off, uint64_t n_chunks, uint32_t k,
-                         uint32_t S, uint64_t* nstr, uint64_t* nkm);
+                         uint32_t S, uint64_t* nstr, uint64_t* nkm, uint64_t* long_list);
 void launch_fill_strips(hipStream_t st, uint64_t n_chunks, const uint64_t* nstr_excl,
                         const uint64_t* nkm_excl, const uint64_t* nkm, uint32_t S,
-                        uint32_t* strip_chunk, uint32_t* strip_tb, uint64_t* strip_slot);
+                        uint32_t* strip_chunk, uint32_t* strip_tb, uint64_t* strip_slot,
+                        uint64_t* long_list, uint64_t list_words);
 // exclusive scan of n u64 values into out[0..n] (out[n] = total); tmp >= scan_tmp_words(n)
 uint64_t scan_tmp_words(uint64_t n);
 void launch_scan_u64(hipStream_t st, const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* tmp);
@@ -198,8 +202,13 @@ void launch_u32_to_u64(hipStream_t st, const uint32_t* in, uint64_t n, uint64_t*
 void launch_superkmers(hipStream_t st, const SkArgs& a, uint32_t rw);
 void launch_superkmers_wide(hipStream_t st, const SkArgs& a);
 void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw);
-void launch_wide_drain(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
-                       uint64_t* out_keys, uint32_t* out_cnt, unsigned long long* counters);
+// drain blocks of a table: per-block totals (written, distinct), scanned by the
+// caller, then the written keys at the scanned block offsets
+uint32_t wide_drain_blocks(uint64_t slots);
+void launch_wide_drain_count(hipStream_t st, const WideArgs& w, uint32_t ci, uint64_t cx, uint64_t* blk_keep,
+                             uint64_t* blk_occ);
+void launch_wide_drain_write(hipStream_t st, const WideArgs& w, uint32_t nw, uint32_t ci, uint64_t cx, uint32_t onw,
+                             const uint64_t* blk_off, uint64_t* out_keys, uint32_t* out_cnt);
 void launch_route_hist(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint32_t rw);
 void launch_route_colscan(hipStream_t st, const RouteArgs& a, uint32_t ntiles, uint64_t* part_sk,
                           uint64_t* part_km);
