@@ -123,6 +123,13 @@ int tg_set_plain_sequence(tg_ctx* ctx, int on);
  * used (work items x LDS table size; tables are fixed, never resized).  Work
  * items then never span two partitions; results are unchanged. */
 int tg_set_debug_stats(tg_ctx* ctx, int on);
+/* Entries of the per-lane deque that the superkmer pass of k > 64 keeps for
+ * the window minimum (1..16, default 16).  A strip whose deque would grow past
+ * it is counted again by the pass that keeps the whole window, so the totals
+ * do not depend on it; small values send most strips there (a test of that
+ * pass).  No counterpart in the reference, whose superkmer pass keeps the
+ * window (src/superkmer_helpers.hpp). */
+int tg_set_wide_deque(tg_ctx* ctx, uint32_t entries);
 /* Copies the statistics of the last such job; returns the number of
  * partitions, TG_ECAP when n is smaller, TG_EINVAL when none were recorded.
  * Null arrays are skipped. */

@@ -112,3 +112,23 @@ def test_batched_engine_refuses_k_above_64(eng):
         eng.count_many(bases, off, np.array([0, 1, 2], np.uint64), 127, 21)
     with pytest.raises(T.TunaGpuError, match="k <= 64"):
         eng.superkmers(bases, off, 127, 21, 16)
+
+
+@pytest.mark.parametrize("cap", [1, 2, 3, 5])
+def test_wide_deque_overflow_recounted(eng, cap):
+    # a deque of `cap` entries overflows on most strips (an ascending pair of
+    # m-mer hashes suffices for cap 1): those strips are counted again by the
+    # pass that keeps the whole window, so totals and counts are unchanged
+    rng = random.Random(900 + cap)
+    eng.set_wide_deque(cap)
+    try:
+        check(eng, mixed_inputs(rng, 127), 127, 21, label=f"deque{cap}")
+        check(eng, mixed_inputs(rng, 200), 200, 31, label=f"deque{cap}")
+    finally:
+        eng.set_wide_deque(16)
+
+
+def test_wide_deque_setting_range(eng):
+    for bad in (0, 17):
+        with pytest.raises(T.TunaGpuError, match="deque entries"):
+            eng.set_wide_deque(bad)

@@ -724,13 +724,14 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     HIPOK(hipStreamSynchronize(s));
     const uint64_t n_strips = tot[0], n_kmers = tot[1];
     if (n_strips >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
-    uint32_t *strip_chunk, *strip_tb, *flag_list, *flags;
+    uint32_t *strip_chunk, *strip_tb, *flag_list, *ovf_list, *flags;
     uint64_t* strip_slot;
     unsigned long long* sk_count;
     const uint64_t ns1 = n_strips + 1;
     if ((rc = ensure_t(c, "strip_chunk", ns1, &strip_chunk)) || (rc = ensure_t(c, "strip_tb", ns1, &strip_tb)) ||
         (rc = ensure_t(c, "strip_slot", ns1, &strip_slot)) || (rc = ensure_t(c, "flag_list", ns1, &flag_list)) ||
-        (rc = ensure_t(c, "sk_flags", 64, &flags)) || (rc = ensure_t(c, "sk_count", 8, &sk_count)))
+        (rc = ensure_t(c, "ovf_list", ns1, &ovf_list)) || (rc = ensure_t(c, "sk_flags", 64, &flags)) ||
+        (rc = ensure_t(c, "sk_count", 8, &sk_count)))
         return rc;
     {
         Prof pr(c, "strips");
@@ -760,43 +761,58 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
     a.sk_count = sk_count;
     a.flag_list = flag_list;
     a.flag_count = flags + 1;
+    a.dq_cap = c->wide_dq;
+    a.ovf_list = ovf_list;
+    a.ovf_count = flags + 3;
+    // The superkmer pass only totals superkmers, and nothing in the table work
+    // reads it: it runs on a second stream beside the insert (LDS-bound with
+    // few waves per CU beside a latency-bound insert), unless kernels are timed
+    hipStream_t ss = s;
+    if (!c->profiling) {
+        if (!c->side && (hipStreamCreate(&c->side) != hipSuccess || hipEventCreate(&c->ev_side) != hipSuccess))
+            return fail(TG_EDEVICE, "HIP stream setup failed");
+        ss = c->side;
+        HIPOK(hipEventRecord(c->ev_side, s));
+        HIPOK(hipStreamWaitEvent(ss, c->ev_side, 0));
+    }
     {
         Prof pr(c, "superkmers");
-        tg::launch_superkmers_wide(s, a);
+        tg::launch_superkmers_wide(ss, a);
     }
     HIPOK(hipGetLastError());
-    uint32_t n_flag = 0;
-    HIPOK(hipMemcpyAsync(&n_flag, flags + 1, 4, hipMemcpyDeviceToHost, s));
-    HIPOK(hipStreamSynchronize(s));
-    if (n_flag) {
-        // strips whose look-back held no cut: counted again from the chunk start
-        a.list = flag_list;
-        a.n = n_flag;
-        a.lookback = tg::LOOKBACK_FULL;
-        a.flag_count = flags + 2;  // nothing is flagged with full look-back
-        Prof pr(c, "superkmers_fixup");
-        tg::launch_superkmers_wide(s, a);
-    }
-    HIPOK(hipEventRecord(c->ev_b, s));
-    // strips of the insert: 256 k-mers, so that a long chunk gives many lanes
-    // (a lane's k-mers are one chain of dependent table probes)
-    uint64_t n_ins = n_strips;
-    if (SI != S) {
-        {
-            Prof pr(c, "strips");
-            tg::launch_chunk_counts(s, d_off, nch, k, SI, nstr, nkm, strip_long);
-            tg::launch_scan_u64(s, nstr, nch, nstr_x, tmp);
-        }
-        HIPOK(hipGetLastError());
-        HIPOK(hipMemcpyAsync(&n_ins, nstr_x + nch, 8, hipMemcpyDeviceToHost, s));
-        HIPOK(hipStreamSynchronize(s));
-        if (n_ins >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
-        if ((rc = ensure_t(c, "strip_chunk", n_ins + 1, &strip_chunk)) ||
-            (rc = ensure_t(c, "strip_tb", n_ins + 1, &strip_tb)) ||
-            (rc = ensure_t(c, "strip_slot", n_ins + 1, &strip_slot)))
-            return rc;
+    HIPOK(hipEventRecord(c->ev_b, ss));
+    // every return from here on leaves the second stream idle (its kernel
+    // reads buffers the next job writes)
+    struct SideJoin {
+        hipStream_t st;
+        ~SideJoin() { hipStreamSynchronize(st); }
+    } join{ss};
+    // strips of the insert, in tables of their own: 256 k-mers, so that a long
+    // chunk gives many lanes (a lane's k-mers are one chain of dependent table
+    // probes)
+    uint64_t *inst, *inst_x, *inkm;
+    if ((rc = ensure_t(c, "inst", nch + 1, &inst)) || (rc = ensure_t(c, "inst_x", nch + 1, &inst_x)) ||
+        (rc = ensure_t(c, "inkm", nch + 1, &inkm)))
+        return rc;
+    {
         Prof pr(c, "strips");
-        tg::launch_fill_strips(s, nch, nstr_x, nkm_x, nkm, SI, strip_chunk, strip_tb, strip_slot, strip_long,
+        tg::launch_chunk_counts(s, d_off, nch, k, SI, inst, inkm, strip_long);
+        tg::launch_scan_u64(s, inst, nch, inst_x, tmp);
+    }
+    HIPOK(hipGetLastError());
+    uint64_t n_ins = 0;
+    HIPOK(hipMemcpyAsync(&n_ins, inst_x + nch, 8, hipMemcpyDeviceToHost, s));
+    HIPOK(hipStreamSynchronize(s));
+    if (n_ins >= (1ull << 31)) return fail(TG_EINVAL, "too many strips");
+    uint32_t *istrip_chunk, *istrip_tb;
+    uint64_t* istrip_slot;
+    if ((rc = ensure_t(c, "istrip_chunk", n_ins + 1, &istrip_chunk)) ||
+        (rc = ensure_t(c, "istrip_tb", n_ins + 1, &istrip_tb)) ||
+        (rc = ensure_t(c, "istrip_slot", n_ins + 1, &istrip_slot)))
+        return rc;
+    {
+        Prof pr(c, "strips");
+        tg::launch_fill_strips(s, nch, inst_x, nkm_x, inkm, SI, istrip_chunk, istrip_tb, istrip_slot, strip_long,
                                long_words);
     }
     // the k-mer table: 16-byte slots {tag, count}, a power of two of at least
@@ -821,10 +837,10 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         tg::WideArgs w{};
         w.packed = packed;
         w.off = d_off;
-        w.strip_chunk = strip_chunk;
-        w.strip_tb = strip_tb;
+        w.strip_chunk = istrip_chunk;
+        w.strip_tb = istrip_tb;
         w.n_strips = (uint32_t)n_ins;
-        w.n_dev = nstr_x + nch;
+        w.n_dev = inst_x + nch;
         w.S = SI;
         w.k = k;
         w.tab = tab;
@@ -865,6 +881,34 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         {
             Prof pr(c, "job_gather");
             tg::launch_wide_drain_write(s, w, nw, p->ci, p->cx, onw, blk_x, out_keys, out_cnt);
+        }
+        HIPOK(hipGetLastError());
+        // the superkmer total: after the main pass, the strips whose deque
+        // overflowed counted again by the window kernel with the same
+        // look-back (a look-back without a cut joins the flagged strips),
+        // then the flagged strips counted again from their chunk start
+        HIPOK(hipStreamWaitEvent(s, c->ev_b, 0));
+        uint32_t nf[3] = {0, 0, 0};  // flagged, (unused), overflowed
+        HIPOK(hipMemcpyAsync(nf, flags + 1, sizeof(nf), hipMemcpyDeviceToHost, s));
+        HIPOK(hipStreamSynchronize(s));
+        if (nf[2]) {
+            a.list = ovf_list;
+            a.n = nf[2];
+            {
+                Prof pr(c, "superkmers_fixup");
+                tg::launch_superkmers_wide(s, a);
+            }
+            HIPOK(hipGetLastError());
+            HIPOK(hipMemcpyAsync(nf, flags + 1, 4, hipMemcpyDeviceToHost, s));
+            HIPOK(hipStreamSynchronize(s));
+        }
+        if (nf[0]) {
+            a.list = flag_list;
+            a.n = nf[0];
+            a.lookback = tg::LOOKBACK_FULL;
+            a.flag_count = flags + 2;  // nothing is flagged with full look-back
+            Prof pr(c, "superkmers_fixup");
+            tg::launch_superkmers_wide(s, a);
         }
         HIPOK(hipGetLastError());
         unsigned long long nsk = 0;
@@ -1023,6 +1067,11 @@ void tg_ctx_destroy(tg_ctx* c)
     hipEventDestroy(c->ev_a);
     hipEventDestroy(c->ev_b);
     hipEventDestroy(c->ev_c);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+        hipEventDestroy(c->ev_side);
+    }
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1117,6 +1166,14 @@ int tg_set_plain_sequence(tg_ctx* c, int on)
 {
     if (!c) return fail(TG_EINVAL, "null context");
     c->plain = on != 0;
+    return TG_OK;
+}
+
+int tg_set_wide_deque(tg_ctx* c, uint32_t entries)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    if (entries < 1 || entries > tg::WDQ_MAX) return fail(TG_EINVAL, "deque entries must be in [1, %u]", tg::WDQ_MAX);
+    c->wide_dq = entries;
     return TG_OK;
 }
 

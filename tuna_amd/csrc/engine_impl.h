@@ -90,6 +90,10 @@ struct tg_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::map<std::string, tgi::KTime> ktimes;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr;
+    // second stream (default priority) of the k > 64 path, made on first use
+    hipStream_t side = nullptr;
+    hipEvent_t ev_side = nullptr;
+    uint32_t wide_dq = 16;  // deque entries of its superkmer pass (tg_set_wide_deque)
     // last job
     uint64_t n_out = 0;
     uint32_t nw = 1;

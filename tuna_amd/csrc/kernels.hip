@@ -965,11 +965,13 @@ constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 // position in the packed stream (no routing: a key of 4 or 8 words would leave
 // room for few keys in an LDS table).
 
-// Superkmers of k > 64, counted only.  The window holds W <= 256 m-mer hashes
-// per lane in LDS ([position][lane], one wave per workgroup) and keeps their
-// values only: a cut depends on the value of the window minimum, never on its
-// position.  The decision is the general one (look-back, flags, cap) with the
-// cap of the reference's header type (2k - m <= 255: 255 bases, else 65535).
+// Superkmers of k > 64, counted only (the pass over flagged strips; the main
+// pass, k_superkmers_wdq below, keeps a deque instead of the window).  The
+// window holds W <= 256 m-mer hashes per lane in LDS ([position][lane], one
+// wave per workgroup) and keeps their values only: a cut depends on the value
+// of the window minimum, never on its position.  The decision is the general
+// one (look-back, flags, cap) with the cap of the reference's header type
+// (2k - m <= 255: 255 bases, else 65535).
 __device__ __forceinline__ void wide_backward(uint64_t* sv, uint32_t W, uint32_t lane)
 {
     uint64_t cur = 0;
@@ -1034,9 +1036,98 @@ __global__ __launch_bounds__(64) void k_superkmers_wide(SkArgs a)
     if (nsk) atomicAdd(&a.sk_count[L.job], (unsigned long long)nsk);
 }
 
+// The main pass of the same count with the window minimum kept by a monotone
+// deque instead of the whole window: the m-mers of the window that no later
+// one undercuts, in increasing value (an older m-mer of equal value is dropped:
+// the cut rule reads the minimum's value only).  For hash values in random
+// order it holds about ln W entries; a lane keeps up to WDQ of them in LDS
+// ([entry][thread], 12 bytes each), so a CU runs 12 waves where the window
+// kernel runs 2.  A lane whose deque would grow past a.dq_cap (m-mers whose
+// values ascend, about one position in 10^6 for W = 107 and random sequence)
+// stops with nothing counted and lists its strip in ovf_list; the window
+// kernel counts the listed strips again with the same look-back.
+constexpr uint32_t WDQ = WDQ_MAX, WDQ_BLOCK = 256;
+
+__global__ __launch_bounds__(WDQ_BLOCK) void k_superkmers_wdq(SkArgs a)
+{
+    __shared__ uint64_t s_tab[34];  // forward [0, 16), reverse [16, 32), init [32, 34)
+    __shared__ uint64_t s_dv[WDQ * WDQ_BLOCK];
+    __shared__ uint32_t s_dp[WDQ * WDQ_BLOCK];
+    sk_tables(a, s_tab, s_tab + 16, s_tab + 32);
+    __syncthreads();
+    const uint32_t tid = threadIdx.x;
+    const uint32_t s = blockIdx.x * WDQ_BLOCK + tid;
+    uint32_t n = a.n;
+    if (a.n_dev) n = (uint32_t)min((uint64_t)n, *a.n_dev);
+    uint32_t nsk = 0;
+    if (s < n) {
+        SkLane L;
+        sk_init(L, a, s, s_tab + 32);
+        SkStage S{};
+        const uint32_t W = a.W;
+        for (uint32_t i = 1; i < a.m; ++i) sk_roll(L, s_tab);
+        // deque: entries [head, head + sz) mod WDQ; its front cached in (fv, fp)
+        uint32_t head = 0, sz = 0, fp = 0;
+        const uint32_t cap = min(max(a.dq_cap, 1u), WDQ);
+        uint64_t fv = 0;
+        bool over = false;
+        auto push = [&](uint64_t h, uint32_t pos) {
+            while (sz > 0u && s_dv[((head + sz - 1u) & (WDQ - 1u)) * WDQ_BLOCK + tid] >= h) --sz;
+            if (sz == cap) {
+                over = true;
+                return;
+            }
+            const uint32_t e = ((head + sz) & (WDQ - 1u)) * WDQ_BLOCK + tid;
+            s_dv[e] = h;
+            s_dp[e] = pos;
+            if (sz == 0u) {
+                fv = h;
+                fp = pos;
+            }
+            ++sz;
+        };
+        for (uint32_t r = 0; r < W && !over; ++r) {
+            sk_roll(L, s_tab);
+            push(L.f ^ L.rv, r);
+        }
+        // k-mer t of the walk (chunk k-mer L.ts + t) has the m-mers [t, t + W)
+        for (uint32_t t = 0; !over;) {
+            sk_kmer<2, false, false>(L, a, S, L.ts + t, fv, 0u);
+            if (L.done) break;
+            sk_roll(L, s_tab);
+            push(L.f ^ L.rv, t + W);
+            ++t;
+            if (fp < t) {  // the front left the window (at most one per step)
+                head = (head + 1u) & (WDQ - 1u);
+                --sz;
+                fv = s_dv[head * WDQ_BLOCK + tid];
+                fp = s_dp[head * WDQ_BLOCK + tid];
+            }
+        }
+        if (over) {
+            // counted again with the same look-back by the window kernel
+            const uint32_t slot = atomicAdd(a.ovf_count, 1u);
+            a.ovf_list[slot] = s;
+        } else {
+            if (L.flagged) {
+                const uint32_t slot = atomicAdd(a.flag_count, 1u);
+                a.flag_list[slot] = s;
+            }
+            nsk = L.nsk + ((L.tail && L.t0p == L.t0) ? 1u : 0u);
+        }
+    }
+    // one atomic per wave (a single job's total is one counter)
+    for (uint32_t d = 32; d; d >>= 1) nsk += __shfl_xor(nsk, d, 64);
+    if ((tid & 63u) == 0u && nsk) atomicAdd(&a.sk_count[0], (unsigned long long)nsk);
+}
+
 void launch_superkmers_wide(hipStream_t st, const SkArgs& a)
 {
     if (a.n == 0) return;
+    if (!a.list) {
+        k_superkmers_wdq<<<(a.n + WDQ_BLOCK - 1u) / WDQ_BLOCK, WDQ_BLOCK, 0, st>>>(a);
+        return;
+    }
     static std::once_flag once;
     std::call_once(once, [] {
         (void)hipFuncSetAttribute((const void*)k_superkmers_wide, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -68,12 +68,16 @@ struct SkArgs {
     unsigned long long* sk_count;  // superkmers (first pieces) per job (one entry for a single job)
     uint32_t* flag_list;
     uint32_t* flag_count;
+    uint32_t dq_cap;            // k > 64 main pass: deque entries per lane (1..WDQ_MAX)
+    uint32_t* ovf_list;         // k > 64 main pass: strips whose deque overflowed
+    uint32_t* ovf_count;
 };
 
 // k > 64 (keys of 4 or 8 words): the k-mers of every strip go into one
 // open-addressed HBM table whose slots hold a key's position in the packed
 // stream (positions below 2^40) and its count.
 constexpr uint32_t WIDE_WMAX = 256;  // m-mers per window (k - m + 1) on that path
+constexpr uint32_t WDQ_MAX = 16;     // deque entries per lane of its superkmer pass
 struct WideArgs {
     const uint32_t* packed;
     const uint64_t* off;        // chunk offsets (global base index), n_chunks+1
