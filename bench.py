@@ -12,9 +12,9 @@ uploaded to HBM before the timed region.  One step = one whole counting job on
 the device (pack, superkmers, routing, counting, drain).
 
 Beside it: the short-read workload (250 Mbp of 150 bp reads at 50x, k=31 m=21
-ci=2 n=256) measured the same way, the end-to-end CLI rate (FASTQ file to TSV
-file) of both, the many-sample batched leg, the serving leg and the CPU
-baseline.
+ci=2 n=256) and a k = 127 workload (250 Mbp of 250 bp reads, m=21 ci=2)
+measured the same way, the end-to-end CLI rate (FASTQ file to TSV file) of the
+first two, the many-sample batched leg, the serving leg and the CPU baseline.
 
 Prints one JSON line.  --gpus N > 1 is not supported: this engine runs on one
 GPU.
@@ -54,6 +54,13 @@ WORKLOADS = {
         "config": "kmers_per_bucket=32768,max_tiles=1024,load_factor=0.9",
     },
 }
+WORKLOADS["long_k"] = {
+    "desc": ("k = 127 on 250 bp reads: 5 Mbp genome seed 2, 1,000,000 reads (250 Mbp), 0.5% substitutions, "
+             "N 0.01%, k=127 m=21 ci=2"),
+    "spec": "kind=reads_fq genome=5000000 seed=2 gc=0.5 reads=1000000 rl=250 sub=0.005 nrate=0.0001",
+    "k": 127, "m": 21, "ci": 2, "n": 16,
+    "config": None,  # k > 64 does not run through the batched engine
+}
 HEADLINE = "repeat_reads"
 
 
@@ -77,6 +84,22 @@ def alg_bytes(n_bases, n_sk, sum_len, n_unique, n_out, k):
     count = P + 2 * n_unique * s_slot + n_out * s_slot
     superkmers = n_bases // 4 + P
     return job, count, superkmers, P
+
+
+def alg_bytes_wide(n_bases, n_unique, n_out, k):
+    """Algorithmic bytes of a k > 64 job (no partition records: every k-mer
+    goes from the packed stream into the table).
+
+    Job: N (read ASCII) + N/4 (write packed) + 2 N/4 (superkmer pass and
+       table insert read the packed stream) + 2 U s_slot (insert + drain)
+       + U_out s_slot (output), s_slot = 8 ceil(k/32) + 4.
+    Insert ("count"): N/4 + 2 U s_slot + U_out s_slot, as the counting kernel
+    of k <= 64 with the packed stream in place of the records.
+    """
+    s_slot = 8 * ((k + 31) // 32) + 4
+    tab = 2 * n_unique * s_slot + n_out * s_slot
+    job = n_bases + 3 * (n_bases // 4) + tab
+    return job, n_bases // 4 + tab, n_bases // 4
 
 
 def host_threads():
@@ -403,7 +426,9 @@ def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
     t_tune = time.perf_counter()
     jc = np.array([0, nch], dtype=np.uint64)
     jb = np.array([0, N], dtype=np.uint64)
-    if config:
+    if K > 64:
+        tuned = {"path": "k > 64: HBM table of positions"}
+    elif config:
         kw = {}
         for item in config.split(","):
             key, v = item.split("=")
@@ -412,7 +437,7 @@ def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
     else:
         tuned, _ = eng.autotune_many(d_b, N, d_o, nch, jc, jb, K, M, CI)
     t_tune = time.perf_counter() - t_tune
-    log(f"{name}: {'configured' if config else 'autotuned'} in {t_tune:.1f} s: {tuned}")
+    log(f"{name}: {'configured' if config or K > 64 else 'autotuned'} in {t_tune:.1f} s: {tuned}")
     for _ in range(warmup):
         eng.count_device(d_b, N, d_o, nch, K, M, NPART, CI)
     eng.sync()
@@ -437,7 +462,11 @@ def bench_single(T, eng, name, bases, off, steps, warmup, config=None):
     # exact counters of the job for the algorithmic-bytes figures
     # a superkmer of j k-mers has j + k - 1 bases
     sum_len = st.n_kmers + st.n_superkmers * (K - 1)
-    job_b, count_b, sk_b, P = alg_bytes(N, st.n_superkmers, sum_len, st.n_unique, st.n_out, K)
+    if K > 64:
+        job_b, count_b, sk_b = alg_bytes_wide(N, st.n_unique, st.n_out, K)
+        P = 0
+    else:
+        job_b, count_b, sk_b, P = alg_bytes(N, st.n_superkmers, sum_len, st.n_unique, st.n_out, K)
     job_roof_gbases = HBM_PEAK_GBS * 1e9 * N / job_b / 1e9
     kern = {k_: (ms / max(n, 1), n) for k_, (ms, n) in kt.items()}
     per_launch = {"count": count_b, "superkmers": sk_b}
@@ -600,7 +629,7 @@ def main():
     if args.only == "serving":
         print(json.dumps({"serving": bench_serving(T, args.requests, lambdas, threads)}))
         return
-    names = [HEADLINE, "short_reads"]
+    names = [HEADLINE, "short_reads", "long_k"]
     if args.only == "single" and args.workload:
         names = [args.workload]
     eng = T.Engine(0)

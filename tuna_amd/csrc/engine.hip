@@ -847,9 +847,12 @@ int count_wide(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N
         w.mask = slots - 1;
         w.err = flags + 8;
         {
-            Prof pr(c, "count");
+            Prof pr(c, "table_clear");
             tg::launch_fill_u64(s, tab, 0ull, 2 * slots);
             tg::launch_fill_u32(s, flags + 8, 0u, 1);
+        }
+        {
+            Prof pr(c, "count");
             tg::launch_wide_insert(s, w, nw);
         }
         HIPOK(hipGetLastError());
