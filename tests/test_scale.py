@@ -1,5 +1,6 @@
-"""Full-size workloads, checked through size-independent properties (the
-oracle would take too long at these sizes):
+"""Full-size workloads.  The 2 Gbp repetitive-genome job is compared with the
+oracle's multithreaded two-phase count; all of them are also checked through
+size-independent properties:
   * the counts add up to the number of k-mers, and every distinct k-mer is
     reported when ci = 1;
   * the multiset of (k-mer, count) pairs, reduced to an order-free checksum,
@@ -118,3 +119,35 @@ def test_job_above_2_31_bases():
         eng.dev_free(d_b)
         eng.dev_free(d_o)
         eng.close()
+
+
+def test_full_size_repeat_reads_against_oracle():
+    """The 2 Gbp repetitive-genome job at its own settings (k=31 m=23 ci=1,
+    n=4096) against the oracle's two-phase count on the host's threads: the
+    same totals (k-mers, superkmers, distinct), exactly the same set of keys,
+    and the same (key, count) multiset by an order-free checksum of the pairs."""
+    import os
+    import oracle as O
+    try:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        threads = 1
+    bases, off = T.synth_parsed(C3)
+    say(f"repeat reads vs oracle: {len(bases)} bases, {len(off) - 1} chunks, {threads} host threads")
+    eng = T.Engine(0)
+    try:
+        gk, gc = eng.count(bases, off, 31, 23, 4096, 1)
+        st = eng.last_stats
+    finally:
+        eng.close()
+    say(f"  device: {st.n_kmers} k-mers, {st.n_superkmers} superkmers, {st.n_unique} distinct")
+    ok, oc, info = O.count_parallel(bases, off, 31, 23, 4096, 1, 2**64 - 1, threads)
+    del bases, off
+    say(f"  oracle: {info['kmers']} k-mers, {info['superkmers']} superkmers, {info['unique']} distinct "
+        f"({info['phase1_s'] + info['phase2_s']:.1f} s)")
+    assert (st.n_kmers, st.n_superkmers, st.n_unique, st.n_out) == \
+        (info["kmers"], info["superkmers"], info["unique"], info["out"])
+    assert len(gc) == len(oc) and int(gc.sum(dtype=np.uint64)) == int(oc.sum(dtype=np.uint64))
+    assert checksum(gk, gc) == checksum(ok, oc)
+    assert np.array_equal(np.sort(gk[:, 0]), np.sort(ok[:, 0]))
+    say("  same keys, same (key, count) multiset")
