@@ -1238,15 +1238,16 @@ __device__ __forceinline__ bool wide_eq(const uint64_t (&x)[NW], const uint64_t 
     return e;
 }
 
-// The k-mer table holds where a key is, not the key: slot = {tag, count}, two
-// words.  tag = fingerprint (bits 63:41, the top of the key's hash) | bit 40
-// (the canonical key is the reverse complement of the bases at the position)
-// | position + 1 (bits 39:0, base index of the k-mer's first base in the
-// packed stream).  A tag is claimed with one 64-bit compare-and-swap and never
-// changes afterwards; a key is compared with a slot's by reading the slot's
-// bases from the packed stream, which no kernel writes while the table is
-// filled, and counts are added with relaxed atomics.  No slot state is
-// published in parts, so neither side needs an acquire or release.
+// The k-mer table holds where a key is, not the key: slot = {tag, count - 1},
+// two words.  tag = fingerprint (bits 63:41, the top of the key's hash) | bit
+// 40 (the canonical key is the reverse complement of the bases at the
+// position) | position + 1 (bits 39:0, base index of the k-mer's first base in
+// the packed stream).  A tag is claimed with one 64-bit compare-and-swap and
+// never changes afterwards; a key is compared with a slot's by reading the
+// slot's bases from the packed stream, which no kernel writes while the table
+// is filled, and every later occurrence adds 1 with a relaxed atomic (the
+// claim itself adds nothing: the count word holds count - 1).  No slot state
+// is published in parts, so neither side needs an acquire or release.
 constexpr uint32_t WIDE_POS_BITS = 40;
 constexpr uint64_t WIDE_POS_MASK = (1ull << WIDE_POS_BITS) - 1ull;
 constexpr uint64_t WIDE_FP_MASK = ~((1ull << (WIDE_POS_BITS + 1)) - 1ull);
@@ -1256,7 +1257,7 @@ constexpr uint32_t WIDE_PROBES = 4096;
 
 struct __align__(16) WideSlot {
     uint64_t tag;
-    uint64_t cnt;  // low 32 bits: the count
+    uint64_t cnt;  // low 32 bits: the count - 1 (of an occupied slot)
 };
 
 template <int NW>
@@ -1274,10 +1275,7 @@ __device__ __forceinline__ bool wide_insert(const WideArgs& w, const uint64_t (&
         uint64_t v = __hip_atomic_load(tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (v == 0u) {
             v = atomicCAS((unsigned long long*)tp, 0ull, (unsigned long long)mine);
-            if (v == 0u) {
-                atomicAdd(reinterpret_cast<uint32_t*>(tp + 1), 1u);
-                return true;
-            }
+            if (v == 0u) return true;
         }
         if (((v ^ fp) & WIDE_FP_MASK) == 0u) {
             uint64_t y[NW];
@@ -1338,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_wide_drain_count(WideArgs w, uint64_t p
     uint32_t keep = 0, occ = 0;
     for (uint64_t i = b0 + threadIdx.x; i < b0 + per_block; i += 256u) {
         const WideSlot e = tab[i];
-        const uint32_t c = (uint32_t)e.cnt;
+        const uint32_t c = (uint32_t)e.cnt + 1u;
         occ += e.tag != 0u ? 1u : 0u;
         keep += (e.tag != 0u && c >= ci && (uint64_t)c <= cx) ? 1u : 0u;
     }
@@ -1357,47 +1355,55 @@ __global__ __launch_bounds__(256) void k_wide_drain_count(WideArgs w, uint64_t p
     }
 }
 
+// Each thread takes WIDE_PT consecutive slots per step; one block scan of the
+// per-thread totals places them, so a step costs one scan for 256 WIDE_PT
+// slots and all of a thread's slot loads are in flight together.
+constexpr uint32_t WIDE_PT = 8;
+
+__device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t& total);
+
 template <int NW>
 __global__ __launch_bounds__(256) void k_wide_drain_write(WideArgs w, uint64_t per_block, uint32_t ci, uint64_t cx,
                                                           uint32_t onw, const uint64_t* __restrict__ blk_off,
                                                           uint64_t* __restrict__ out_keys,
                                                           uint32_t* __restrict__ out_cnt)
 {
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_red[4];
     const WideSlot* tab = reinterpret_cast<const WideSlot*>(w.tab);
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t tw = (2u * w.k - 1u) / 64u;
     const uint32_t topb = 2u * w.k - 64u * tw;
     const uint64_t topmask = topb >= 64u ? ~0ull : ((1ull << topb) - 1u);
     uint64_t base = blk_off[blockIdx.x];
     const uint64_t b0 = blockIdx.x * per_block;
-    for (uint64_t i0 = b0; i0 < b0 + per_block; i0 += 256u) {
-        const WideSlot e = tab[i0 + threadIdx.x];
-        const uint32_t c = (uint32_t)e.cnt;
-        const bool keep = e.tag != 0u && c >= ci && (uint64_t)c <= cx;
-        const uint64_t km = __ballot(keep);
-        if (lane == 0) s_w[wv] = (uint32_t)__popcll(km);
-        __syncthreads();
-        uint32_t before = 0, tot = 0;
+    for (uint64_t i0 = b0; i0 < b0 + per_block; i0 += 256u * WIDE_PT) {
+        const uint64_t s0 = i0 + (uint64_t)threadIdx.x * WIDE_PT;
+        WideSlot e[WIDE_PT];
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t nq = s_w[q];
-            before += q < wv ? nq : 0u;
-            tot += nq;
+        for (uint32_t j = 0; j < WIDE_PT; ++j) e[j] = tab[s0 + j];
+        uint32_t keep = 0, nk = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < WIDE_PT; ++j) {
+            const uint32_t c = (uint32_t)e[j].cnt + 1u;
+            const bool kp = e[j].tag != 0u && c >= ci && (uint64_t)c <= cx;
+            keep |= (kp ? 1u : 0u) << j;
+            nk += kp ? 1u : 0u;
         }
-        __syncthreads();
-        if (keep) {
-            const uint64_t o = base + before + lane_rank(km);
+        uint32_t tot;
+        uint64_t o = base + block_excl_u32(nk, s_red, tot);
+        base += tot;
+#pragma unroll
+        for (uint32_t j = 0; j < WIDE_PT; ++j) {
+            if (!((keep >> j) & 1u)) continue;
             uint64_t y[NW], z[NW];
-            wide_key_at<NW>(w.packed, (e.tag & WIDE_POS_MASK) - 1u, w.k, tw, y);
+            wide_key_at<NW>(w.packed, (e[j].tag & WIDE_POS_MASK) - 1u, w.k, tw, y);
             wide_rc<NW>(y, w.k, tw, topmask, z);
-            const bool rcf = (e.tag >> WIDE_POS_BITS) & 1u;
+            const bool rcf = (e[j].tag >> WIDE_POS_BITS) & 1u;
 #pragma unroll
             for (int q = 0; q < NW; ++q)
                 if ((uint32_t)q < onw) out_keys[onw * o + q] = rcf ? z[q] : y[q];
-            out_cnt[o] = c;
+            out_cnt[o] = (uint32_t)e[j].cnt + 1u;
+            ++o;
         }
-        base += tot;
     }
 }
 
@@ -1411,8 +1417,9 @@ void launch_wide_insert(hipStream_t st, const WideArgs& w, uint32_t nw)
 
 uint32_t wide_drain_blocks(uint64_t slots)
 {
-    // slots is a power of two >= 2^16: every block gets a multiple of 256
-    return (uint32_t)std::min<uint64_t>(slots / 256u, 4096u);
+    // slots is a power of two >= 2^16: every block gets a multiple of
+    // 256 WIDE_PT slots
+    return (uint32_t)std::min<uint64_t>(slots / (256u * WIDE_PT), 4096u);
 }
 
 void launch_wide_drain_count(hipStream_t st, const WideArgs& w, uint32_t ci, uint64_t cx, uint64_t* blk_keep,
