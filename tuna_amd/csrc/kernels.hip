@@ -1356,8 +1356,9 @@ __global__ __launch_bounds__(256) void k_wide_drain_count(WideArgs w, uint64_t p
 }
 
 // Each thread takes WIDE_PT consecutive slots per step; one block scan of the
-// per-thread totals places them, so a step costs one scan for 256 WIDE_PT
-// slots and all of a thread's slot loads are in flight together.
+// per-thread totals places them, the written slots of the step are listed in
+// LDS in slot order, and the block rebuilds their keys one per thread, so the
+// base reads of a step are in flight together.
 constexpr uint32_t WIDE_PT = 8;
 
 __device__ __forceinline__ uint32_t block_excl_u32(uint32_t v, uint32_t* sh, uint32_t& total);
@@ -1369,6 +1370,8 @@ __global__ __launch_bounds__(256) void k_wide_drain_write(WideArgs w, uint64_t p
                                                           uint32_t* __restrict__ out_cnt)
 {
     __shared__ uint32_t s_red[4];
+    __shared__ uint64_t s_tag[256 * WIDE_PT];
+    __shared__ uint32_t s_cnt[256 * WIDE_PT];
     const WideSlot* tab = reinterpret_cast<const WideSlot*>(w.tab);
     const uint32_t tw = (2u * w.k - 1u) / 64u;
     const uint32_t topb = 2u * w.k - 64u * tw;
@@ -1389,21 +1392,30 @@ __global__ __launch_bounds__(256) void k_wide_drain_write(WideArgs w, uint64_t p
             nk += kp ? 1u : 0u;
         }
         uint32_t tot;
-        uint64_t o = base + block_excl_u32(nk, s_red, tot);
-        base += tot;
+        uint32_t at = block_excl_u32(nk, s_red, tot);  // ends with a barrier
 #pragma unroll
         for (uint32_t j = 0; j < WIDE_PT; ++j) {
-            if (!((keep >> j) & 1u)) continue;
-            uint64_t y[NW], z[NW];
-            wide_key_at<NW>(w.packed, (e[j].tag & WIDE_POS_MASK) - 1u, w.k, tw, y);
-            wide_rc<NW>(y, w.k, tw, topmask, z);
-            const bool rcf = (e[j].tag >> WIDE_POS_BITS) & 1u;
-#pragma unroll
-            for (int q = 0; q < NW; ++q)
-                if ((uint32_t)q < onw) out_keys[onw * o + q] = rcf ? z[q] : y[q];
-            out_cnt[o] = (uint32_t)e[j].cnt + 1u;
-            ++o;
+            if ((keep >> j) & 1u) {
+                s_tag[at] = e[j].tag;
+                s_cnt[at] = (uint32_t)e[j].cnt + 1u;
+                ++at;
+            }
         }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < tot; q += 256u) {
+            const uint64_t tag = s_tag[q];
+            uint64_t y[NW], z[NW];
+            wide_key_at<NW>(w.packed, (tag & WIDE_POS_MASK) - 1u, w.k, tw, y);
+            wide_rc<NW>(y, w.k, tw, topmask, z);
+            const bool rcf = (tag >> WIDE_POS_BITS) & 1u;
+            const uint64_t o = base + q;
+#pragma unroll
+            for (int r = 0; r < NW; ++r)
+                if ((uint32_t)r < onw) out_keys[onw * o + r] = rcf ? z[r] : y[r];
+            out_cnt[o] = s_cnt[q];
+        }
+        base += tot;
+        __syncthreads();  // the list is rewritten by the next step
     }
 }
 
