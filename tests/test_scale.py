@@ -151,3 +151,37 @@ def test_full_size_repeat_reads_against_oracle():
     assert checksum(gk, gc) == checksum(ok, oc)
     assert np.array_equal(np.sort(gk[:, 0]), np.sort(ok[:, 0]))
     say("  same keys, same (key, count) multiset")
+
+
+def test_wide_job_above_2_32_bases():
+    """k = 127 on a 2.2 Gbp read set A, then on A followed by a copy of itself
+    (4.4 Gbp: the copy's k-mers sit past base 2^32, so the table's positions
+    and the packed stream's offsets need more than 32 bits).  Every key of
+    A + A is a key of A with twice its count: the same distinct keys, twice
+    the k-mers and superkmers, and the (key, count) checksum of A + A equal
+    to that of A with doubled counts."""
+    spec = "kind=reads_fq genome=20000000 seed=9 gc=0.5 reads=14700000 rl=150 sub=0.001 nrate=0"
+    bases, off = T.synth_parsed(spec)
+    N, nch = len(bases), len(off) - 1
+    say(f"wide above 2^32: A = {N} bases, {nch} chunks")
+    assert 2 * N > 2**32
+    eng = T.Engine(0)
+    try:
+        ka, ca = eng.count(bases, off, 127, 21, 16, 1)
+        sa = eng.last_stats
+        say(f"  A: {sa.n_kmers} k-mers, {sa.n_superkmers} superkmers, {sa.n_unique} distinct")
+        ref = (len(ca), checksum(ka, ca.astype(np.uint64) * np.uint64(2)))
+        del ka, ca
+        bb = np.concatenate([bases, bases])
+        oo = np.concatenate([off, off[1:] + np.uint64(N)])
+        del bases, off
+        kb, cb = eng.count(bb, oo, 127, 21, 16, 1)
+        sb = eng.last_stats
+        say(f"  A + A: {sb.n_kmers} k-mers, {sb.n_superkmers} superkmers, {sb.n_unique} distinct")
+    finally:
+        eng.close()
+    assert (sb.n_kmers, sb.n_superkmers, sb.n_unique, sb.n_out) == \
+        (2 * sa.n_kmers, 2 * sa.n_superkmers, sa.n_unique, sa.n_out)
+    assert int(cb.sum(dtype=np.uint64)) == sb.n_kmers and int(cb.min()) >= 2
+    assert (len(cb), checksum(kb, cb.astype(np.uint64))) == ref
+    say("  every key twice as often")
