@@ -194,3 +194,12 @@ def auto_n(sizes, gz, k, threads, l2, l3, fd):
 def test_auto_partitions_formula_pinned(case, expect):
     (sizes, gz), k, t, l2, l3, fd = case
     assert auto_n(sizes, gz, k, t, l2, l3, fd) == expect
+
+
+def test_server_refuses_k_above_64_before_any_device_call():
+    # parameter checks run on the host: the message names the entry points
+    # that count such jobs
+    with pytest.raises(T.TunaGpuError, match="the server counts k <= 64"):
+        T.Server(127, 21)
+    with pytest.raises(T.TunaGpuError, match="invalid k/m"):
+        T.Server(300, 21)

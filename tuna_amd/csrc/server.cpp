@@ -303,6 +303,11 @@ int tg_server_create(int device, const tg_params* p, const tg_server_config* cfg
         set_host_error("null argument");
         return TG_EINVAL;
     }
+    if (p->k > 64 && p->k <= 256) {
+        set_host_error("the server counts k <= 64; count jobs of k = " + std::to_string(p->k) +
+                       " one at a time (tg_count_host, tg_count_device, tg_count_files)");
+        return TG_EINVAL;
+    }
     if (p->k < 2 || p->k > 64 || p->m < 1 || p->m > 32 || p->m >= p->k) {
         set_host_error("invalid k/m for the server");
         return TG_EINVAL;
