@@ -130,6 +130,14 @@ int tg_set_debug_stats(tg_ctx* ctx, int on);
  * pass).  No counterpart in the reference, whose superkmer pass keeps the
  * window (src/superkmer_helpers.hpp). */
 int tg_set_wide_deque(tg_ctx* ctx, uint32_t entries);
+/* Bytes of input that tg_count_files keeps on the device (0, the default: half
+ * the free device memory at the call).  Larger inputs of k <= 64 are read into
+ * host memory and counted slice by slice (tg_batch_config.slice_bases), each
+ * slice uploaded, counted and added into the device merge table, so device
+ * memory bounds the distinct k-mers, not the input.  Replaces the reference's
+ * choice of its disk pipeline for inputs larger than memory
+ * (src/pipeline.hpp:77-98,186-263). */
+int tg_set_input_budget(tg_ctx* ctx, uint64_t bytes);
 /* Copies the statistics of the last such job; returns the number of
  * partitions, TG_ECAP when n is smaller, TG_EINVAL when none were recorded.
  * Null arrays are skipped. */

@@ -1,6 +1,7 @@
 """A single job larger than one pass is counted in slices of whole chunks,
 each slice counted without the count filter and merged on the device
-(tg_batch_config.slice_bases).  The slice size is forced small here so that
+(tg_batch_config.slice_bases); an input in host memory (tg_count_host, or
+files above the device input budget) is uploaded one slice at a time.  The slice size is forced small here so that
 small inputs span many slices; every result is compared with the oracle,
 bit-exact, in both single-job modes (batched engine, plain sequence)."""
 import random
@@ -84,6 +85,39 @@ def test_sliced_device_resident_input(seng):
     finally:
         seng.dev_free(d_b)
         seng.dev_free(d_o)
+
+
+def test_files_larger_than_the_input_budget(seng, tmp_path):
+    # input files above the device input budget are gathered in host memory
+    # and counted slice by slice (each slice uploaded alone); also gzip
+    import gzip
+    text = bytes(T.synth("kind=reads_fq genome=60000 seed=23 cov=12 rl=150 sub=0.005 nrate=0.001"))
+    rng = random.Random(5)
+    fa = (">a\n" + "".join(rng.choice("ACGT") for _ in range(20000)) + "\n>b\n" +
+          "".join(rng.choice("ACGTN") for _ in range(15000)) + "\n").encode()
+    files = [tmp_path / "r.fq", tmp_path / "g.fa.gz"]
+    files[0].write_bytes(text)
+    files[1].write_bytes(gzip.compress(fa))
+    b1, o1 = O.parse(text)
+    b2, o2 = O.parse(fa)
+    b = np.concatenate([b1, b2])
+    o = np.concatenate([o1, o2[1:] + o1[-1]]).astype(np.uint64)
+    try:
+        for k, m, ci in ((31, 21, 2), (63, 21, 1)):
+            ok, oc, tot, nu = O.count(b, o, k, ci)
+            ok, oc = _sorted(ok, oc)
+            for budget, sl in ((0, 0), (4096, 0), (4096, 20000)):
+                seng.set_input_budget(budget)
+                seng.set_batch_config(slice_bases=sl)
+                gk, gc = seng.count_files([str(f) for f in files], k, m, 16, ci, threads=4)
+                st = seng.last_stats
+                say(f"  files k={k} budget {budget} slice {sl}: {len(gc)} reported")
+                assert (st.n_kmers, st.n_unique, st.n_out) == (tot, nu, len(oc))
+                gk, gc = _sorted(gk, gc)
+                assert np.array_equal(gk, ok) and np.array_equal(gc, oc)
+    finally:
+        seng.set_input_budget(0)
+        seng.set_batch_config(slice_bases=0)
 
 
 def test_slice_config_validation(seng):

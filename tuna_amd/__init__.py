@@ -35,7 +35,7 @@ EXPORTS = [
     "tg_synth_alloc", "tg_server_create", "tg_server_destroy", "tg_server_submit",
     "tg_server_poll", "tg_server_wait", "tg_server_result", "tg_server_release", "tg_server_stats",
     "tg_host_alloc", "tg_host_free", "tg_autotune_many", "tg_set_plain_sequence",
-    "tg_set_debug_stats", "tg_set_wide_deque", "tg_debug_partitions", "tg_debug_min_coverage",
+    "tg_set_debug_stats", "tg_set_wide_deque", "tg_set_input_budget", "tg_debug_partitions", "tg_debug_min_coverage",
     "tg_result_tsv", "tg_result_tsv_noalloc", "tg_result_kff", "tg_result_write", "tg_parse_buffer_mt", "tg_map_file",
     "tg_unmap_file", "tg_parse_file", "tg_count_files",
 ]
@@ -213,6 +213,7 @@ def lib():
     L.tg_set_plain_sequence.argtypes = [_vp, ctypes.c_int]
     L.tg_set_debug_stats.argtypes = [_vp, ctypes.c_int]
     L.tg_set_wide_deque.argtypes = [_vp, ctypes.c_uint32]
+    L.tg_set_input_budget.argtypes = [_vp, ctypes.c_uint64]
     L.tg_debug_partitions.argtypes = [_vp, _u64p, _u64p, _u64p, _u64p, ctypes.c_uint64]
     L.tg_debug_partitions.restype = ctypes.c_int64
     L.tg_debug_min_coverage.argtypes = [_vp, ctypes.POINTER(TgParams), _u8p, ctypes.c_uint64, _u64p,
@@ -506,6 +507,11 @@ class Engine:
         """Deque entries per lane of the k > 64 superkmer pass (1..16); a strip
         that needs more is counted by the pass that keeps the whole window."""
         _check(lib().tg_set_wide_deque(self._h, int(entries)))
+
+    def set_input_budget(self, nbytes=0):
+        """Bytes of input count_files keeps on the device (0: half the free
+        device memory); larger inputs are counted from host memory in slices."""
+        _check(lib().tg_set_input_budget(self._h, int(nbytes)))
 
     def set_debug_stats(self, on=True):
         """Record table statistics per reference partition for jobs counted

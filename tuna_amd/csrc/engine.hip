@@ -11,6 +11,7 @@
 #include <sys/stat.h>
 
 #include <chrono>
+#include <functional>
 #include <unordered_map>
 
 namespace tgi {
@@ -530,22 +531,20 @@ int plan_slices(const uint64_t* off, uint64_t nch, uint64_t limit, std::vector<u
     return TG_OK;
 }
 
+// Counts the slice of chunks [c0, c1) (bases [b0, b1)) with the parameters q
+// (no count filter) into the context's result.
+using SliceFn = std::function<int(uint64_t c0, uint64_t c1, uint64_t b0, uint64_t b1, const tg_params* q,
+                                  tg_stats* ss)>;
+
 // A job of more bases than the slice limit: every slice is counted as a job of
 // its own with no count filter, its pairs are added into the device merge
 // table, and the filtered table becomes the job's result.  Exact: the k-mers
 // of a job are the k-mers of its chunks, and every chunk is in one slice.
-// h_off: the chunk offsets on the host (nullptr: read from d_off).
-int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
-                 uint64_t nch, const uint64_t* h_off, uint64_t limit, tg_stats* out)
+// h_off: the chunk offsets on the host.
+int count_slices(tg_ctx* c, const tg_params* p, uint64_t N, uint64_t nch, const uint64_t* h_off, uint64_t limit,
+                 const SliceFn& count_slice, tg_stats* out)
 {
     hipStream_t st = c->stream;
-    std::vector<uint64_t> own;
-    if (!h_off) {
-        own.resize(nch + 1);
-        HIPOK(hipMemcpyAsync(own.data(), d_off, (nch + 1) * 8ull, hipMemcpyDeviceToHost, st));
-        HIPOK(hipStreamSynchronize(st));
-        h_off = own.data();
-    }
     if (h_off[nch] != N || h_off[0] != 0)
         return fail(TG_EINVAL, "chunk offsets span [%llu, %llu), not the %llu bases of the input",
                     (unsigned long long)h_off[0], (unsigned long long)h_off[nch], (unsigned long long)N);
@@ -578,18 +577,7 @@ int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
     for (size_t i = 0; i + 1 < cuts.size(); ++i) {
         const uint64_t c0 = cuts[i], c1 = cuts[i + 1], b0 = h_off[c0], b1 = h_off[c1];
         tg_stats ss{};
-        if (c->plain) {
-            // the plain sequence takes offsets relative to its bases
-            std::vector<uint64_t> rel(c1 - c0 + 1);
-            for (uint64_t j = c0; j <= c1; ++j) rel[j - c0] = h_off[j] - b0;
-            uint64_t* d_rel;
-            if ((rc = ensure_t(c, "slice_off", rel.size(), &d_rel))) return rc;
-            HIPOK(hipMemcpyAsync(d_rel, rel.data(), rel.size() * 8ull, hipMemcpyHostToDevice, st));
-            rc = count_device(c, &q, d_bases + b0, b1 - b0, d_rel, c1 - c0, &ss);
-        } else {
-            rc = count_range_batched(c, &q, d_bases, N, d_off, nch, c0, c1, b0, b1, &ss);
-        }
-        if (rc) return rc;
+        if ((rc = count_slice(c0, c1, b0, b1, &q, &ss))) return rc;
         const auto t0 = std::chrono::steady_clock::now();
         const uint64_t n = c->n_out;
         uint64_t want = 1ull << 20;
@@ -669,6 +657,57 @@ int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t
     sum.ms_phase2 += ms_merge;
     if (out) *out = sum;
     return TG_OK;
+}
+
+// Slices of a device-resident input (h_off: its chunk offsets on the host, or
+// nullptr to read them from d_off).
+int count_sliced(tg_ctx* c, const tg_params* p, const uint8_t* d_bases, uint64_t N, const uint64_t* d_off,
+                 uint64_t nch, const uint64_t* h_off, uint64_t limit, tg_stats* out)
+{
+    hipStream_t st = c->stream;
+    std::vector<uint64_t> own;
+    if (!h_off) {
+        own.resize(nch + 1);
+        HIPOK(hipMemcpyAsync(own.data(), d_off, (nch + 1) * 8ull, hipMemcpyDeviceToHost, st));
+        HIPOK(hipStreamSynchronize(st));
+        h_off = own.data();
+    }
+    auto slice = [&](uint64_t c0, uint64_t c1, uint64_t b0, uint64_t b1, const tg_params* q, tg_stats* ss) -> int {
+        if (!c->plain) return count_range_batched(c, q, d_bases, N, d_off, nch, c0, c1, b0, b1, ss);
+        // the plain sequence takes offsets relative to its bases
+        std::vector<uint64_t> rel(c1 - c0 + 1);
+        for (uint64_t j = c0; j <= c1; ++j) rel[j - c0] = h_off[j] - b0;
+        uint64_t* d_rel;
+        int rc;
+        if ((rc = ensure_t(c, "slice_off", rel.size(), &d_rel))) return rc;
+        HIPOK(hipMemcpyAsync(d_rel, rel.data(), rel.size() * 8ull, hipMemcpyHostToDevice, st));
+        return count_device(c, q, d_bases + b0, b1 - b0, d_rel, c1 - c0, ss);
+    };
+    return count_slices(c, p, N, nch, h_off, limit, slice, out);
+}
+
+int upload_inputs(tg_ctx* c, const uint8_t* bases, uint64_t N, const uint64_t* off, uint64_t nch,
+                  uint8_t** d_b, uint64_t** d_o);
+
+// Slices of an input in host memory: each slice's bases and chunk offsets are
+// uploaded into device staging and counted there, so the device holds one
+// slice (and the merge table) however large the input is.
+int count_host_sliced(tg_ctx* c, const tg_params* p, const uint8_t* h_bases, uint64_t N, const uint64_t* h_off,
+                      uint64_t nch, uint64_t limit, tg_stats* out)
+{
+    hipStream_t st = c->stream;
+    auto slice = [&](uint64_t c0, uint64_t c1, uint64_t b0, uint64_t b1, const tg_params* q, tg_stats* ss) -> int {
+        std::vector<uint64_t> rel(c1 - c0 + 1);
+        for (uint64_t j = c0; j <= c1; ++j) rel[j - c0] = h_off[j] - b0;
+        uint8_t* d_b;
+        uint64_t* d_o;
+        int rc;
+        if ((rc = upload_inputs(c, h_bases + b0, b1 - b0, rel.data(), c1 - c0, &d_b, &d_o))) return rc;
+        HIPOK(hipStreamSynchronize(st));  // rel leaves scope
+        if (c->plain) return count_device(c, q, d_b, b1 - b0, d_o, c1 - c0, ss);
+        return count_one_batched(c, q, d_b, b1 - b0, d_o, c1 - c0, ss);
+    };
+    return count_slices(c, p, N, nch, h_off, limit, slice, out);
 }
 
 // k > 64: the superkmers are counted (main pass, then the flagged strips with
@@ -1208,6 +1247,44 @@ int64_t tg_debug_partitions(tg_ctx* c, uint64_t* inserted, uint64_t* lds_unique,
     return (int64_t)np;
 }
 
+// A job whose input is in host memory: one that needs slices (k <= 64, more
+// bases than the slice limit) is uploaded slice by slice; any other is
+// uploaded whole.
+int count_host_job(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases, const uint64_t* off,
+                   uint64_t n_chunks, tg_stats* st)
+{
+    const uint64_t limit = slice_limit(c);
+    if (p->k <= 64 && n_bases > limit) {
+        c->dbg_inserted.clear();
+        c->dbg_unique.clear();
+        c->dbg_spilled.clear();
+        c->dbg_cap.clear();
+        const bool dbg = c->debug;
+        c->debug = false;  // per-partition statistics are kept for one-pass jobs only
+        const int rc = count_host_sliced(c, p, bases, n_bases, off, n_chunks, limit, st);
+        c->debug = dbg;
+        return rc;
+    }
+    uint8_t* d_b;
+    uint64_t* d_o;
+    int rc;
+    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
+    return count_job(c, p, d_b, n_bases, d_o, n_chunks, off, st);
+}
+
+// Bytes of input a file count keeps on the device: the set budget, else half
+// the free device memory.
+uint64_t input_budget(tg_ctx* c)
+{
+    if (c->input_budget) return c->input_budget;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return free_b / 2;
+}
+
 int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t n_bases,
                   const uint64_t* off, uint64_t n_chunks, tg_stats* st)
 {
@@ -1215,10 +1292,14 @@ int tg_count_host(tg_ctx* c, const tg_params* p, const uint8_t* bases, uint64_t 
     int rc = check_params(p, n_bases, n_chunks);
     if (rc) return rc;
     HIPOK(hipSetDevice(c->device));
-    uint8_t* d_b;
-    uint64_t* d_o;
-    if ((rc = upload_inputs(c, bases, n_bases, off, n_chunks, &d_b, &d_o))) return rc;
-    return count_job(c, p, d_b, n_bases, d_o, n_chunks, off, st);
+    return count_host_job(c, p, bases, n_bases, off, n_chunks, st);
+}
+
+int tg_set_input_budget(tg_ctx* c, uint64_t bytes)
+{
+    if (!c) return fail(TG_EINVAL, "null context");
+    c->input_budget = bytes;
+    return TG_OK;
 }
 
 // Input files read into device memory as they are parsed: the bases of each
@@ -1241,6 +1322,30 @@ int tg_count_files(tg_ctx* c, const tg_params* p, const char* const* paths, uint
         const std::string f(paths[i] ? paths[i] : "");
         const bool gz = f.size() > 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
         if (stat(f.c_str(), &sb) == 0) guess += (uint64_t)sb.st_size * (gz ? 3u : 1u);
+    }
+    if (p->k <= 64 && guess > input_budget(c)) {
+        // larger than the device keeps: the bases are gathered in host memory
+        // and counted slice by slice
+        std::vector<uint8_t> hb;
+        std::vector<uint64_t> off{0}, fo;
+        hb.reserve(guess);
+        uint64_t base0 = 0;
+        for (uint32_t i = 0; i < n_files; ++i) {
+            auto ready = [&](const uint8_t* q, uint64_t n, uint64_t at, bool) -> int {
+                const uint64_t g = base0 + at;
+                if (hb.size() < g + n) hb.resize(g + n);
+                memcpy(hb.data() + g, q, n);
+                return TG_OK;
+            };
+            uint64_t nb = 0;
+            if ((rc = tgi::parse_file_sections(paths[i], threads, 0, ready, fo, &nb))) return rc;
+            off.reserve(off.size() + fo.size() - 1);
+            for (size_t j = 1; j < fo.size(); ++j) off.push_back(base0 + fo[j]);
+            base0 += nb;
+        }
+        const uint64_t nch = off.size() - 1;
+        if ((rc = check_params(p, base0, nch))) return rc;
+        return count_host_job(c, p, hb.data(), base0, off.data(), nch, st);
     }
     Buf& dst = c->bufs["file_bases"];
     auto room = [&](uint64_t need, uint64_t used) -> int {

@@ -94,6 +94,7 @@ struct tg_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_side = nullptr;
     uint32_t wide_dq = 16;  // deque entries of its superkmer pass (tg_set_wide_deque)
+    uint64_t input_budget = 0;  // bytes of file input kept on the device (0: half the free memory)
     // last job
     uint64_t n_out = 0;
     uint32_t nw = 1;
