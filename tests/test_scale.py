@@ -89,7 +89,8 @@ def test_job_above_2_31_bases():
     """2.5 Gbp of repetitive-genome reads (the 2 Gbp workload's generator at
     25x): more than 2^31 bases, so the job is counted in slices.  With ci = 1
     the counts add up to the k-mers and every distinct k-mer is reported; the
-    multiset is the same with another slice size and in the plain sequence."""
+    multiset is the same with another slice size, in the plain sequence, and
+    counted from host memory (each slice uploaded alone)."""
     bases, off = T.synth_parsed(C3_25X)
     N, nch = len(bases), len(off) - 1
     say(f"2.5 Gbp reads: {N} bases, {nch} chunks")
@@ -97,7 +98,6 @@ def test_job_above_2_31_bases():
     eng = T.Engine(0)
     d_b = eng.dev_upload(bases)
     d_o = eng.dev_upload(off)
-    del bases
     try:
         st = eng.count_device(d_b, N, d_o, nch, 31, 23, 4096, 1)
         ref = summary(eng, 31)
@@ -115,8 +115,17 @@ def test_job_above_2_31_bases():
         assert (st3.n_kmers, st3.n_unique, st3.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
         assert summary(eng, 31) == ref
         say("  plain sequence: same multiset")
-    finally:
+        eng.set_plain_sequence(False)
+        eng.set_batch_config(slice_bases=0)
         eng.dev_free(d_b)
+        d_b = None
+        st4 = eng.count(bases, off, 31, 23, 4096, 1, fetch=False)
+        assert (st4.n_kmers, st4.n_unique, st4.n_superkmers) == (st.n_kmers, st.n_unique, st.n_superkmers)
+        assert summary(eng, 31) == ref
+        say("  host-resident input, slices uploaded one at a time: same multiset")
+    finally:
+        if d_b is not None:
+            eng.dev_free(d_b)
         eng.dev_free(d_o)
         eng.close()
 
