@@ -104,7 +104,8 @@ def alg_bytes_wide(n_bases, n_unique, n_out, k):
 
 def host_threads():
     """Threads for host work: the CPUs this process may run on, at most 16 (a
-    GPU box gives one GPU's job a 16-CPU share whatever the machine shows)."""
+    shared multi-GPU host grants one GPU's job a 16-CPU share, whatever the
+    CPU count it reports)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
