@@ -965,13 +965,15 @@ constexpr uint32_t LOCKED32 = 0xFFFFFFFFu;
 // position in the packed stream (no routing: a key of 4 or 8 words would leave
 // room for few keys in an LDS table).
 
-// Superkmers of k > 64, counted only (the pass over flagged strips; the main
-// pass, k_superkmers_wdq below, keeps a deque instead of the window).  The
-// window holds W <= 256 m-mer hashes per lane in LDS ([position][lane], one
-// wave per workgroup) and keeps their values only: a cut depends on the value
-// of the window minimum, never on its position.  The decision is the general
-// one (look-back, flags, cap) with the cap of the reference's header type
-// (2k - m <= 255: 255 bases, else 65535).
+// Superkmers of k > 64, counted only: the pass over the strips the main pass
+// (k_superkmers_wdq below, which keeps a deque instead of the window) lists,
+// those whose deque overflowed (same look-back) and those whose look-back held
+// no cut (from their chunk start).  The window holds W <= 256 m-mer hashes per
+// lane in LDS ([position][lane], one wave per workgroup) and keeps their values
+// only: a cut depends on the value of the window minimum, never on its
+// position.  The decision is the general one (look-back, flags, cap) with the
+// cap of the reference's header type sk_hdr_t (src/superkmer_io.hpp:45-48:
+// 2k - m <= 255 gives a one-byte length, 255 bases, else 65535).
 __device__ __forceinline__ void wide_backward(uint64_t* sv, uint32_t W, uint32_t lane)
 {
     uint64_t cur = 0;
