@@ -132,3 +132,19 @@ def test_wide_deque_setting_range(eng):
     for bad in (0, 17):
         with pytest.raises(T.TunaGpuError, match="deque entries"):
             eng.set_wide_deque(bad)
+
+
+def test_wide_with_kernel_timing(eng):
+    # with per-kernel timing the superkmer pass runs on the context stream
+    # (one stream), otherwise beside the table insert: the same results
+    rng = random.Random(77)
+    strings = mixed_inputs(rng, 127)
+    eng.set_profiling(True)
+    try:
+        check(eng, strings, 127, 21, ci=2, label="timed")
+        names = set(eng.kernel_times())
+        assert {"superkmers", "count", "drain", "job_gather"} <= names
+    finally:
+        eng.set_profiling(False)
+        eng.reset_kernel_times()
+    check(eng, strings, 127, 21, ci=2, label="untimed")
