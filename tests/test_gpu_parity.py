@@ -84,6 +84,36 @@ def test_pack_matches_kache_codes(eng):
     assert list(w) == exp
 
 
+@pytest.mark.parametrize("plain", [False, True])
+def test_counts_from_misaligned_device_inputs(eng, plain):
+    # an input that starts anywhere inside a device buffer (a batch or a job
+    # inside a concatenation): packing reads it with aligned 16-byte loads
+    # realigned in registers, and the last words base by base
+    text = T.synth("kind=reads_fq genome=20000 seed=41 cov=4 rl=150 sub=0.01 nrate=0.002")
+    b, o = T.parse_buffer(text)
+    pad = np.frombuffer(b"ACGTTGCAACGTTGCAAC", dtype=np.uint8)
+    eng.set_plain_sequence(plain)
+    try:
+        for shift in range(18):
+            n = len(b) - (shift % 7)  # also inputs that end inside a 16-byte word
+            oo = np.minimum(o, np.uint64(n))
+            d_b = eng.dev_upload(np.concatenate([pad[:shift], b, pad[:5]]))
+            d_o = eng.dev_upload(oo)
+            try:
+                say(f"misaligned input: shift {shift}, {n} bases, plain={plain}")
+                eng.count_device(T.ctypes.c_void_p(d_b.value + shift), n, d_o, len(oo) - 1, 31, 21, 16, 1)
+                gk, gc = eng.result(31)
+                ok, oc, _, _ = O.count(b[:n], oo, 31, 1)
+                gi = np.argsort(gk[:, 0], kind="stable")
+                oi = np.argsort(ok[:, 0], kind="stable")
+                assert np.array_equal(gk[gi], ok[oi]) and np.array_equal(gc[gi], oc[oi]), shift
+            finally:
+                eng.dev_free(d_b)
+                eng.dev_free(d_o)
+    finally:
+        eng.set_plain_sequence(False)
+
+
 @pytest.mark.parametrize("name", sorted(GOLD["inputs"]))
 def test_partition_records_match_reference(eng, name):
     text = GOLD["inputs"][name].encode()

@@ -26,42 +26,74 @@ __device__ __forceinline__ uint32_t pack4(uint32_t x)
     return ((t & 3u) << 6) | (((t >> 8) & 3u) << 4) | (((t >> 16) & 3u) << 2) | (t >> 24);
 }
 
-// Word i of the packed stream: bases 16i..16i+15 (zero past n).
-__device__ __forceinline__ uint32_t pack_word(const uint8_t* __restrict__ b, uint64_t n, uint64_t i,
-                                              int aligned)
+__device__ __forceinline__ uint32_t pack16(uint4 v)
+{
+    return (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
+}
+
+// The 16 bytes from byte 4Q + r of the 32 bytes lo:hi (little-endian words).
+template <int Q>
+__device__ __forceinline__ uint4 bytes_at(uint4 lo, uint4 hi, uint32_t r)
+{
+    const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    return make_uint4(__builtin_amdgcn_alignbyte(w[Q + 1], w[Q], r), __builtin_amdgcn_alignbyte(w[Q + 2], w[Q + 1], r),
+                      __builtin_amdgcn_alignbyte(w[Q + 3], w[Q + 2], r), __builtin_amdgcn_alignbyte(w[Q + 4], w[Q + 3], r));
+}
+
+// Word i of the packed stream: bases 16i..16i+15 (zero past n).  mis = the
+// base pointer's offset from 16-byte alignment (the same for every thread): a
+// misaligned input is read as the two aligned 16-byte words that cover the 16
+// bases, while the second lies inside the input (the first lies in the input's
+// aligned block, so in the same page); the last words go base by base.
+template <int Q>
+__device__ __forceinline__ uint32_t pack_word_q(const uint8_t* __restrict__ b, uint64_t n, uint64_t i, uint32_t mis)
 {
     const uint64_t b0 = i * 16u;
-    uint32_t w = 0;
-    if (aligned && b0 + 16u <= n) {
-        const uint4 v = *reinterpret_cast<const uint4*>(b + b0);
-        w = (pack4(v.x) << 24) | (pack4(v.y) << 16) | (pack4(v.z) << 8) | pack4(v.w);
-    } else {
-        for (uint32_t j = 0; j < 16u; ++j)
-            if (b0 + j < n) w |= kache_code(b[b0 + j]) << (30u - 2u * j);
+    if (Q == 0 && mis == 0) {
+        if (b0 + 16u <= n) return pack16(*reinterpret_cast<const uint4*>(b + b0));
+    } else if (b0 + 32u <= n + mis) {
+        const uint4* a = reinterpret_cast<const uint4*>(b - mis) + i;
+        return pack16(bytes_at<Q>(a[0], a[1], mis & 3u));
     }
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < 16u; ++j)
+        if (b0 + j < n) w |= kache_code(b[b0 + j]) << (30u - 2u * j);
     return w;
 }
 
-__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ b, uint64_t n,
-                                              uint32_t* __restrict__ out, uint64_t n_words,
-                                              int aligned)
+template <int Q>
+__device__ __forceinline__ void pack_range(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ out,
+                                           uint64_t n_words, uint32_t mis)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += stride)
-        out[i] = pack_word(b, n, i, aligned);
+        out[i] = pack_word_q<Q>(b, n, i, mis);
+}
+
+__device__ __forceinline__ void pack_any(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ out,
+                                         uint64_t n_words)
+{
+    const uint32_t mis = (uint32_t)((uintptr_t)b & 15u);
+    switch (mis >> 2) {
+    case 0: pack_range<0>(b, n, out, n_words, mis); break;
+    case 1: pack_range<1>(b, n, out, n_words, mis); break;
+    case 2: pack_range<2>(b, n, out, n_words, mis); break;
+    default: pack_range<3>(b, n, out, n_words, mis); break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ b, uint64_t n,
+                                              uint32_t* __restrict__ out, uint64_t n_words)
+{
+    pack_any(b, n, out, n_words);
 }
 
 // Batch variant: bases and their count come from the batch descriptor.
 __global__ __launch_bounds__(256) void k_pack_desc(const BatchDesc* __restrict__ d,
                                                    uint32_t* __restrict__ out, uint64_t cap_words)
 {
-    const uint8_t* b = d->bases;
     const uint64_t n = d->n_bases;
-    const uint64_t nw = min(cap_words, (n + 15) / 16 + 8);
-    const int aligned = ((uintptr_t)b & 15u) == 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nw; i += stride)
-        out[i] = pack_word(b, n, i, aligned);
+    pack_any(d->bases, n, out, min(cap_words, (n + 15) / 16 + 8));
 }
 
 void launch_pack_desc(hipStream_t st, const BatchDesc* d, uint32_t* out, uint64_t cap_words)
@@ -74,11 +106,10 @@ void launch_pack_desc(hipStream_t st, const BatchDesc* d, uint32_t* out, uint64_
 
 void launch_pack(hipStream_t st, const uint8_t* bases, uint64_t n, uint32_t* out, uint64_t n_words)
 {
-    const int aligned = ((uintptr_t)bases & 15u) == 0;
     uint64_t blocks = (n_words + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    k_pack<<<(unsigned)blocks, 256, 0, st>>>(bases, n, out, n_words, aligned);
+    k_pack<<<(unsigned)blocks, 256, 0, st>>>(bases, n, out, n_words);
 }
 
 // ============================================================ strips ====
