@@ -2063,8 +2063,6 @@ void launch_zero_regions(hipStream_t st, const ZeroRegions& z)
 }
 
 // ========================================================== counting ====
-// never in a one-word table: slots hold EMPTY64 or a key below 2^64 - 2^62
-constexpr uint64_t NOMATCH64 = 0xFFFFFFFFFFFFFFFEull;
 constexpr uint32_t SPILL_PROBES = 4096;
 
 // Spill-table view passed by value to the out-of-line insert paths (a
@@ -2427,7 +2425,9 @@ constexpr uint32_t count_stage() { return NW == 1 ? 64u * 3u + 8u : 64u * 7u + 8
 // record's first k-mer (computed once per record) when it crosses into it.
 //
 // Tables: NW == 1, one 64-bit word per key claimed by compare-and-swap, plus a
-// count word for the occurrences after the claiming one.  NW == 2, two key words per slot and the count word as the slot
+// count word for the occurrences after the claiming one; the first probe of a
+// k-mer only reads the slot, and claims go through the deferred pass.  NW == 2,
+// two key words per slot and the count word as the slot
 // state (0 empty, LOCKED32 being written, otherwise the count).  A key probes
 // slot h, h+1, h+3, h+6, ... (triangular steps, which visit every slot of a
 // power-of-two table and break up the clusters of linear probing).  Either
@@ -2590,30 +2590,28 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                     pc = 2u * (s * (16u * PW) + gb + k) + 1u;
                 }
                 __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): settle the set-up reads
-                // A probe result is used one iteration after the probe is issued,
-                // so two probes per wave are in flight.  hit: count; miss: push
-                // the key with its next slot.
+                // A probe is a plain LDS read of the key's slot, used one
+                // iteration after it is issued, so two probes per wave are in
+                // flight.  The slot holds the key: count it; empty: push the key
+                // with that slot, to be claimed by compare-and-swap in the
+                // wave's deferred pass; another key: push the key with its next
+                // slot.  A read is cheaper than a compare-and-swap, and on
+                // repeated k-mers most probes find their key.
                 // WT: the batch holds a periodic record, so a probe stands for a
                 // weight pw of occurrences (the loop without weights is kept for
                 // the common case: pw == 1, no weight bookkeeping)
                 auto settle = [&](auto WT, bool pa, uint64_t px, uint32_t ps, uint64_t po, uint32_t pw) {
-                    const bool hit = pa && (po == EMPTY64 || po == px);
+                    const bool hit = pa && po == px;
                     const bool miss = pa && !hit;
-                    // the occurrence that claimed the slot is counted by the drain
-                    if constexpr (decltype(WT)::value) {
-                        if (hit) {
-                            const uint32_t add = po == px ? pw : pw - 1u;
-                            if (add) atomicAdd(&s_cnt[ps], add);
-                        }
-                    } else {
-                        if (pa && po == px) atomicAdd(&s_cnt[ps], 1u);
-                    }
+                    if (hit) atomicAdd(&s_cnt[ps], decltype(WT)::value ? pw : 1u);
                     const uint64_t mm = __ballot(miss);
                     if (mm) {
                         if (miss) {
+                            // an empty slot is claimed by the deferred pass at the same slot
+                            const bool emp = po == EMPTY64;
                             const uint32_t at = qn + lane_rank(mm);
                             s_qk[wv][at] = px;
-                            s_qs[wv][at] = ((ps + 1u) & (CAP - 1u)) | (1u << DEFER_PSH) |
+                            s_qs[wv][at] = ((ps + (emp ? 0u : 1u)) & (CAP - 1u)) | ((emp ? 0u : 1u) << DEFER_PSH) |
                                            ((decltype(WT)::value ? pw : 1u) << DEFER_WSH);
                         }
                         qn = __builtin_amdgcn_readfirstlane(qn + (uint32_t)__popcll(mm));
@@ -2641,10 +2639,8 @@ __global__ __launch_bounds__(count_block(NW)) void k_count(CountArgs a)
                     slot = slot_of<1, CAP>(0ull, x);
                     if constexpr (decltype(WT)::value) w = (wcur & 0xFFFFu) + (rpos < (wcur >> 16) ? 1u : 0u);
                     else w = 1u;
-                    // every lane probes; a lane without a k-mer compares against
-                    // a value no slot holds and writes nothing
-                    old = atomicCAS((unsigned long long*)&s_k0[slot],
-                                    (unsigned long long)(act ? EMPTY64 : NOMATCH64), (unsigned long long)x);
+                    // every lane reads a slot; a lane without a k-mer ignores it
+                    old = __hip_atomic_load(&s_k0[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     const uint32_t b = __builtin_amdgcn_ubfe(wn, ~pc, 2u);
                     if (act && i + 1u < nmine) {
                         if (xing) {
