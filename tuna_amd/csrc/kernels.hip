@@ -61,13 +61,40 @@ __device__ __forceinline__ uint32_t pack_word_q(const uint8_t* __restrict__ b, u
     return w;
 }
 
+// Four words per thread per trip with their loads issued together, while the
+// four lie where every load is in range (word i reads at most bytes
+// 16i - mis .. 16i - mis + 31 < n when 16i + 32 <= n); the rest word by word.
 template <int Q>
 __device__ __forceinline__ void pack_range(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ out,
                                            uint64_t n_words, uint32_t mis)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n_words; i += stride)
-        out[i] = pack_word_q<Q>(b, n, i, mis);
+    const uint64_t fast_end = min(n_words, n >= 32u ? (n - 32u) / 16u + 1u : 0ull);
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    constexpr bool ALIGNED = Q == 0;
+    if (ALIGNED && mis == 0) {
+        const uint4* a = reinterpret_cast<const uint4*>(b);
+        for (; i + 3u * stride < fast_end; i += 4u * stride) {
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = a[i + u * stride];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) out[i + u * stride] = pack16(v[u]);
+        }
+    } else {
+        const uint4* a = reinterpret_cast<const uint4*>(b - mis);
+        for (; i + 3u * stride < fast_end; i += 4u * stride) {
+            uint4 lo[4], hi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                lo[u] = a[i + u * stride];
+                hi[u] = a[i + u * stride + 1u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) out[i + u * stride] = pack16(bytes_at<Q>(lo[u], hi[u], mis & 3u));
+        }
+    }
+    for (; i < n_words; i += stride) out[i] = pack_word_q<Q>(b, n, i, mis);
 }
 
 __device__ __forceinline__ void pack_any(const uint8_t* __restrict__ b, uint64_t n, uint32_t* __restrict__ out,
