@@ -70,6 +70,7 @@ struct BatchBufs {
     uint32_t* route_meta;                 // meta word of every coarse-ordered record
     uint64_t* split_off;                  // fine scatter's split-group starts
     uint32_t* split_flags;                // coarse buckets it orders by split group
+    uint32_t* fine_hist;                  // group counts per record range of those buckets
     uint32_t* metas;                      // meta word per record slot
     uint32_t *hsk, *hkm, *bucket_job, *item_job;
     uint64_t *part_sk, *part_km, *part_off, *bucket_item;
@@ -222,6 +223,7 @@ int alloc_class(BatchState* s, const BatchClass& c)
         (rc = balloc(s, &b.route_meta, c.rec_cap + 1)) ||
         (rc = balloc(s, &b.split_off, (1ull << tg::HSEL_BITS) + 2)) ||
         (rc = balloc(s, &b.split_flags, 256)) ||
+        (rc = balloc(s, &b.fine_hist, (uint64_t)tg::FINE_RANGES << tg::HSEL_BITS)) ||
         (rc = balloc(s, &b.hsk, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.hkm, (uint64_t)c.max_tiles * c.nsp)) ||
         (rc = balloc(s, &b.part_sk, c.nsp + 1)) || (rc = balloc(s, &b.part_km, c.nsp + 1)) ||
@@ -351,6 +353,7 @@ void enqueue_batch(BatchState* s, hipStream_t st, tg_ctx* prof, int parts)
     r.split_km = b.part_km;
     r.split_scale = &b.desc->scale;
     r.split_flags = b.split_flags;
+    r.fine_hist = b.fine_hist;
     {
         Prof pr(prof, "route_scan");
         tg::launch_u32_to_u64(st, b.run_cnt, b.n_runs, b.run_cnt64);

@@ -1685,7 +1685,8 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_hist(RouteArgs a)
 template <int RW>
 __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t log2f, uint32_t* tmp);
 template <int RW>
-__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp);
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp,
+                                                      uint32_t nr);
 
 // Dynamic-LDS limits of the routing kernels, set once per process at their
 // largest sizes, so no attribute call happens while a launch sequence is
@@ -1874,28 +1875,77 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
     }
 }
 
-// Scatter, level 2: one workgroup per coarse bucket.  Its records are put in
-// groups: the sub-partition, then the top split_bits of the split field (G =
-// F << split_bits <= 512 groups).  A first pass over the region's meta words
-// sizes the groups, so every group is written contiguously at its final place;
-// the second pass reads the region in chunks that fit in LDS (128 KiB), groups
-// each chunk there (histogram, scan, place) and writes the groups out as runs,
-// so consecutive threads write consecutive addresses.  A bucket's records are
-// thus ordered by split group, and a work item that counts 1/2^i of a bucket
-// reads one contiguous range.
+// Group counts of the record ranges of the split-ordered coarse buckets, from
+// their coarse-ordered meta words: workgroup (c, r) counts range r of bucket c
+// (the bucket cut into FINE_RANGES near-equal ranges, as the fine scatter cuts
+// it) into fine_hist[((c << gbits) + g) * FINE_RANGES + r].
+__global__ __launch_bounds__(256) void k_fine_hist(RouteArgs a, uint32_t log2f)
+{
+    constexpr uint32_t R = FINE_RANGES;
+    __shared__ uint32_t hist[FINE_F_MAX];
+    // range-major: consecutive workgroups (spread over the XCDs) take
+    // different buckets
+    const uint32_t nc = a.nsp >> log2f;
+    const uint32_t c = blockIdx.x % nc, r = blockIdx.x / nc, tid = threadIdx.x;
+    if (!a.split_flags[c] || !a.split_bits) return;
+    const uint32_t gbits = log2f + a.split_bits, G = 1u << gbits, gm = G - 1u;
+    const uint32_t shift = REC_HSEL_SHIFT + a.sb - a.split_bits;
+    const uint64_t b = a.part_off[c << log2f], e = a.part_off[(c + 1) << log2f];
+    const uint64_t rs = (e - b + R - 1) / R;
+    const uint64_t lo = min(e, b + r * rs), hi = min(e, lo + rs);
+    for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
+    __syncthreads();
+    constexpr uint32_t U = 8;  // loads in flight per thread
+    for (uint64_t i0 = lo + tid; i0 < hi; i0 += U * 256u) {
+        // unconditional loads (past the end: the range's first word)
+        uint32_t mv[U];
+#pragma unroll
+        for (uint32_t j = 0; j < U; ++j) {
+            const uint64_t i = i0 + (uint64_t)j * 256u;
+            mv[j] = a.tmp_meta[i < hi ? i : lo];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < U; ++j)
+            if (i0 + (uint64_t)j * 256u < hi) atomicAdd(&hist[(mv[j] >> shift) & gm], 1u);
+    }
+    __syncthreads();
+    uint32_t* out = a.fine_hist + ((uint64_t)c << gbits) * R + r;
+    for (uint32_t g = tid; g < G; g += blockDim.x) out[(uint64_t)g * R] = hist[g];
+}
+
+// Scatter, level 2: the records of a coarse bucket are put in groups: the
+// sub-partition, then the top split_bits of the split field (G = F <<
+// split_bits <= 512 groups).  The group sizes are known before any record
+// moves, so every group is written contiguously at its final place: a bucket
+// that is not split-ordered has the sub-partitions' starts; a split-ordered
+// one counts its meta words, either per record range in k_fine_hist (nr > 1:
+// workgroup (c, r) moves range r of bucket c, its group cursors being the
+// group starts plus the group's counts in the ranges before r) or, with one
+// workgroup per bucket (nr == 1), in a first pass of its own.  Records are
+// then read in chunks that fit in LDS (128 KiB), grouped there (histogram,
+// scan, place) and written out as runs, so consecutive threads write
+// consecutive addresses.  A bucket's records are thus ordered by split group,
+// and a work item that counts 1/2^i of a bucket reads one contiguous range.
 template <int RW>
-__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp)
+__global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t log2f, const uint32_t* tmp,
+                                                      uint32_t nr)
 {
     constexpr uint32_t CH = 32768u / RW;  // records per chunk (128 KiB): >= 16 per group
     constexpr uint32_t PER = CH / ROUTE_T;
     __shared__ Rec<RW> s_rec[CH];
     __shared__ uint32_t s_red[ROUTE_T / 64];
     extern __shared__ uint32_t s_dyn[];  // G cursors, G counts, G group starts
-    const uint32_t F = 1u << log2f, c = blockIdx.x, tid = threadIdx.x;
+    // range-major: the first nc workgroups take range 0 of every bucket (a
+    // bucket that is not split-ordered is all range 0), so consecutive
+    // workgroups, which the dispatcher spreads over the XCDs, take different
+    // buckets
+    const uint32_t nc = a.nsp >> log2f;
+    const uint32_t F = 1u << log2f, c = blockIdx.x % nc, r = blockIdx.x / nc, tid = threadIdx.x;
     // split groups only in coarse buckets holding a bucket the work items
     // will split (split_flags); elsewhere the groups are the sub-partitions,
-    // whose starts are known
+    // whose starts are known, and one workgroup moves the bucket
     const uint32_t sbits = a.split_flags && a.split_flags[c] ? a.split_bits : 0u;
+    if (r && !sbits) return;
     const uint32_t gbits = log2f + sbits, G = 1u << gbits;
     uint32_t* cur = s_dyn;
     uint32_t* hist = s_dyn + G;
@@ -1903,8 +1953,34 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
     const uint64_t b = a.part_off[c << log2f], e = a.part_off[(c + 1) << log2f];
     const uint32_t shift = REC_HSEL_SHIFT + a.sb - sbits;
     const uint32_t gm = G - 1u;
+    uint64_t lo = b, hi = e;  // the records this workgroup moves
     if (!sbits) {
         for (uint32_t f = tid; f < F; f += blockDim.x) cur[f] = (uint32_t)a.part_off[(c << log2f) + f];
+    } else if (nr > 1) {
+        const uint64_t rs = (e - b + nr - 1) / nr;
+        lo = min(e, b + r * rs);
+        hi = min(e, lo + rs);
+        const uint32_t* h = a.fine_hist + ((uint64_t)c << gbits) * nr;
+        uint32_t carry = 0;
+        for (uint32_t g0 = 0; g0 < G; g0 += blockDim.x) {
+            const uint32_t g = g0 + tid;
+            uint32_t all = 0, before = 0;
+            if (g < G)
+                for (uint32_t q = 0; q < nr; ++q) {
+                    const uint32_t v = h[(uint64_t)g * nr + q];
+                    all += v;
+                    before += q < r ? v : 0u;
+                }
+            uint32_t tot;
+            const uint32_t ex = block_excl_u32(g < G ? all : 0u, s_red, tot);
+            if (g < G) {
+                cur[g] = (uint32_t)b + carry + ex + before;
+                if (r == 0) a.split_off[((uint64_t)c << gbits) + g] = b + carry + ex;
+            }
+            carry += tot;
+            __syncthreads();
+        }
+        if (r == 0 && tid == 0) a.split_off[(uint64_t)(c + 1) << gbits] = e;
     } else {
         // group sizes of the whole region (from the meta words), then their
         // absolute starts; the end of the region closes its last group
@@ -1940,7 +2016,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
         if (tid == 0) a.split_off[(uint64_t)(c + 1) << gbits] = e;
     }
     __syncthreads();
-    for (uint64_t cb = b; cb < e; cb += CH) {
+    for (uint64_t cb = lo; cb < hi; cb += CH) {
         for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
         __syncthreads();
         // all PER loads back to back (lanes past the end load the chunk's
@@ -1949,13 +2025,13 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
             const uint64_t i = cb + j * ROUTE_T + tid;
-            v[j] = load_rec<RW>(tmp, i < e ? i : cb);
+            v[j] = load_rec<RW>(tmp, i < hi ? i : cb);
         }
         uint32_t fk[PER], pos[PER];
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
             fk[j] = 0xFFFFFFFFu;
-            if (cb + j * ROUTE_T + tid < e) {
+            if (cb + j * ROUTE_T + tid < hi) {
                 fk[j] = (rec_meta<RW>(v[j]) >> shift) & gm;
                 pos[j] = atomicAdd(&hist[fk[j]], 1u);
             }
@@ -1975,7 +2051,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_fine(RouteArgs a, uint32_t lo
         for (uint32_t j = 0; j < PER; ++j)
             if (fk[j] != 0xFFFFFFFFu) s_rec[gst[fk[j]] + pos[j]] = v[j];
         __syncthreads();
-        const uint32_t nch = (uint32_t)min((uint64_t)CH, e - cb);
+        const uint32_t nch = (uint32_t)min((uint64_t)CH, hi - cb);
         for (uint32_t i = tid; i < nch; i += ROUTE_T) {
             const Rec<RW> r = s_rec[i];
             const uint32_t g = (rec_meta<RW>(r) >> shift) & gm;
@@ -2026,16 +2102,16 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     set_kernel_attributes();
     if (a.split_flags) k_route_flags<<<1, 256, 0, st>>>(a, log2f);
     const size_t lc = 0, lf = 3ull * (1u << (log2f + a.split_bits)) * sizeof(uint32_t);
-    if (rw == 2) {
-        k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
-        k_route_fine<2><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
-    } else if (rw == 4) {
-        k_route_coarse<4><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
-        k_route_fine<4><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
-    } else {
-        k_route_coarse<8><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
-        k_route_fine<8><<<nc, ROUTE_T, lf, st>>>(a, log2f, tmp);
-    }
+    // split-ordered buckets are moved in FINE_RANGES ranges when their
+    // group counts per range have somewhere to go
+    const uint32_t nr = a.split_flags && a.fine_hist && a.split_bits ? FINE_RANGES : 1u;
+    if (rw == 2) k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+    else if (rw == 4) k_route_coarse<4><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+    else k_route_coarse<8><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
+    if (nr > 1) k_fine_hist<<<nc * nr, 256, 0, st>>>(a, log2f);
+    if (rw == 2) k_route_fine<2><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
+    else if (rw == 4) k_route_fine<4><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
+    else k_route_fine<8><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
 }
 
 // ============================================================== fill ====

@@ -126,7 +126,16 @@ struct RouteArgs {
     const uint64_t* split_km;
     const float* split_scale;
     uint32_t* split_flags;  // per coarse bucket (256 words): 1 = split-ordered
+    // group counts of the FINE_RANGES record ranges of every split-ordered
+    // coarse bucket, FINE_RANGES << HSEL_BITS words (nullptr: each bucket is
+    // scattered by one workgroup, which sizes its groups itself)
+    uint32_t* fine_hist;
 };
+
+// Record ranges a split-ordered coarse bucket is cut into for the fine
+// scatter, one workgroup each, so that a large bucket does not hold up the
+// pass (the buckets of a skewed input differ in size by 2x and more).
+constexpr uint32_t FINE_RANGES = 4;
 
 // Split-field bits the fine scatter orders records by (at most 512 groups
 // per coarse bucket).
