@@ -1879,7 +1879,7 @@ __global__ __launch_bounds__(ROUTE_T) void k_route_coarse(RouteArgs a, uint32_t 
 // their coarse-ordered meta words: workgroup (c, r) counts range r of bucket c
 // (the bucket cut into FINE_RANGES near-equal ranges, as the fine scatter cuts
 // it) into fine_hist[((c << gbits) + g) * FINE_RANGES + r].
-__global__ __launch_bounds__(256) void k_fine_hist(RouteArgs a, uint32_t log2f)
+__global__ __launch_bounds__(ROUTE_T) void k_fine_hist(RouteArgs a, uint32_t log2f)
 {
     constexpr uint32_t R = FINE_RANGES;
     __shared__ uint32_t hist[FINE_F_MAX];
@@ -1896,17 +1896,17 @@ __global__ __launch_bounds__(256) void k_fine_hist(RouteArgs a, uint32_t log2f)
     for (uint32_t g = tid; g < G; g += blockDim.x) hist[g] = 0;
     __syncthreads();
     constexpr uint32_t U = 8;  // loads in flight per thread
-    for (uint64_t i0 = lo + tid; i0 < hi; i0 += U * 256u) {
+    for (uint64_t i0 = lo + tid; i0 < hi; i0 += U * ROUTE_T) {
         // unconditional loads (past the end: the range's first word)
         uint32_t mv[U];
 #pragma unroll
         for (uint32_t j = 0; j < U; ++j) {
-            const uint64_t i = i0 + (uint64_t)j * 256u;
+            const uint64_t i = i0 + (uint64_t)j * ROUTE_T;
             mv[j] = a.tmp_meta[i < hi ? i : lo];
         }
 #pragma unroll
         for (uint32_t j = 0; j < U; ++j)
-            if (i0 + (uint64_t)j * 256u < hi) atomicAdd(&hist[(mv[j] >> shift) & gm], 1u);
+            if (i0 + (uint64_t)j * ROUTE_T < hi) atomicAdd(&hist[(mv[j] >> shift) & gm], 1u);
     }
     __syncthreads();
     uint32_t* out = a.fine_hist + ((uint64_t)c << gbits) * R + r;
@@ -2108,7 +2108,7 @@ void launch_route_scatter(hipStream_t st, const RouteArgs& a, uint32_t ntiles, u
     if (rw == 2) k_route_coarse<2><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
     else if (rw == 4) k_route_coarse<4><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
     else k_route_coarse<8><<<ntiles, ROUTE_T, lc, st>>>(a, log2f, tmp);
-    if (nr > 1) k_fine_hist<<<nc * nr, 256, 0, st>>>(a, log2f);
+    if (nr > 1) k_fine_hist<<<nc * nr, ROUTE_T, 0, st>>>(a, log2f);
     if (rw == 2) k_route_fine<2><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
     else if (rw == 4) k_route_fine<4><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
     else k_route_fine<8><<<nc * nr, ROUTE_T, lf, st>>>(a, log2f, tmp, nr);
