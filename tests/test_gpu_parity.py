@@ -250,6 +250,24 @@ def test_repeat_reads_c3_settings(ceng):
     assert st.n_parts == 4096
 
 
+def test_split_buckets_in_record_ranges():
+    # 10 Mbp of repetitive reads from a 1 Mbp genome with 2^20 bases per
+    # bucket wanted: 16 sub-partitions of about 10^5 distinct keys each, far
+    # more than one LDS table, so every sub-partition splits into many work
+    # items, every coarse bucket is split-ordered, and the fine scatter moves
+    # each in four record ranges
+    e = T.Engine(0)
+    try:
+        e.set_batch_config(kmers_per_bucket=1 << 20)
+        bases, off = T.synth_parsed("kind=repeat_fq genome=1000000 seed=3 gc=0.45 cov=10 rl=150 sub=0.003 "
+                                    "fam=50 flen=300 div=0.05 rep=0.35 tan=0.10")
+        st = check_counts(e, bases, off, 31, 23, 4096, label="split buckets in record ranges")
+        say(f"  sub-partitions {st.n_subparts}, work items {st.n_lds_groups}")
+        assert st.n_lds_groups > 4 * st.n_subparts
+    finally:
+        e.close()
+
+
 def test_repeated_calls_reuse_buffers(ceng):
     rng = random.Random(9)
     for trial in range(3):
